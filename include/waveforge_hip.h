@@ -1,0 +1,161 @@
+/*
+ * waveforge_hip.h -- C ABI of the MI355X (gfx950) 2D TM FDTD + DAS library
+ * (libwaveforge_hip.so).
+ *
+ * The reference (shahzaibshazoo/waveforge) has no FFI: its boundary is the
+ * Python object protocol of src/core (FDTD2D, MurABC, SourceCollection,
+ * materials as Ca/Cb tensors) plus the DAS functions of its imaging
+ * examples. Each entry point below replaces one eager-tensor sequence of that
+ * protocol; the citation on each names the reference code it stands in for.
+ *
+ * Conventions
+ *   - extern "C", plain pointers and sizes, no C++ types, no exceptions.
+ *   - All pointers inside argument structs are DEVICE pointers unless a field
+ *     says otherwise. The library allocates no device memory: callers pass
+ *     every buffer (fields, Mur strips, tables, traces, workspaces).
+ *   - Work is enqueued on the given stream and is asynchronous.
+ *   - Return 0 on success, a negative WF_ERR_* code on failure; the message
+ *     is available from wf_last_error() (thread-local).
+ *   - Fields are fp32, row-major (i, j) with j fastest: cell = i*ny + j, the
+ *     layout of a contiguous (Nx, Ny, 1) torch tensor (src/core/fields.py:156-168).
+ */
+#ifndef WAVEFORGE_HIP_H
+#define WAVEFORGE_HIP_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WF_ABI_VERSION 1
+
+#define WF_OK 0
+#define WF_ERR_ARG (-1)
+#define WF_ERR_HIP (-2)
+#define WF_ERR_UNSUPPORTED (-3)
+
+/* Field component codes of an injection (src/core/sources.py:49). */
+#define WF_COMP_EX 0
+#define WF_COMP_EY 1
+#define WF_COMP_HZ 2
+
+/* One soft-source contribution: field[cell] += wave[n] * amp.
+ * Contributions that hit the same (comp, cell) form a group and are applied in
+ * array order, which is the SourceCollection order
+ * (src/core/sources.py:1060-1071, point :882-887, line :1031-1035, plane
+ * :715-736 with amp = 1). */
+typedef struct wf_inj2d {
+    const float *wave;   /* waveform table, wave_len entries */
+    int32_t wave_len;
+    int32_t comp;        /* WF_COMP_* */
+    int32_t cell;        /* i*ny + j */
+    float amp;
+} wf_inj2d;
+
+/* One simulation (one TX run / one batch member). The descriptor is read by
+ * every entry point below; `step` is the global index of the next step. */
+typedef struct wf_run2d {
+    float *ex, *ey, *hz;          /* live state, (nx, ny) fp32 each */
+    float *ex2, *ey2, *hz2;       /* second state buffer (temporally blocked path) */
+    const float *ca, *cb;         /* per-cell coefficients; NULL => free space */
+    float *pstrip;                /* 4*(nx+ny) floats: Mur snapshot strips (per-step path) */
+    const wf_inj2d *inj;          /* contributions sorted by group */
+    const int32_t *grp_start;     /* n_grp+1 offsets into inj */
+    const int32_t *det_cell;      /* n_det Hz detector cells (i*ny + j) */
+    float *traces;                /* traces[d*trace_stride + (n - trace_n0)] */
+    int64_t trace_stride;
+    int32_t n_grp, n_inj;
+    int32_t n_det, trace_n0, trace_len;
+    int32_t nx, ny;
+    float dh, de, dx, dy;         /* fp32(dt/MU0), fp32(dt/EPS0), fp32(dx), fp32(dy) */
+    float cmx, cmy;               /* fp32 Mur coefficients (src/core/boundaries.py:177-178) */
+    int32_t step;
+    int32_t flags;                /* reserved, 0 */
+} wf_run2d;
+
+int wf_abi_version(void);
+const char *wf_last_error(void);
+
+/* Advance `n_steps` reference-exact leapfrog steps in place, starting at
+ * run->step, one step per launch group (snapshot+inject, Faraday+Mur,
+ * Ampere+detectors). Replaces one call of FDTD2D.step()
+ * (src/core/fdtd2d.py:187-262) per step: MurABC.snapshot
+ * (boundaries.py:230-237), SourceCollection.step (sources.py:1060-1071),
+ * the Faraday slice update (fdtd2d.py:232-235), MurABC.apply
+ * (boundaries.py:239-274), the Ampere slice updates (fdtd2d.py:244-257), and
+ * the callers' detector gather (examples/brain_mimo_imaging.py:196-198).
+ * `runs` is a HOST array of n_runs descriptors (batch members). The caller
+ * checks waveform lengths (stepping past a table end raises in the
+ * reference, sources.py:882-887). */
+int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps,
+                   hipStream_t stream);
+
+/* max|ex|, max|ey|, max|hz| over n cells each -> out3[0..2] (device).
+ * `partial` is a device workspace of at least 3*n_partial floats; n_partial
+ * is the number of reduction blocks (1..4096). Replaces the three
+ * tensor.abs().max() reductions of FDTD2D._check_stability
+ * (src/core/fdtd2d.py:280-285); NaN propagates as in torch. */
+int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz,
+                     int64_t n, float *partial, int32_t n_partial,
+                     float *out3, hipStream_t stream);
+
+/* Temporally blocked advance of many runs in one launch per block of steps.
+ * Each launch advances every run by `depth` steps (1..WF_TB_MAX_DEPTH):
+ * reads state A (ex/ey/hz) and writes state B (ex2/ey2/hz2) when parity is 0,
+ * the reverse when parity is 1. Tiles of each run are listed in `tiles`
+ * (device, 2 int32 per tile: run index, tile index within the run); tile
+ * geometry comes from tile_i x tile_j and the run's nx, ny. Source, Mur and
+ * detector handling is exactly the per-step semantics above; step indices
+ * are run->step + step_offset + level. If maxabs_out is non-NULL, the launch
+ * also folds max|E|,|H| of its final level into maxabs_out[3*run + c] (as
+ * uint bit patterns, caller zeroes it first). Replaces FDTD2D.run's Python
+ * loop over step() (src/core/fdtd2d.py:328-329) and run_mimo's per-TX loop
+ * (examples/brain_mimo_imaging.py:176-201). */
+#define WF_TB_MAX_DEPTH 8
+typedef struct wf_tb_args {
+    const wf_run2d *runs;      /* device array of n_runs descriptors */
+    const int32_t *tiles;      /* device, 2*n_tiles */
+    int32_t n_runs, n_tiles;
+    int32_t tile_i, tile_j;    /* core tile shape, see wf_tb_tile_shapes */
+    int32_t depth, parity;
+    int32_t step_offset;
+    int32_t has_materials;     /* 1 if every run has ca/cb, 0 if none has */
+    uint32_t *maxabs_out;      /* NULL or device 3*n_runs words */
+} wf_tb_args;
+
+int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
+
+/* Supported (tile_i, tile_j) pairs for wf_tb_args, written into out as
+ * 2*count int32 (host), returns count (<= cap). */
+int wf_tb_tile_shapes(int32_t *out, int32_t cap);
+
+/* Delay-and-sum backprojection of one trace set onto a pixel grid.
+ * Replaces das_backprojection of examples/brain_mimo_imaging.py:211-266 and
+ * its variants (time gate: examples/09_brain_clot_dataset_sample.py:176-178;
+ * other medium speed: examples/10_breast_tumor_mimo.py:148-165; zero instead
+ * of clip outside [0, T): src/datasets/brain/antenna.py:333-336). fp64
+ * geometry and accumulation in the reference's tx-major, rx-minor order,
+ * round-half-even sample index, image = sum^2. */
+typedef struct wf_das2d_args {
+    const float *traces;       /* (n_tx, n_rx, n_t) fp32, contiguous */
+    const double *ant_x;       /* n_ant antenna x positions in metres */
+    const double *ant_y;
+    double *image;             /* (nx_img, ny_img) fp64 output */
+    int32_t n_tx, n_rx, n_t;
+    int32_t nx_img, ny_img;
+    int32_t gate_start;        /* samples with index < gate_start read as 0 */
+    int32_t clip;              /* 1: clip index to [0, n_t-1]; 0: drop outside */
+    int32_t pad;
+    double pitch_x, pitch_y;   /* pixel pitch in metres */
+    double v_medium, dt;
+} wf_das2d_args;
+
+int wf_das2d(const wf_das2d_args *args, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WAVEFORGE_HIP_H */

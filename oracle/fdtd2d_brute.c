@@ -1,0 +1,91 @@
+/* Independent per-cell loop restatement of one 2D TM FDTD step (oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: built by oracle/Makefile and loaded by tests to
+ * pin oracle/fdtd2d_ref.py (the torch restatement). It is written from the
+ * update equations as plain scalar loops, so a slicing mistake in the torch
+ * restatement (an off-by-one range, a swapped operand, a wrong sign) shows up
+ * as a mismatch. Built with -ffp-contract=off so every product and sum is
+ * rounded separately, as eager torch does on the CPU.
+ *
+ * Semantics followed (reference file:line):
+ *   snapshot before injection      src/core/fdtd2d.py:219-224, boundaries.py:237
+ *   Faraday on [0,Nx-1)x[0,Ny-1)   src/core/fdtd2d.py:232-235
+ *   Mur x-edges then y-edges       src/core/boundaries.py:259-274
+ *   Ampere, free / material        src/core/fdtd2d.py:244-257
+ *
+ * Layout: row-major (i, j), index i*Ny + j, one grid (no batch).
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(i, j) ((size_t)(i) * (size_t)Ny + (size_t)(j))
+
+#define DEFINE_STEP(NAME, T)                                                        \
+void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,           \
+          T Dh, T De, T dx, T dy, T cx, T cy,                                       \
+          int n_inj, const int *inj_comp, const int *inj_cell, const T *inj_val)   \
+{                                                                                   \
+    size_t n = (size_t)Nx * (size_t)Ny;                                             \
+    T *P = (T *)malloc(n * sizeof(T));                                              \
+    T *Hn = (T *)malloc(n * sizeof(T));                                             \
+    memcpy(P, Hz, n * sizeof(T));                 /* 1. snapshot */                 \
+    for (int s = 0; s < n_inj; ++s) {             /* 2. soft sources, in order */   \
+        T *f = inj_comp[s] == 0 ? Ex : (inj_comp[s] == 1 ? Ey : Hz);                \
+        f[inj_cell[s]] = f[inj_cell[s]] + inj_val[s];                               \
+    }                                                                               \
+    memcpy(Hn, Hz, n * sizeof(T));                                                  \
+    for (int i = 0; i < Nx - 1; ++i)              /* 3. Faraday */                  \
+        for (int j = 0; j < Ny - 1; ++j) {                                          \
+            T a = (Ex[IDX(i, j + 1)] - Ex[IDX(i, j)]) / dy;                         \
+            T b = (Ey[IDX(i + 1, j)] - Ey[IDX(i, j)]) / dx;                         \
+            T c = a - b;                                                            \
+            T d = Dh * c;                                                           \
+            Hn[IDX(i, j)] = Hz[IDX(i, j)] + d;                                      \
+        }                                                                           \
+    /* 4. Mur: left and right rows first (every j), then bottom/top columns */     \
+    for (int j = 0; j < Ny; ++j) {                                                  \
+        Hn[IDX(0, j)] = P[IDX(1, j)] + cx * (Hn[IDX(1, j)] - P[IDX(0, j)]);         \
+    }                                                                               \
+    for (int j = 0; j < Ny; ++j) {                                                  \
+        Hn[IDX(Nx - 1, j)] = P[IDX(Nx - 2, j)]                                      \
+                            + cx * (Hn[IDX(Nx - 2, j)] - P[IDX(Nx - 1, j)]);        \
+    }                                                                               \
+    for (int i = 0; i < Nx; ++i) {                                                  \
+        Hn[IDX(i, 0)] = P[IDX(i, 1)] + cy * (Hn[IDX(i, 1)] - P[IDX(i, 0)]);         \
+    }                                                                               \
+    for (int i = 0; i < Nx; ++i) {                                                  \
+        Hn[IDX(i, Ny - 1)] = P[IDX(i, Ny - 2)]                                      \
+                            + cy * (Hn[IDX(i, Ny - 2)] - P[IDX(i, Ny - 1)]);        \
+    }                                                                               \
+    memcpy(Hz, Hn, n * sizeof(T));                                                  \
+    /* 5. Ampere from the post-Mur Hz */                                           \
+    for (int i = 0; i < Nx; ++i)                                                    \
+        for (int j = 1; j < Ny; ++j) {                                              \
+            T g = (Hz[IDX(i, j)] - Hz[IDX(i, j - 1)]) / dy;                         \
+            if (Ca) {                                                               \
+                T t1 = Ca[IDX(i, j)] * Ex[IDX(i, j)];                               \
+                T t2 = Cb[IDX(i, j)] * g;                                           \
+                Ex[IDX(i, j)] = t1 + t2;                                            \
+            } else {                                                                \
+                T t2 = De * g;                                                      \
+                Ex[IDX(i, j)] = Ex[IDX(i, j)] + t2;                                 \
+            }                                                                       \
+        }                                                                           \
+    for (int i = 1; i < Nx; ++i)                                                    \
+        for (int j = 0; j < Ny; ++j) {                                              \
+            T g = (Hz[IDX(i, j)] - Hz[IDX(i - 1, j)]) / dx;                         \
+            if (Ca) {                                                               \
+                T t1 = Ca[IDX(i, j)] * Ey[IDX(i, j)];                               \
+                T t2 = Cb[IDX(i, j)] * g;                                           \
+                Ey[IDX(i, j)] = t1 - t2;                                            \
+            } else {                                                                \
+                T t2 = De * g;                                                      \
+                Ey[IDX(i, j)] = Ey[IDX(i, j)] - t2;                                 \
+            }                                                                       \
+        }                                                                           \
+    free(P);                                                                        \
+    free(Hn);                                                                       \
+}
+
+DEFINE_STEP(wf_brute_step_f32, float)
+DEFINE_STEP(wf_brute_step_f64, double)

@@ -1,0 +1,111 @@
+"""Pin the torch oracle with the independent per-cell C stepper.
+
+``oracle/fdtd2d_ref.py`` restates the reference's eager slice operations;
+``oracle/fdtd2d_brute.c`` is written from the update equations as scalar
+loops. They share no code, so an off-by-one slice, a swapped sign or operand,
+or a Mur edge in the wrong order makes them disagree. In fp32 they must agree
+bit for bit (both round every operation once, in the same order); the fp64
+variants must agree too.
+"""
+
+import ctypes
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from cases import build_oracle, case
+
+ORACLE = Path(__file__).resolve().parent.parent / "oracle"
+LIB = ORACLE / "build" / "libwf_brute.so"
+
+
+@pytest.fixture(scope="module")
+def brute():
+    subprocess.run(["make", "-s", "-C", str(ORACLE)], check=True)
+    lib = ctypes.CDLL(str(LIB))
+    for name, ct in (("wf_brute_step_f32", ctypes.c_float), ("wf_brute_step_f64", ctypes.c_double)):
+        fn = getattr(lib, name)
+        P = ctypes.c_void_p
+        fn.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, P, P, ct, ct, ct, ct, ct, ct, ctypes.c_int, P, P, P]
+    return lib
+
+
+def brute_run(lib, o, steps, dtype):
+    """Run the C stepper from a fresh copy of the oracle's initial state."""
+    npdt = np.float32 if dtype == torch.float32 else np.float64
+    nx, ny = o.Nx, o.Ny
+    ex = o.Ex.reshape(nx, ny).numpy().astype(npdt).copy()
+    ey = o.Ey.reshape(nx, ny).numpy().astype(npdt).copy()
+    hz = o.Hz.reshape(nx, ny).numpy().astype(npdt).copy()
+    ca = cb = None
+    if o.Ca is not None:
+        ca = o.Ca.reshape(nx, ny).numpy().astype(npdt).copy()
+        cb = o.Cb.reshape(nx, ny).numpy().astype(npdt).copy()
+    fn = lib.wf_brute_step_f32 if dtype == torch.float32 else lib.wf_brute_step_f64
+    comp_code = {"Ex": 0, "Ey": 1, "Hz": 2}
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+    traces = []
+    for n in range(steps):
+        comps, cells, vals = [], [], []
+        for s in o.sources:
+            if s.kind == "cells":
+                v = (s.wave[n:n + 1] * s.amp).numpy().astype(npdt)
+                for i, j, val in zip(s.i_idx.tolist(), s.j_idx.tolist(), v.tolist()):
+                    comps.append(comp_code[s.component]); cells.append(i * ny + j); vals.append(val)
+            else:
+                w = float(s.wave[n])
+                if s.kind == "plane_xz":
+                    cl = [i * ny + s.position for i in range(nx)]
+                else:
+                    cl = [s.position * ny + j for j in range(ny)]
+                for cc in cl:
+                    comps.append(comp_code[s.component]); cells.append(cc); vals.append(w)
+        comps_a = np.array(comps, dtype=np.int32)
+        cells_a = np.array(cells, dtype=np.int32)
+        vals_a = np.array(vals, dtype=npdt)
+        fn(nx, ny, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb),
+           o.Dh, o.De, o.dx, o.dy, o.cx, o.cy, len(comps), ptr(comps_a), ptr(cells_a), ptr(vals_a))
+        if o.detectors:
+            traces.append([hz[i, j] for i, j in o.detectors])
+    return ex, ey, hz, np.array(traces, dtype=npdt).T if traces else None
+
+
+BRUTE_CASES = [
+    case(4, 4, steps=10, init=True),
+    case(5, 9, steps=20, init=True, material=True, seed=3),
+    case(23, 17, steps=40, material=True, sources=[
+        ("point", "Hz", 11, 8, "dgauss"), ("point", "Hz", 1, 3, "gauss"), ("point", "Ex", 0, 5, "sin"),
+        ("line", "Ey", "x", 12, 2, 15, "gauss", [0.1 * k for k in range(13)]), ("point", "Hz", 11, 8, "sin"),
+        ("plane", "Hz", "yz", 21, "dgauss"), ("point", "Hz", 22, 16, "gauss")],
+        detectors=[(0, 0), (11, 8), (22, 16), (1, 3)]),
+    case(31, 30, steps=60, init=True, sources=[("plane", "Ey", "xz", 4, "sin"), ("point", "Hz", 15, 15, "gauss")]),
+]
+
+
+@pytest.mark.parametrize("k", range(len(BRUTE_CASES)))
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_oracle_matches_brute(brute, k, dtype):
+    c = BRUTE_CASES[k]
+    o = build_oracle(c, dtype=dtype)
+    ex, ey, hz, tr = brute_run(brute, o, c["steps"], dtype)
+    o.run(c["steps"])
+    np.testing.assert_array_equal(o.Ex.reshape(c["nx"], c["ny"]).numpy(), ex)
+    np.testing.assert_array_equal(o.Ey.reshape(c["nx"], c["ny"]).numpy(), ey)
+    np.testing.assert_array_equal(o.Hz.reshape(c["nx"], c["ny"]).numpy(), hz)
+    if tr is not None:
+        np.testing.assert_array_equal(o.trace_array().numpy(), tr)
+    # a non-trivial trajectory: fields moved away from the initial state
+    assert float(o.Hz.abs().max()) > 0.0
+
+
+def test_brute_detects_planted_sign_error(brute):
+    """A flipped Faraday sign in the oracle must be caught by the comparison."""
+    c = BRUTE_CASES[2]
+    o = build_oracle(c)
+    ex, ey, hz, _ = brute_run(brute, o, 5, torch.float32)
+    o.Dh = -o.Dh  # plant the mistake in the oracle (CPU only)
+    o.run(5)
+    assert not np.array_equal(o.Hz.reshape(c["nx"], c["ny"]).numpy(), hz)

@@ -1,0 +1,154 @@
+"""ctypes binding of libwaveforge_hip.so (declared in include/waveforge_hip.h).
+
+The library is the only compute path of this package: if it is missing or
+cannot be loaded, every entry point raises ``RuntimeError`` naming the
+problem. There is no CPU or eager-torch fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libwaveforge_hip.so"
+ABI_VERSION = 1
+
+WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
+TB_MAX_DEPTH = 8
+
+# Every symbol the header declares; tests check the built library exports them.
+EXPORTED_SYMBOLS = (
+    "wf_abi_version",
+    "wf_last_error",
+    "wf_fdtd2d_step",
+    "wf_fdtd2d_maxabs",
+    "wf_fdtd2d_tb_advance",
+    "wf_tb_tile_shapes",
+    "wf_das2d",
+)
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f32 = ctypes.c_float
+_f64 = ctypes.c_double
+
+
+class WfInj2d(ctypes.Structure):
+    """``wf_inj2d``."""
+
+    _fields_ = [
+        ("wave", _vp),
+        ("wave_len", _i32),
+        ("comp", _i32),
+        ("cell", _i32),
+        ("amp", _f32),
+    ]
+
+
+class WfRun2d(ctypes.Structure):
+    """``wf_run2d``."""
+
+    _fields_ = [
+        ("ex", _vp), ("ey", _vp), ("hz", _vp),
+        ("ex2", _vp), ("ey2", _vp), ("hz2", _vp),
+        ("ca", _vp), ("cb", _vp),
+        ("pstrip", _vp),
+        ("inj", _vp), ("grp_start", _vp),
+        ("det_cell", _vp), ("traces", _vp),
+        ("trace_stride", _i64),
+        ("n_grp", _i32), ("n_inj", _i32),
+        ("n_det", _i32), ("trace_n0", _i32), ("trace_len", _i32),
+        ("nx", _i32), ("ny", _i32),
+        ("dh", _f32), ("de", _f32), ("dx", _f32), ("dy", _f32),
+        ("cmx", _f32), ("cmy", _f32),
+        ("step", _i32), ("flags", _i32),
+    ]
+
+
+class WfTbArgs(ctypes.Structure):
+    """``wf_tb_args``."""
+
+    _fields_ = [
+        ("runs", _vp), ("tiles", _vp),
+        ("n_runs", _i32), ("n_tiles", _i32),
+        ("tile_i", _i32), ("tile_j", _i32),
+        ("depth", _i32), ("parity", _i32),
+        ("step_offset", _i32), ("has_materials", _i32),
+        ("maxabs_out", _vp),
+    ]
+
+
+class WfDas2dArgs(ctypes.Structure):
+    """``wf_das2d_args``."""
+
+    _fields_ = [
+        ("traces", _vp), ("ant_x", _vp), ("ant_y", _vp), ("image", _vp),
+        ("n_tx", _i32), ("n_rx", _i32), ("n_t", _i32),
+        ("nx_img", _i32), ("ny_img", _i32),
+        ("gate_start", _i32), ("clip", _i32), ("pad", _i32),
+        ("pitch_x", _f64), ("pitch_y", _f64),
+        ("v_medium", _f64), ("dt", _f64),
+    ]
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+def load(path: Path | None = None) -> ctypes.CDLL:
+    """Load (once) and return the library, with argument types declared."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        p = Path(path) if path is not None else LIB_PATH
+        if not p.exists():
+            raise RuntimeError(
+                f"waveforge_amd: HIP library {p} is not built. Build it with "
+                "`python -m waveforge_amd.build` (hipcc, gfx950); there is no "
+                "CPU fallback."
+            )
+        try:
+            lib = ctypes.CDLL(str(p))
+        except OSError as exc:
+            raise RuntimeError(f"waveforge_amd: cannot load {p}: {exc}") from exc
+        lib.wf_abi_version.restype = ctypes.c_int
+        lib.wf_last_error.restype = ctypes.c_char_p
+        lib.wf_fdtd2d_step.argtypes = [ctypes.POINTER(WfRun2d), _i32, _i32, _vp]
+        lib.wf_fdtd2d_step.restype = ctypes.c_int
+        lib.wf_fdtd2d_maxabs.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp]
+        lib.wf_fdtd2d_maxabs.restype = ctypes.c_int
+        lib.wf_fdtd2d_tb_advance.argtypes = [ctypes.POINTER(WfTbArgs), _vp]
+        lib.wf_fdtd2d_tb_advance.restype = ctypes.c_int
+        lib.wf_tb_tile_shapes.argtypes = [ctypes.POINTER(_i32), _i32]
+        lib.wf_tb_tile_shapes.restype = ctypes.c_int
+        lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]
+        lib.wf_das2d.restype = ctypes.c_int
+        ver = lib.wf_abi_version()
+        if ver != ABI_VERSION:
+            raise RuntimeError(f"waveforge_amd: ABI version {ver} != expected {ABI_VERSION}")
+        _lib = lib
+        return lib
+
+
+def check(rc: int, what: str) -> None:
+    """Raise ``RuntimeError`` with the library's message on a non-zero code."""
+    if rc != 0:
+        msg = load().wf_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def tile_shapes() -> list[tuple[int, int]]:
+    lib = load()
+    buf = (_i32 * 64)()
+    n = lib.wf_tb_tile_shapes(buf, 32)
+    return [(int(buf[2 * k]), int(buf[2 * k + 1])) for k in range(n)]
+
+
+def stream_handle(device) -> int:
+    """Raw ``hipStream_t`` of torch's current stream on ``device``."""
+    import torch
+
+    return int(torch.cuda.current_stream(device).cuda_stream)

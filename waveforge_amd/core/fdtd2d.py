@@ -1,0 +1,348 @@
+"""2D TM FDTD stepper on HIP kernels (drop-in for the reference ``core.fdtd2d``).
+
+Same constructor, attributes, telemetry and errors as ``src/core/fdtd2d.py``
+(``FDTD2D`` :69-423, ``SimulationDivergedError`` :48-62). What changes is
+where the work runs:
+
+* ``step()`` runs one reference step as three HIP launches (snapshot+sources,
+  Faraday+Mur, Ampere+detectors) instead of ~38 eager tensor ops;
+* ``run(n)`` advances in blocks of ``depth`` steps per launch with the
+  temporally blocked LDS kernel, split at the ``n_check`` boundaries so the
+  stability check sees the same fields at the same steps as the reference.
+
+Sources are read from ``self._sources`` at every call (tests reassign it), and
+the field tensors of the ``FieldSet`` are the live state, read and written in
+place, so callers may inspect or modify them between calls. Optional
+``detectors`` (a ``DetectorSet``) record Hz inside the kernels after every
+step, replacing the callers' ``buf[:, step] = Hz[rx_i, rx_j, 0]`` loop.
+"""
+
+from __future__ import annotations
+
+import time
+from typing import Optional
+
+import torch
+
+from .. import _native
+from ..engine import (
+    RunState,
+    SourceTable,
+    StepCoefficients,
+    TBPlan,
+    maxabs,
+    per_step_advance,
+    require_gpu_tensor,
+    tb_limits_ok,
+)
+from .boundaries import MurABC
+from .detectors import DetectorSet
+from .fields import FieldSet
+from .grid import YeeGrid
+
+EPS0: float = 8.8541878128e-12
+MU0: float = 1.2566370614e-6
+
+DEFAULT_TILE = (32, 64)
+DEFAULT_DEPTH = 4
+
+
+class SimulationDivergedError(RuntimeError):
+    """Raised when a field component exceeds the stability threshold."""
+
+    def __init__(self, message: str) -> None:
+        super().__init__(message)
+
+
+class FDTD2D:
+    """2D TM Yee leapfrog stepper ({Ex, Ey, Hz}) running on gfx950 HIP kernels."""
+
+    def __init__(
+        self,
+        grid: YeeGrid,
+        fields: FieldSet,
+        boundary: MurABC,
+        sources=None,
+        *,
+        Ca: Optional[torch.Tensor] = None,
+        Cb: Optional[torch.Tensor] = None,
+        stability_threshold: float = 1e10,
+        n_check: int = 100,
+        detectors: Optional[DetectorSet] = None,
+        depth: Optional[int] = None,
+        tile: Optional[tuple[int, int]] = None,
+    ) -> None:
+        if grid.Nx < 4:
+            raise ValueError(f"grid.Nx must be >= 4 for a valid Yee stencil, got {grid.Nx}")
+        if grid.Ny < 4:
+            raise ValueError(f"grid.Ny must be >= 4 for a valid Yee stencil, got {grid.Ny}")
+        if boundary._Nx != grid.Nx or boundary._Ny != grid.Ny:
+            raise ValueError(
+                f"boundary was built for a grid of shape ({boundary._Nx}, {boundary._Ny}) but grid has shape "
+                f"({grid.Nx}, {grid.Ny}). They must match."
+            )
+        if (Ca is None) != (Cb is None):
+            raise ValueError("Ca and Cb must be provided together or both omitted.")
+        self._grid = grid
+        self._fields = fields
+        self._boundary = boundary
+        self._sources = sources
+        self._Dh: float = grid.dt / MU0
+        self._De: float = grid.dt / EPS0
+        self._dx: float = grid.dx
+        self._dy: float = grid.dy
+        self._Ca = Ca
+        self._Cb = Cb
+        self._has_materials = Ca is not None
+        self._threshold = stability_threshold
+        self._n_check = n_check
+        self._detectors = detectors
+        self._depth = depth
+        self._tile = tile
+        self.steps_completed: int = 0
+        self.elapsed_time: float = 0.0
+        self.mcells_per_second: float = 0.0
+        self.last_field_max: float = -1.0
+        self.n_stability_checks: int = 0
+        self._cache_key = None
+        self._runs: list[RunState] = []
+        self._src_table: Optional[SourceTable] = None
+        self._src_key = None
+        self._second = None
+        self._tb_plan: Optional[TBPlan] = None
+        self._tb_key = None
+
+    # ------------------------------------------------------------------
+    # device state
+    # ------------------------------------------------------------------
+    def _device_check(self) -> None:
+        for name in ("Ex", "Ey", "Hz"):
+            require_gpu_tensor(getattr(self._fields, name), f"fields.{name}")
+        if self._Ca is not None:
+            require_gpu_tensor(self._Ca, "Ca")
+            require_gpu_tensor(self._Cb, "Cb")
+        _native.load()
+
+    def _stream(self) -> int:
+        return _native.stream_handle(self._fields.Hz.device)
+
+    def _source_list(self):
+        if self._sources is None:
+            return []
+        return list(getattr(self._sources, "_sources", self._sources))
+
+    def _source_table(self) -> SourceTable:
+        srcs = self._source_list()
+        key = (id(self._sources), tuple(id(s) for s in srcs))
+        if key != self._src_key:
+            self._src_table = SourceTable(srcs, self._grid.Nx, self._grid.Ny, self._fields.Hz.device)
+            self._src_key = key
+            self._tb_key = None
+        return self._src_table
+
+    def _members(self, t: torch.Tensor) -> list[torch.Tensor]:
+        """Per-batch-member (Nx, Ny) views of a field or coefficient tensor."""
+        Nx, Ny = self._grid.Nx, self._grid.Ny
+        if t.dim() == 4:
+            return [t[b].reshape(Nx, Ny) for b in range(t.shape[0])]
+        return [t.reshape(Nx, Ny)]
+
+    def _build_runs(self) -> list[RunState]:
+        f = self._fields
+        tensors = (f.Ex, f.Ey, f.Hz)
+        for t in tensors:
+            if t.dim() == 4:
+                if not all(t[b].is_contiguous() for b in range(t.shape[0])):
+                    raise ValueError("waveforge_amd: each batch member of a field must be contiguous")
+            elif not t.is_contiguous():
+                raise ValueError("waveforge_amd: field tensors must be contiguous")
+        srcs = self._source_table()
+        det = self._detectors
+        key = (
+            tuple(t.data_ptr() for t in tensors),
+            None if self._Ca is None else (self._Ca.data_ptr(), self._Cb.data_ptr()),
+            self._src_key,
+            id(det),
+        )
+        if key == self._cache_key:
+            return self._runs
+        exs, eys, hzs = (self._members(t) for t in tensors)
+        B = len(hzs)
+        cas = cbs = [None] * B
+        if self._Ca is not None:
+            ca_m = self._members(self._Ca.contiguous())
+            cb_m = self._members(self._Cb.contiguous())
+            cas = ca_m if len(ca_m) == B else ca_m * B
+            cbs = cb_m if len(cb_m) == B else cb_m * B
+        coef = StepCoefficients.from_grid(self._grid.dt, self._grid.dx, self._grid.dy,
+                                          self._boundary.C_mur_x, self._boundary.C_mur_y)
+        Nx, Ny = self._grid.Nx, self._grid.Ny
+        runs = []
+        for b in range(B):
+            rs = RunState(exs[b], eys[b], hzs[b], cas[b], cbs[b], Nx, Ny, coef, sources=srcs)
+            rs.pstrip = torch.empty(4 * (Nx + Ny), dtype=torch.float32, device=hzs[b].device)
+            if det is not None and len(det):
+                rs.det_cell = det.cell_index
+                rs.traces = det.traces if det.traces.dim() == 2 else det.traces[b]
+                rs.trace_n0 = det.start_step
+            runs.append(rs)
+        self._runs = runs
+        self._cache_key = key
+        self._tb_key = None
+        return runs
+
+    # ------------------------------------------------------------------
+    # stepping
+    # ------------------------------------------------------------------
+    def _apply_inert(self, n0: int, n_steps: int) -> None:
+        """Sources on components the 2D TM update never reads (Ez, Hx, Hy)
+        still add into their tensors, as the reference does."""
+        inert = self._src_table.inert if self._src_table is not None else []
+        if not inert:
+            return
+        d = self._fields.as_dict()
+        for n in range(n0, n0 + n_steps):
+            for src in inert:
+                src.step(d, n)
+
+    def _wave_limit(self) -> Optional[int]:
+        t = self._source_table()
+        return t.min_wave_len
+
+    def _advance(self, n_steps: int) -> None:
+        """Advance ``n_steps`` steps starting at ``steps_completed`` (no checks)."""
+        if n_steps <= 0:
+            return
+        self._device_check()
+        runs = self._build_runs()
+        stream = self._stream()
+        inert = self._src_table.inert if self._src_table is not None else []
+        tile = self._tile or DEFAULT_TILE
+        depth = self._depth or DEFAULT_DEPTH
+        use_tb = n_steps >= 2 and not inert and all(tb_limits_ok(r, tile, depth) for r in runs)
+        if use_tb:
+            plan = self._tb_plan_for(runs, tile)
+            plan.advance(n_steps, depth, stream, base_step=self.steps_completed)
+        else:
+            for r in runs:
+                r.step = self.steps_completed
+            per_step_advance(runs, n_steps, stream)
+            self._apply_inert(self.steps_completed, n_steps)
+
+    def _tb_plan_for(self, runs, tile) -> TBPlan:
+        if self._second is None or self._second[0].shape != self._fields.Hz.shape:
+            shape = self._fields.Hz.shape
+            dev = self._fields.Hz.device
+            self._second = tuple(torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3))
+            self._tb_key = None
+        ex2, ey2, hz2 = (self._members(t) for t in self._second)
+        key = (self._cache_key, tile)
+        if key != self._tb_key:
+            for b, r in enumerate(runs):
+                r.second = (ex2[b], ey2[b], hz2[b])
+                r.step = 0  # launches pass the global step as step_offset
+            self._tb_plan = TBPlan(runs, tile, self._fields.Hz.device)
+            self._tb_key = key
+        return self._tb_plan
+
+    def step(self) -> None:
+        """Advance all fields by one time step in place (``src/core/fdtd2d.py:187-266``)."""
+        n = self.steps_completed
+        limit = self._wave_limit()
+        if limit is not None and n >= limit:
+            raise RuntimeError(
+                f"source waveform table has {limit} samples; cannot inject at step {n} "
+                "(build sources with a larger N_steps)"
+            )
+        self._device_check()
+        runs = self._build_runs()
+        for r in runs:
+            r.step = n
+        per_step_advance(runs, 1, self._stream())
+        self._apply_inert(n, 1)
+        self.steps_completed += 1
+        if self.steps_completed % self._n_check == 0:
+            self._check_stability()
+
+    def _check_stability(self) -> None:
+        """Max-abs check of Ex, Ey, Hz (``src/core/fdtd2d.py:268-295``)."""
+        self.n_stability_checks += 1
+        f = self._fields
+        vals = maxabs(f.Ex, f.Ey, f.Hz, self._stream()).cpu().tolist()
+        for name, val in zip(("Ex", "Ey", "Hz"), vals):
+            if val > self.last_field_max:
+                self.last_field_max = val
+            if val > self._threshold:
+                raise SimulationDivergedError(
+                    f"Simulation diverged at step {self.steps_completed}: {name} max = {val:.3e} exceeds "
+                    f"threshold {self._threshold:.3e}. Check CFL condition (dt={self._grid.dt:.3e}) and "
+                    f"source amplitude."
+                )
+
+    def run(self, n_steps: int, *, verbose: bool = False) -> None:
+        """Run ``n_steps`` steps (``src/core/fdtd2d.py:301-359``), same telemetry."""
+        is_cuda = str(self._grid.device).startswith("cuda")
+        report = max(1, n_steps // 10)
+        if is_cuda:
+            torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        start = self.steps_completed
+        done = 0
+        while done < n_steps:
+            chunk = n_steps - done
+            chunk = min(chunk, self._n_check - (self.steps_completed % self._n_check))
+            if verbose:
+                chunk = min(chunk, report - (done % report))
+            limit = self._wave_limit()
+            if limit is not None and self.steps_completed + chunk > limit:
+                chunk = limit - self.steps_completed
+                if chunk <= 0:
+                    raise RuntimeError(
+                        f"source waveform table has {limit} samples; cannot inject at step "
+                        f"{self.steps_completed} (build sources with a larger N_steps)"
+                    )
+            self._advance(chunk)
+            self.steps_completed += chunk
+            done += chunk
+            if self.steps_completed % self._n_check == 0:
+                self._check_stability()
+            if verbose and done % report == 0:
+                if is_cuda:
+                    torch.cuda.synchronize()
+                elapsed = time.perf_counter() - t_start
+                if elapsed > 0.0:
+                    self.mcells_per_second = (
+                        (self.steps_completed - start) * self._grid.Nx * self._grid.Ny / elapsed / 1e6
+                    )
+                self.elapsed_time = elapsed
+                print(
+                    f"  {100 * done // n_steps:3d}%  step {self.steps_completed}  "
+                    f"field_max={self.last_field_max:.3e}  {self.mcells_per_second:.1f} Mcells/s"
+                )
+        if is_cuda:
+            torch.cuda.synchronize()
+        total = time.perf_counter() - t_start
+        self.elapsed_time += total
+        if total > 0.0:
+            self.mcells_per_second = (self.steps_completed - start) * self._grid.Nx * self._grid.Ny / total / 1e6
+
+    def reset(self) -> None:
+        self._fields.zero_()
+        self.steps_completed = 0
+        self.elapsed_time = 0.0
+        self.mcells_per_second = 0.0
+        self.last_field_max = -1.0
+        self.n_stability_checks = 0
+
+    grid = property(lambda self: self._grid)
+    fields = property(lambda self: self._fields)
+    dt = property(lambda self: self._grid.dt)
+    time = property(lambda self: self.steps_completed * self._grid.dt)
+    throughput = property(lambda self: self.mcells_per_second)
+
+    def __repr__(self) -> str:
+        g = self._grid
+        return (
+            f"FDTD2D(shape=({g.Nx}, {g.Ny}, {g.Nz}), steps={self.steps_completed}, time={self.time:.4e} s, "
+            f"throughput={self.mcells_per_second:.2f} Mcells/s, device={g.device})"
+        )

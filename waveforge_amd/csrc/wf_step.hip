@@ -1,0 +1,218 @@
+// Reference-exact per-step path: three launches per leapfrog step, in place.
+//
+//   k_pre  (1 workgroup): Mur snapshot of the rows {0,1,nx-2,nx-1} and columns
+//          {0,1,ny-2,ny-1} of Hz (the only parts of MurABC.Hz_prev that
+//          MurABC.apply reads, src/core/boundaries.py:259-274), then the soft
+//          sources in collection order (src/core/sources.py:1060-1071).
+//   k_h    Faraday on interior cells (src/core/fdtd2d.py:232-235). Each Mur
+//          edge cell is written by the interior thread whose Faraday value it
+//          needs, so the launch is race-free in place: (1,j) writes (0,j),
+//          (nx-2,j) writes (nx-1,j), (i,1) writes (i,0), (i,ny-2) writes
+//          (i,ny-1), and the four corners go to (1,1), (1,ny-2), (nx-2,1),
+//          (nx-2,ny-2). Faraday values on row 0 / column 0 are never needed:
+//          Mur overwrites them.
+//   k_e    Ampere on every cell (src/core/fdtd2d.py:244-257) plus the Hz
+//          detector gather the reference's callers do after step()
+//          (examples/brain_mimo_imaging.py:196-198).
+//
+// All arithmetic is fp32 with one rounding per operation, in the reference's
+// operation order; the library is compiled with -ffp-contract=off so no
+// multiply-add is fused. Division is IEEE division by fp32(dx), as torch's
+// CPU `tensor / python_float` is.
+#include "wf_common.hpp"
+
+namespace {
+
+constexpr int kBX = 64;
+constexpr int kBY = 4;
+
+__device__ __forceinline__ float prow(const wf_run2d &r, int s, int j) {
+    return r.pstrip[s * r.ny + j];  // s: 0 -> row 0, 1 -> row 1, 2 -> row nx-2, 3 -> row nx-1
+}
+__device__ __forceinline__ float pcol(const wf_run2d &r, int s, int i) {
+    return r.pstrip[4 * r.ny + s * r.nx + i];  // s: 0 -> col 0, 1 -> col 1, 2 -> col ny-2, 3 -> col ny-1
+}
+
+__global__ void __launch_bounds__(1024) k_pre(wf_run2d r, int n) {
+    const int nx = r.nx, ny = r.ny;
+    const int tid = threadIdx.x;
+    for (int t = tid; t < 4 * ny; t += blockDim.x) {
+        const int s = t / ny, j = t - s * ny;
+        const int i = s < 2 ? s : nx - 4 + s;
+        r.pstrip[t] = r.hz[(size_t)i * ny + j];
+    }
+    for (int t = tid; t < 4 * nx; t += blockDim.x) {
+        const int s = t / nx, i = t - s * nx;
+        const int j = s < 2 ? s : ny - 4 + s;
+        r.pstrip[4 * ny + t] = r.hz[(size_t)i * ny + j];
+    }
+    __syncthreads();
+    for (int g = tid; g < r.n_grp; g += blockDim.x) {
+        const int a = r.grp_start[g], b = r.grp_start[g + 1];
+        const wf_inj2d e0 = r.inj[a];
+        float *f = e0.comp == WF_COMP_EX ? r.ex : (e0.comp == WF_COMP_EY ? r.ey : r.hz);
+        float v = f[e0.cell];
+        for (int c = a; c < b; ++c) {
+            const float w = r.inj[c].wave[n] * r.inj[c].amp;  // values = w_now * amp
+            v = v + w;                                        // target[idx] += values
+        }
+        f[e0.cell] = v;
+    }
+}
+
+__global__ void __launch_bounds__(kBX * kBY) k_h(wf_run2d r) {
+    const int nx = r.nx, ny = r.ny;
+    const int j = 1 + blockIdx.x * kBX + threadIdx.x;
+    const int i = 1 + blockIdx.y * kBY + threadIdx.y;
+    if (i > nx - 2 || j > ny - 2) return;
+    const size_t c = (size_t)i * ny + j;
+    const float h1 = r.hz[c];
+    const float a = (r.ex[c + 1] - r.ex[c]) / r.dy;
+    const float b = (r.ey[c + ny] - r.ey[c]) / r.dx;
+    const float F = h1 + r.dh * (a - b);
+    r.hz[c] = F;
+    const float cx = r.cmx, cy = r.cmy;
+    if (i == 1) {  // left edge (0, j), then the corners that use it
+        const float L = prow(r, 1, j) + cx * (F - prow(r, 0, j));
+        r.hz[j] = L;
+        if (j == 1) r.hz[0] = prow(r, 0, 1) + cy * (L - prow(r, 0, 0));
+        if (j == ny - 2) r.hz[ny - 1] = prow(r, 0, ny - 2) + cy * (L - prow(r, 0, ny - 1));
+    }
+    if (i == nx - 2) {  // right edge (nx-1, j)
+        const float R = prow(r, 2, j) + cx * (F - prow(r, 3, j));
+        r.hz[(size_t)(nx - 1) * ny + j] = R;
+        if (j == 1) r.hz[(size_t)(nx - 1) * ny] = prow(r, 3, 1) + cy * (R - prow(r, 3, 0));
+        if (j == ny - 2)
+            r.hz[(size_t)(nx - 1) * ny + ny - 1] = prow(r, 3, ny - 2) + cy * (R - prow(r, 3, ny - 1));
+    }
+    if (j == 1) r.hz[(size_t)i * ny] = pcol(r, 1, i) + cy * (F - pcol(r, 0, i));
+    if (j == ny - 2) r.hz[(size_t)i * ny + ny - 1] = pcol(r, 2, i) + cy * (F - pcol(r, 3, i));
+}
+
+template <bool kMat>
+__global__ void __launch_bounds__(kBX * kBY) k_e(wf_run2d r, int n) {
+    const int nx = r.nx, ny = r.ny;
+    const int j = blockIdx.x * kBX + threadIdx.x;
+    const int i = blockIdx.y * kBY + threadIdx.y;
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        const int t = n - r.trace_n0;
+        if (t >= 0 && t < r.trace_len) {
+            for (int d = threadIdx.y * kBX + threadIdx.x; d < r.n_det; d += kBX * kBY)
+                r.traces[d * r.trace_stride + t] = r.hz[r.det_cell[d]];
+        }
+    }
+    if (i >= nx || j >= ny) return;
+    const size_t c = (size_t)i * ny + j;
+    const float h = r.hz[c];
+    if (j >= 1) {
+        const float g = (h - r.hz[c - 1]) / r.dy;
+        if (kMat) {
+            const float t1 = r.ca[c] * r.ex[c];
+            const float t2 = r.cb[c] * g;
+            r.ex[c] = t1 + t2;
+        } else {
+            const float t2 = r.de * g;
+            r.ex[c] = r.ex[c] + t2;
+        }
+    }
+    if (i >= 1) {
+        const float g = (h - r.hz[c - ny]) / r.dx;
+        if (kMat) {
+            const float t1 = r.ca[c] * r.ey[c];
+            const float t2 = r.cb[c] * g;
+            r.ey[c] = t1 - t2;
+        } else {
+            const float t2 = r.de * g;
+            r.ey[c] = r.ey[c] - t2;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_maxabs_partial(const float *__restrict__ ex,
+                                                        const float *__restrict__ ey,
+                                                        const float *__restrict__ hz, int64_t n,
+                                                        unsigned int *__restrict__ partial) {
+    unsigned int m[3] = {0u, 0u, 0u};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        m[0] = max(m[0], wf::abs_bits(ex[k]));
+        m[1] = max(m[1], wf::abs_bits(ey[k]));
+        m[2] = max(m[2], wf::abs_bits(hz[k]));
+    }
+    __shared__ unsigned int sm[3][256];
+    for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = m[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = max(sm[c][threadIdx.x], sm[c][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_maxabs_final(const unsigned int *__restrict__ partial, int np,
+                                                      float *__restrict__ out3) {
+    __shared__ unsigned int sm[3][256];
+    unsigned int m[3] = {0u, 0u, 0u};
+    for (int b = threadIdx.x; b < np; b += blockDim.x)
+        for (int c = 0; c < 3; ++c) m[c] = max(m[c], partial[3 * b + c]);
+    for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = m[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = max(sm[c][threadIdx.x], sm[c][threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) out3[threadIdx.x] = __uint_as_float(sm[threadIdx.x][0]);
+}
+
+int validate_run(const wf_run2d &r) {
+    if (!r.ex || !r.ey || !r.hz) { wf::set_error("run: null field pointer"); return WF_ERR_ARG; }
+    if (r.nx < 4 || r.ny < 4) { wf::set_error("run: grid must be at least 4x4, got %dx%d", r.nx, r.ny); return WF_ERR_ARG; }
+    if ((r.ca == nullptr) != (r.cb == nullptr)) { wf::set_error("run: ca and cb must both be set or both NULL"); return WF_ERR_ARG; }
+    if (r.n_grp < 0 || r.n_det < 0) { wf::set_error("run: negative table size"); return WF_ERR_ARG; }
+    if (r.n_grp > 0 && (!r.inj || !r.grp_start)) { wf::set_error("run: source tables missing"); return WF_ERR_ARG; }
+    if (r.n_det > 0 && (!r.det_cell || !r.traces)) { wf::set_error("run: detector tables missing"); return WF_ERR_ARG; }
+    return WF_OK;
+}
+
+}  // namespace
+
+extern "C" int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream) {
+    if (!runs || n_runs < 1 || n_steps < 0) { wf::set_error("wf_fdtd2d_step: bad arguments"); return WF_ERR_ARG; }
+    for (int b = 0; b < n_runs; ++b) {
+        const int rc = validate_run(runs[b]);
+        if (rc) return rc;
+        if (!runs[b].pstrip) { wf::set_error("wf_fdtd2d_step: pstrip workspace is NULL"); return WF_ERR_ARG; }
+    }
+    for (int s = 0; s < n_steps; ++s) {
+        for (int b = 0; b < n_runs; ++b) {
+            const wf_run2d &r = runs[b];
+            const int n = r.step + s;
+            const int pre_threads = 1024;
+            hipLaunchKernelGGL(k_pre, dim3(1), dim3(pre_threads), 0, stream, r, n);
+            dim3 gh((r.ny - 2 + kBX - 1) / kBX, (r.nx - 2 + kBY - 1) / kBY);
+            hipLaunchKernelGGL(k_h, gh, dim3(kBX, kBY), 0, stream, r);
+            dim3 ge((r.ny + kBX - 1) / kBX, (r.nx + kBY - 1) / kBY);
+            if (r.ca)
+                hipLaunchKernelGGL(k_e<true>, ge, dim3(kBX, kBY), 0, stream, r, n);
+            else
+                hipLaunchKernelGGL(k_e<false>, ge, dim3(kBX, kBY), 0, stream, r, n);
+        }
+        const int rc = wf::check_launch("wf_fdtd2d_step");
+        if (rc) return rc;
+    }
+    return WF_OK;
+}
+
+extern "C" int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz, int64_t n, float *partial,
+                                int32_t n_partial, float *out3, hipStream_t stream) {
+    if (!ex || !ey || !hz || !partial || !out3 || n <= 0 || n_partial < 1 || n_partial > 4096) {
+        wf::set_error("wf_fdtd2d_maxabs: bad arguments");
+        return WF_ERR_ARG;
+    }
+    unsigned int *p = reinterpret_cast<unsigned int *>(partial);
+    hipLaunchKernelGGL(k_maxabs_partial, dim3(n_partial), dim3(256), 0, stream, ex, ey, hz, n, p);
+    hipLaunchKernelGGL(k_maxabs_final, dim3(1), dim3(256), 0, stream, p, n_partial, out3);
+    return wf::check_launch("wf_fdtd2d_maxabs");
+}

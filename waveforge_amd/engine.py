@@ -1,0 +1,324 @@
+"""Host driver of the HIP stepping kernels.
+
+Builds the device tables the C ABI reads (run descriptors, flattened source
+contributions, detector cells, tile lists) from the reference-style objects
+and launches either
+
+* the per-step path (``wf_fdtd2d_step``): three launches per step, in place,
+  used for single steps and as the general path; or
+* the temporally blocked path (``wf_fdtd2d_tb_advance``): ``depth`` steps per
+  launch over LDS tiles, ping-ponging between the live tensors and a second
+  state buffer, with an even number of launches so the result always lands in
+  the live tensors.
+
+Both paths compute the same fp32 values (the kernels share the reference's
+operation order), so the choice is a performance decision only.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native
+from ._native import WfInj2d, WfRun2d, WfTbArgs
+
+EPS0 = 8.8541878128e-12
+MU0 = 1.2566370614e-6
+C0 = 299_792_458.0
+
+INJ_DTYPE = np.dtype(
+    {
+        "names": ["wave", "wave_len", "comp", "cell", "amp"],
+        "formats": ["<u8", "<i4", "<i4", "<i4", "<f4"],
+        "offsets": [0, 8, 12, 16, 20],
+        "itemsize": 24,
+    }
+)
+assert ctypes.sizeof(WfInj2d) == INJ_DTYPE.itemsize
+
+TB_MAX_GROUPS_PER_TILE = 64
+TB_MAX_DETS_PER_TILE = 64
+TB_MAX_GROUPS = 4096
+
+
+def f32(x: float) -> float:
+    """Round a Python float to fp32, as torch does for a scalar operand."""
+    return float(np.float32(x))
+
+
+def require_gpu_tensor(t: torch.Tensor, what: str) -> None:
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+        raise RuntimeError(
+            f"waveforge_amd: {what} must live on the ROCm GPU (device 'cuda'); got "
+            f"{getattr(t, 'device', type(t))}. The HIP kernels are the only compute path."
+        )
+    if t.dtype != torch.float32:
+        raise RuntimeError(f"waveforge_amd: {what} must be float32, got {t.dtype}")
+
+
+@dataclass
+class StepCoefficients:
+    """fp32 scalars of one grid, rounded exactly as the reference's torch ops round them."""
+
+    dh: float
+    de: float
+    dx: float
+    dy: float
+    cmx: float
+    cmy: float
+
+    @classmethod
+    def from_grid(cls, dt: float, dx: float, dy: float, cmx: float | None = None, cmy: float | None = None):
+        if cmx is None:
+            cmx = (C0 * dt - dx) / (C0 * dt + dx)
+        if cmy is None:
+            cmy = (C0 * dt - dy) / (C0 * dt + dy)
+        return cls(f32(dt / MU0), f32(dt / EPS0), f32(dx), f32(dy), f32(cmx), f32(cmy))
+
+
+class SourceTable:
+    """Flattened soft-source contributions of a source list, on the device.
+
+    Contributions are listed in collection order and cell order, then grouped
+    by (component, cell) with a stable sort, so a cell hit by several sources
+    receives them in the reference's order. Components other than Ex/Ey/Hz
+    (inert in 2D TM) are kept aside in ``inert`` for the host to apply.
+    """
+
+    def __init__(self, sources: Sequence, Nx: int, Ny: int, device: torch.device) -> None:
+        from .core.sources import PlaneSource
+
+        comps, cells, amps, waves, lens = [], [], [], [], []
+        self.inert = []
+        self.keepalive = []
+        self.min_wave_len = None
+        for order, src in enumerate(sources):
+            comp = src.component
+            wave = src._waveform_tensor
+            if comp not in _native.WF_COMP:
+                self.inert.append(src)
+                self.min_wave_len = len(wave) if self.min_wave_len is None else min(self.min_wave_len, len(wave))
+                continue
+            require_gpu_tensor(wave, "source waveform")
+            wave = wave.contiguous()
+            self.keepalive.append(wave)
+            if isinstance(src, PlaneSource):
+                i, j, a = src.cells(Nx, Ny)
+            else:
+                i, j, a = src._i_idx.cpu(), src._j_idx.cpu(), src._amp.detach().float().cpu()
+            n = i.numel()
+            comps.append(np.full(n, _native.WF_COMP[comp], dtype=np.int32))
+            cells.append((i.numpy().astype(np.int64) * Ny + j.numpy().astype(np.int64)).astype(np.int32))
+            amps.append(a.numpy().astype(np.float32))
+            waves.append(np.full(n, wave.data_ptr(), dtype=np.uint64))
+            lens.append(np.full(n, wave.numel(), dtype=np.int32))
+            self.min_wave_len = wave.numel() if self.min_wave_len is None else min(self.min_wave_len, wave.numel())
+        if comps:
+            comp = np.concatenate(comps)
+            cell = np.concatenate(cells)
+            key = comp.astype(np.int64) * (int(Nx) * int(Ny)) + cell
+            perm = np.argsort(key, kind="stable")
+            rec = np.zeros(perm.size, dtype=INJ_DTYPE)
+            rec["wave"] = np.concatenate(waves)[perm]
+            rec["wave_len"] = np.concatenate(lens)[perm]
+            rec["comp"] = comp[perm]
+            rec["cell"] = cell[perm]
+            rec["amp"] = np.concatenate(amps)[perm]
+            skey = key[perm]
+            starts = np.flatnonzero(np.r_[True, skey[1:] != skey[:-1]])
+            grp = np.r_[starts, perm.size].astype(np.int32)
+            self.n_inj = int(perm.size)
+            self.n_grp = int(starts.size)
+            self.group_cells = rec["cell"][starts].copy()
+            self.inj = torch.from_numpy(rec.view(np.uint8).copy()).to(device)
+            self.grp_start = torch.from_numpy(grp).to(device)
+        else:
+            self.n_inj = self.n_grp = 0
+            self.group_cells = np.zeros(0, dtype=np.int32)
+            self.inj = None
+            self.grp_start = None
+
+    def ptrs(self) -> tuple[int, int]:
+        return (self.inj.data_ptr() if self.inj is not None else 0,
+                self.grp_start.data_ptr() if self.grp_start is not None else 0)
+
+
+@dataclass
+class RunState:
+    """One simulation's device buffers and scalars, as read by the kernels."""
+
+    ex: torch.Tensor
+    ey: torch.Tensor
+    hz: torch.Tensor
+    ca: Optional[torch.Tensor]
+    cb: Optional[torch.Tensor]
+    nx: int
+    ny: int
+    coef: StepCoefficients
+    sources: Optional[SourceTable] = None
+    det_cell: Optional[torch.Tensor] = None
+    traces: Optional[torch.Tensor] = None
+    trace_n0: int = 0
+    step: int = 0
+    pstrip: Optional[torch.Tensor] = None
+    second: Optional[tuple] = None   # (ex2, ey2, hz2)
+    extra: dict = field(default_factory=dict)
+
+    def struct(self) -> WfRun2d:
+        r = WfRun2d()
+        r.ex, r.ey, r.hz = self.ex.data_ptr(), self.ey.data_ptr(), self.hz.data_ptr()
+        if self.second is not None:
+            r.ex2, r.ey2, r.hz2 = (t.data_ptr() for t in self.second)
+        if self.ca is not None:
+            r.ca, r.cb = self.ca.data_ptr(), self.cb.data_ptr()
+        if self.pstrip is not None:
+            r.pstrip = self.pstrip.data_ptr()
+        if self.sources is not None and self.sources.n_grp:
+            r.inj, r.grp_start = self.sources.ptrs()
+            r.n_grp, r.n_inj = self.sources.n_grp, self.sources.n_inj
+        if self.det_cell is not None and self.det_cell.numel():
+            r.det_cell = self.det_cell.data_ptr()
+            r.traces = self.traces.data_ptr()
+            r.trace_stride = self.traces.stride(-2)
+            r.n_det = self.det_cell.numel()
+            r.trace_n0 = self.trace_n0
+            r.trace_len = self.traces.shape[-1]
+        r.nx, r.ny = self.nx, self.ny
+        c = self.coef
+        r.dh, r.de, r.dx, r.dy, r.cmx, r.cmy = c.dh, c.de, c.dx, c.dy, c.cmx, c.cmy
+        r.step = self.step
+        return r
+
+
+def per_step_advance(runs: Sequence[RunState], n_steps: int, stream: int) -> None:
+    """``n_steps`` reference-exact steps of every run, in place (per-step path)."""
+    if n_steps <= 0:
+        return
+    lib = _native.load()
+    arr = (WfRun2d * len(runs))(*[r.struct() for r in runs])
+    _native.check(lib.wf_fdtd2d_step(arr, len(runs), int(n_steps), ctypes.c_void_p(stream)), "wf_fdtd2d_step")
+
+
+def maxabs(ex: torch.Tensor, ey: torch.Tensor, hz: torch.Tensor, stream: int) -> torch.Tensor:
+    """Device tensor ``[max|ex|, max|ey|, max|hz|]`` via the HIP reduction."""
+    lib = _native.load()
+    n = ex.numel()
+    n_partial = int(max(1, min(1024, (n + 255) // 256)))
+    partial = torch.empty(3 * n_partial, dtype=torch.float32, device=ex.device)
+    out = torch.empty(3, dtype=torch.float32, device=ex.device)
+    _native.check(
+        lib.wf_fdtd2d_maxabs(ex.data_ptr(), ey.data_ptr(), hz.data_ptr(), n, partial.data_ptr(), n_partial,
+                             out.data_ptr(), ctypes.c_void_p(stream)),
+        "wf_fdtd2d_maxabs",
+    )
+    return out
+
+
+def tile_count(nx: int, ny: int, tile: tuple[int, int]) -> int:
+    return ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
+
+
+def tb_limits_ok(run: RunState, tile: tuple[int, int], depth: int) -> bool:
+    """True if every tile region of ``run`` holds at most the per-tile source
+    groups and core detectors the temporally blocked kernel can stage."""
+    TI, TJ = tile
+    nti = (run.nx + TI - 1) // TI
+    ntj = (run.ny + TJ - 1) // TJ
+    if run.sources is not None and run.sources.n_grp:
+        if run.sources.n_grp > TB_MAX_GROUPS or run.sources.inert:
+            return False
+        cells = run.sources.group_cells.astype(np.int64)
+        gi, gj = cells // run.ny, cells % run.ny
+        counts = np.zeros((nti, ntj), dtype=np.int64)
+        lo_i = np.maximum(0, -(-(gi - TI - depth + 1) // TI))
+        hi_i = np.minimum(nti - 1, (gi + depth) // TI)
+        lo_j = np.maximum(0, -(-(gj - TJ - depth + 1) // TJ))
+        hi_j = np.minimum(ntj - 1, (gj + depth) // TJ)
+        for a, b, c, d in zip(lo_i, hi_i, lo_j, hi_j):
+            counts[a:b + 1, c:d + 1] += 1
+        if counts.max() > TB_MAX_GROUPS_PER_TILE:
+            return False
+    if run.det_cell is not None and run.det_cell.numel():
+        cells = run.extra.get("det_cells_np")
+        if cells is None:
+            cells = run.det_cell.cpu().numpy().astype(np.int64)
+            run.extra["det_cells_np"] = cells
+        ti, tj = (cells // run.ny) // TI, (cells % run.ny) // TJ
+        _, cnt = np.unique(ti * ntj + tj, return_counts=True)
+        if cnt.max() > TB_MAX_DETS_PER_TILE:
+            return False
+    return True
+
+
+def split_launches(n_steps: int, depth: int) -> list[int]:
+    """Depths of an even number of launches covering ``n_steps`` (>= 2) steps."""
+    n_launch = -(-n_steps // depth)
+    if n_launch % 2:
+        n_launch += 1
+    base, rem = divmod(n_steps, n_launch)
+    return [base + (1 if k < rem else 0) for k in range(n_launch)]
+
+
+class TBPlan:
+    """Device run table + tile list for the temporally blocked kernel.
+
+    ``TBPlan.event_log``: when set to a list, every launch appends
+    ``(depth, n_cells, start_event, end_event)`` with HIP events recorded on the
+    launch stream around it (used by the benchmark to time the kernel live)."""
+
+    event_log: Optional[list] = None
+
+    def __init__(self, runs: Sequence[RunState], tile: tuple[int, int], device: torch.device) -> None:
+        self.tile = tile
+        self.runs = list(runs)
+        mats = {r.ca is not None for r in runs}
+        if len(mats) != 1:
+            raise ValueError("temporally blocked launch needs all runs with or all without materials")
+        self.has_materials = int(mats.pop())
+        arr = (WfRun2d * len(runs))(*[r.struct() for r in runs])
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+        self.run_table = torch.from_numpy(raw).to(device)
+        tiles = []
+        for k, r in enumerate(runs):
+            nt = tile_count(r.nx, r.ny, tile)
+            tiles.append(np.stack([np.full(nt, k, dtype=np.int32), np.arange(nt, dtype=np.int32)], axis=1))
+        t = np.concatenate(tiles, axis=0)
+        self.n_tiles = int(t.shape[0])
+        self.tile_table = torch.from_numpy(np.ascontiguousarray(t)).to(device)
+
+    def launch(self, depth: int, parity: int, step_offset: int, stream: int,
+               maxabs_out: Optional[torch.Tensor] = None) -> None:
+        a = WfTbArgs()
+        a.runs = self.run_table.data_ptr()
+        a.tiles = self.tile_table.data_ptr()
+        a.n_runs = len(self.runs)
+        a.n_tiles = self.n_tiles
+        a.tile_i, a.tile_j = self.tile
+        a.depth = int(depth)
+        a.parity = int(parity)
+        a.step_offset = int(step_offset)
+        a.has_materials = self.has_materials
+        a.maxabs_out = maxabs_out.data_ptr() if maxabs_out is not None else 0
+        lib = _native.load()
+        log = TBPlan.event_log
+        if log is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+        _native.check(lib.wf_fdtd2d_tb_advance(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_fdtd2d_tb_advance")
+        if log is not None:
+            ev1.record()
+            log.append((int(depth), sum(r.nx * r.ny for r in self.runs), ev0, ev1))
+
+    def advance(self, n_steps: int, depth: int, stream: int, base_step: int = 0) -> None:
+        """Advance every run ``n_steps`` (>= 2) steps, ending in the live buffers.
+        Run ``k`` injects and records step ``table step + base_step + offset``."""
+        offset = 0
+        for k, d in enumerate(split_launches(n_steps, depth)):
+            self.launch(d, k % 2, base_step + offset, stream)
+            offset += d
