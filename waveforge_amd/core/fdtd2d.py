@@ -33,6 +33,7 @@ from ..engine import (
     maxabs,
     per_step_advance,
     require_gpu_tensor,
+    split_launches,
     tb_limits_ok,
 )
 from .boundaries import MurABC
@@ -221,8 +222,17 @@ class FDTD2D:
         depth = self._depth or DEFAULT_DEPTH
         use_tb = n_steps >= 2 and not inert and all(tb_limits_ok(r, tile, depth) for r in runs)
         if use_tb:
-            plan = self._tb_plan_for(runs, tile)
-            plan.advance(n_steps, depth, stream, base_step=self.steps_completed)
+            first = self.steps_completed
+            if split_launches(n_steps, depth) is None:
+                # odd step count at depth 1: one in-place step, then an even split
+                for r in runs:
+                    r.step = first
+                per_step_advance(runs, 1, stream)
+                first += 1
+                n_steps -= 1
+            if n_steps > 0:
+                plan = self._tb_plan_for(runs, tile)
+                plan.advance(n_steps, depth, stream, base_step=first)
         else:
             for r in runs:
                 r.step = self.steps_completed

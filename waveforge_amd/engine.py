@@ -255,10 +255,16 @@ def tb_limits_ok(run: RunState, tile: tuple[int, int], depth: int) -> bool:
     return True
 
 
-def split_launches(n_steps: int, depth: int) -> list[int]:
-    """Depths of an even number of launches covering ``n_steps`` (>= 2) steps."""
+def split_launches(n_steps: int, depth: int) -> Optional[list[int]]:
+    """Depths (each in ``[1, depth]``) of an even number of launches covering
+    ``n_steps`` steps, or ``None`` when no such split exists (odd ``n_steps``
+    with ``depth == 1``, or ``n_steps < 2``)."""
+    if n_steps < 2:
+        return None
     n_launch = -(-n_steps // depth)
     if n_launch % 2:
+        if n_launch == n_steps:
+            return None
         n_launch += 1
     base, rem = divmod(n_steps, n_launch)
     return [base + (1 if k < rem else 0) for k in range(n_launch)]
@@ -318,7 +324,10 @@ class TBPlan:
     def advance(self, n_steps: int, depth: int, stream: int, base_step: int = 0) -> None:
         """Advance every run ``n_steps`` (>= 2) steps, ending in the live buffers.
         Run ``k`` injects and records step ``table step + base_step + offset``."""
+        depths = split_launches(n_steps, depth)
+        if depths is None:
+            raise ValueError(f"cannot split {n_steps} steps into an even number of launches of depth <= {depth}")
         offset = 0
-        for k, d in enumerate(split_launches(n_steps, depth)):
+        for k, d in enumerate(depths):
             self.launch(d, k % 2, base_step + offset, stream)
             offset += d
