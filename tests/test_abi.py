@@ -1,0 +1,99 @@
+"""The C-ABI library: loads without a GPU, exports every declared symbol, and
+its argument structs have the layout the header gives them (no compute)."""
+
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+from waveforge_amd import _native
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "waveforge_hip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(wf_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declarations_match_binding():
+    assert declared_functions() == sorted(_native.EXPORTED_SYMBOLS)
+
+
+def test_library_loads_and_exports_every_symbol():
+    lib = _native.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    assert lib.wf_abi_version() == _native.ABI_VERSION
+    assert isinstance(lib.wf_last_error(), bytes)
+
+
+def test_tile_shapes_listed():
+    shapes = _native.tile_shapes()
+    assert (32, 64) in shapes and all(a > 0 and b > 0 for a, b in shapes)
+
+
+def test_argument_validation_without_gpu():
+    """Bad arguments are rejected on the host before any launch."""
+    lib = _native.load()
+    rc = lib.wf_fdtd2d_step(None, 0, 1, None)
+    assert rc == -1 and b"bad arguments" in lib.wf_last_error()
+    a = _native.WfTbArgs()
+    assert lib.wf_fdtd2d_tb_advance(ctypes.byref(a), None) == -1
+    d = _native.WfDas2dArgs()
+    assert lib.wf_das2d(ctypes.byref(d), None) == -1
+
+
+LAYOUT_C = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "waveforge_hip.h"
+#define O(T, f) printf(#T " " #f " %zu\n", offsetof(T, f));
+int main(void) {
+  printf("wf_inj2d size %zu\nwf_run2d size %zu\nwf_tb_args size %zu\nwf_das2d_args size %zu\n",
+         sizeof(wf_inj2d), sizeof(wf_run2d), sizeof(wf_tb_args), sizeof(wf_das2d_args));
+  O(wf_inj2d, wave) O(wf_inj2d, wave_len) O(wf_inj2d, comp) O(wf_inj2d, cell) O(wf_inj2d, amp)
+  O(wf_run2d, ex) O(wf_run2d, ey) O(wf_run2d, hz) O(wf_run2d, ex2) O(wf_run2d, ey2) O(wf_run2d, hz2)
+  O(wf_run2d, ca) O(wf_run2d, cb) O(wf_run2d, pstrip) O(wf_run2d, inj) O(wf_run2d, grp_start)
+  O(wf_run2d, det_cell) O(wf_run2d, traces) O(wf_run2d, trace_stride) O(wf_run2d, n_grp) O(wf_run2d, n_inj)
+  O(wf_run2d, n_det) O(wf_run2d, trace_n0) O(wf_run2d, trace_len) O(wf_run2d, nx) O(wf_run2d, ny)
+  O(wf_run2d, dh) O(wf_run2d, de) O(wf_run2d, dx) O(wf_run2d, dy) O(wf_run2d, cmx) O(wf_run2d, cmy)
+  O(wf_run2d, step) O(wf_run2d, flags)
+  O(wf_tb_args, runs) O(wf_tb_args, tiles) O(wf_tb_args, n_runs) O(wf_tb_args, n_tiles) O(wf_tb_args, tile_i)
+  O(wf_tb_args, tile_j) O(wf_tb_args, depth) O(wf_tb_args, parity) O(wf_tb_args, step_offset)
+  O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out)
+  O(wf_das2d_args, traces) O(wf_das2d_args, ant_x) O(wf_das2d_args, ant_y) O(wf_das2d_args, image)
+  O(wf_das2d_args, n_tx) O(wf_das2d_args, n_rx) O(wf_das2d_args, n_t) O(wf_das2d_args, nx_img)
+  O(wf_das2d_args, ny_img) O(wf_das2d_args, gate_start) O(wf_das2d_args, clip) O(wf_das2d_args, pad)
+  O(wf_das2d_args, pitch_x) O(wf_das2d_args, pitch_y) O(wf_das2d_args, v_medium) O(wf_das2d_args, dt)
+  return 0;
+}
+"""
+
+
+def test_struct_layout_matches_header(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text(LAYOUT_C)
+    exe = tmp_path / "layout"
+    try:
+        subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", "-I/opt/rocm/include",
+                        str(src), "-o", str(exe)], check=True, capture_output=True)
+    except (FileNotFoundError, subprocess.CalledProcessError) as exc:
+        pytest.skip(f"cannot compile layout probe: {exc}")
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    types = {"wf_inj2d": _native.WfInj2d, "wf_run2d": _native.WfRun2d, "wf_tb_args": _native.WfTbArgs,
+             "wf_das2d_args": _native.WfDas2dArgs}
+    checked = 0
+    for line in out:
+        if not line.strip():
+            continue
+        t, f, v = line.split()
+        if f == "size":
+            assert ctypes.sizeof(types[t]) == int(v), line
+        else:
+            assert getattr(types[t], f).offset == int(v), line
+        checked += 1
+    assert checked > 50
