@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=200)
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--depth", type=int, default=None)
-    p.add_argument("--tile", type=str, default=None, help="TIxTJ, e.g. 32x64")
+    p.add_argument("--tile", type=int, default=None, help="tile rows (tile columns are 64 - 2*depth)")
     p.add_argument("--cpu-steps", type=int, default=8, help="timed steps of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     return p.parse_args()
@@ -101,8 +101,7 @@ def main() -> int:
     if args.depth:
         kw["depth"] = args.depth
     if args.tile:
-        a, b = args.tile.lower().split("x")
-        kw["tile"] = (int(a), int(b))
+        kw["tile"] = args.tile
     total = args.warmup + args.steps
     print(f"[bench] building {args.size}x{args.size} blob medium ({total} steps)", file=sys.stderr, flush=True)
     wl = workloads.blob_medium(dev, n=args.size, n_steps=total, **kw)
@@ -166,7 +165,7 @@ def main() -> int:
                         f"1 GHz modulated Gaussian at centre, Mur, 64 detectors",
             "grid": [args.size, args.size],
             "steps": args.steps,
-            "tile": list(sim._tile or (32, 64)),
+            "tile_rows": sim._tile or 32,
             "depth": sim._depth or 4,
         },
         "roofline": roofline,

@@ -105,31 +105,47 @@ int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz,
  * Each launch advances every run by `depth` steps (1..WF_TB_MAX_DEPTH):
  * reads state A (ex/ey/hz) and writes state B (ex2/ey2/hz2) when parity is 0,
  * the reverse when parity is 1. Tiles of each run are listed in `tiles`
- * (device, 2 int32 per tile: run index, tile index within the run); tile
- * geometry comes from tile_i x tile_j and the run's nx, ny. Source, Mur and
- * detector handling is exactly the per-step semantics above; step indices
- * are run->step + step_offset + level. If maxabs_out is non-NULL, the launch
- * also folds max|E|,|H| of its final level into maxabs_out[3*run + c] (as
- * uint bit patterns, caller zeroes it first). Replaces FDTD2D.run's Python
- * loop over step() (src/core/fdtd2d.py:328-329) and run_mimo's per-TX loop
- * (examples/brain_mimo_imaging.py:176-201). */
+ * (device, 2 int32 per tile: run index, tile index within the run); a tile is
+ * tile_i rows x tile_j columns with tile_j = 64 - 2*depth, numbered row-major
+ * over ceil(nx/tile_i) x ceil(ny/tile_j). Source, Mur and detector handling is
+ * exactly the per-step semantics above; step indices are run->step +
+ * step_offset + level. If maxabs_out is non-NULL, the launch also folds
+ * max|E|,|H| of its final level into maxabs_out[3*run + c] (as uint bit
+ * patterns; the caller zeroes it first). fast_div selects the corrected
+ * reciprocal division, which callers enable only for divisors that
+ * wf_div_verify proved exact on [div_lo, div_lo + div_span).
+ * Replaces FDTD2D.run's Python loop over step() (src/core/fdtd2d.py:328-329)
+ * and run_mimo's per-TX loop (examples/brain_mimo_imaging.py:176-201). */
 #define WF_TB_MAX_DEPTH 8
 typedef struct wf_tb_args {
     const wf_run2d *runs;      /* device array of n_runs descriptors */
     const int32_t *tiles;      /* device, 2*n_tiles */
     int32_t n_runs, n_tiles;
-    int32_t tile_i, tile_j;    /* core tile shape, see wf_tb_tile_shapes */
+    int32_t tile_i, tile_j;    /* core tile shape: tile_i from wf_tb_tile_shapes, tile_j = 64 - 2*depth */
     int32_t depth, parity;
     int32_t step_offset;
     int32_t has_materials;     /* 1 if every run has ca/cb, 0 if none has */
     uint32_t *maxabs_out;      /* NULL or device 3*n_runs words */
+    float rdx, rdy;            /* fp32 reciprocals 1/dx, 1/dy (correctly rounded) */
+    uint32_t div_lo_x, div_span_x;
+    uint32_t div_lo_y, div_span_y;
+    int32_t fast_div;          /* 1: corrected reciprocal division (all runs share dx, dy) */
+    int32_t pad;
 } wf_tb_args;
 
 int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
 
-/* Supported (tile_i, tile_j) pairs for wf_tb_args, written into out as
- * 2*count int32 (host), returns count (<= cap). */
+/* Supported tile_i values for wf_tb_args, written into out as (tile_i, 0)
+ * pairs (host, 2*count int32); returns count (<= cap). */
 int wf_tb_tile_shapes(int32_t *out, int32_t cap);
+
+/* Exhaustive device check that the corrected-reciprocal quotient
+ * q0 = x*r, q = fma(fma(-q0, d, x), r, q0) equals IEEE x/d bit for bit for
+ * every float x with lo <= bits(|x|) < lo + span, both signs, and x = +0.
+ * Adds the number of mismatches to *bad (device uint32, caller zeroes).
+ * No reference counterpart: it licenses the fast division of
+ * wf_fdtd2d_tb_advance, which replaces the reference's tensor / dx. */
+int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, hipStream_t stream);
 
 /* Delay-and-sum backprojection of one trace set onto a pixel grid.
  * Replaces das_backprojection of examples/brain_mimo_imaging.py:211-266 and

@@ -106,7 +106,7 @@ def build_oracle(c, dtype=torch.float32):
     return o
 
 
-def build_product(c, device, *, depth=None, tile=None, n_det_steps=None):
+def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_div=True):
     from waveforge_amd import core
     from waveforge_amd.core.sources import Waveform
 
@@ -157,7 +157,7 @@ def build_product(c, device, *, depth=None, tile=None, n_det_steps=None):
     if ca is not None:
         kw = dict(Ca=ca.to(device), Cb=cb.to(device))
     sim = core.FDTD2D(grid, fields, boundary, coll, stability_threshold=c["threshold"], n_check=c["n_check"],
-                      detectors=det, depth=depth, tile=tile, **kw)
+                      detectors=det, depth=depth, tile=tile, fast_div=fast_div, **kw)
     return sim, det
 
 

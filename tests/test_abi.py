@@ -32,8 +32,9 @@ def test_library_loads_and_exports_every_symbol():
 
 
 def test_tile_shapes_listed():
-    shapes = _native.tile_shapes()
-    assert (32, 64) in shapes and all(a > 0 and b > 0 for a, b in shapes)
+    rows = _native.tile_rows()
+    assert 32 in rows and all(r > 0 for r in rows)
+    assert [_native.tile_cols(d) for d in (1, 4, 8)] == [62, 56, 48]
 
 
 def test_argument_validation_without_gpu():
@@ -64,7 +65,9 @@ int main(void) {
   O(wf_run2d, step) O(wf_run2d, flags)
   O(wf_tb_args, runs) O(wf_tb_args, tiles) O(wf_tb_args, n_runs) O(wf_tb_args, n_tiles) O(wf_tb_args, tile_i)
   O(wf_tb_args, tile_j) O(wf_tb_args, depth) O(wf_tb_args, parity) O(wf_tb_args, step_offset)
-  O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out)
+  O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out) O(wf_tb_args, rdx) O(wf_tb_args, rdy)
+  O(wf_tb_args, div_lo_x) O(wf_tb_args, div_span_x) O(wf_tb_args, div_lo_y) O(wf_tb_args, div_span_y)
+  O(wf_tb_args, fast_div) O(wf_tb_args, pad)
   O(wf_das2d_args, traces) O(wf_das2d_args, ant_x) O(wf_das2d_args, ant_y) O(wf_das2d_args, image)
   O(wf_das2d_args, n_tx) O(wf_das2d_args, n_rx) O(wf_das2d_args, n_t) O(wf_das2d_args, nx_img)
   O(wf_das2d_args, ny_img) O(wf_das2d_args, gate_start) O(wf_das2d_args, clip) O(wf_das2d_args, pad)

@@ -70,14 +70,15 @@ def test_per_step_path_bitwise(device, name):
     _compare(sim, det, o, name)
 
 
-@pytest.mark.parametrize("tile", [(32, 64), (16, 64), (64, 64), (32, 32)])
-@pytest.mark.parametrize("depth", [1, 2, 3, 4, 8])
+@pytest.mark.parametrize("fast_div", [True, False])
+@pytest.mark.parametrize("tile", [32, 16, 48])
+@pytest.mark.parametrize("depth", [1, 2, 3, 4, 5, 8])
 @pytest.mark.parametrize("name", ["tiny5x7_mat", "mat_all_sources", "edge_sources", "odd_tiles_mat", "batch3",
                                   "free_point_hz"])
-def test_tb_path_bitwise(device, name, tile, depth):
+def test_tb_path_bitwise(device, name, tile, depth, fast_div):
     c = CASES[name]
-    print(f"[tb] case={name} tile={tile} depth={depth}", flush=True)
-    sim, det = build_product(c, device, depth=depth, tile=tile)
+    print(f"[tb] case={name} tile_rows={tile} depth={depth} fast_div={fast_div}", flush=True)
+    sim, det = build_product(c, device, depth=depth, tile=tile, fast_div=fast_div)
     o = build_oracle(c)
     sim.run(c["steps"])
     o.run(c["steps"])

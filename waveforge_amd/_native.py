@@ -25,6 +25,7 @@ EXPORTED_SYMBOLS = (
     "wf_fdtd2d_maxabs",
     "wf_fdtd2d_tb_advance",
     "wf_tb_tile_shapes",
+    "wf_div_verify",
     "wf_das2d",
 )
 
@@ -77,6 +78,10 @@ class WfTbArgs(ctypes.Structure):
         ("depth", _i32), ("parity", _i32),
         ("step_offset", _i32), ("has_materials", _i32),
         ("maxabs_out", _vp),
+        ("rdx", _f32), ("rdy", _f32),
+        ("div_lo_x", ctypes.c_uint32), ("div_span_x", ctypes.c_uint32),
+        ("div_lo_y", ctypes.c_uint32), ("div_span_y", ctypes.c_uint32),
+        ("fast_div", _i32), ("pad", _i32),
     ]
 
 
@@ -124,6 +129,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_fdtd2d_tb_advance.restype = ctypes.c_int
         lib.wf_tb_tile_shapes.argtypes = [ctypes.POINTER(_i32), _i32]
         lib.wf_tb_tile_shapes.restype = ctypes.c_int
+        lib.wf_div_verify.argtypes = [_f32, _f32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]
+        lib.wf_div_verify.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]
         lib.wf_das2d.restype = ctypes.c_int
         ver = lib.wf_abi_version()
@@ -140,11 +147,17 @@ def check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed ({rc}): {msg}")
 
 
-def tile_shapes() -> list[tuple[int, int]]:
+def tile_rows() -> list[int]:
+    """Tile heights (tile_i) compiled into the temporally blocked kernel."""
     lib = load()
     buf = (_i32 * 64)()
     n = lib.wf_tb_tile_shapes(buf, 32)
-    return [(int(buf[2 * k]), int(buf[2 * k + 1])) for k in range(n)]
+    return [int(buf[2 * k]) for k in range(n)]
+
+
+def tile_cols(depth: int) -> int:
+    """Tile width for a launch of ``depth`` steps: one 64-lane wavefront per row minus the halo."""
+    return 64 - 2 * depth
 
 
 def stream_handle(device) -> int:

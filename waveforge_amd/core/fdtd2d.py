@@ -44,7 +44,7 @@ from .grid import YeeGrid
 EPS0: float = 8.8541878128e-12
 MU0: float = 1.2566370614e-6
 
-DEFAULT_TILE = (32, 64)
+DEFAULT_TILE_ROWS = 32
 DEFAULT_DEPTH = 4
 
 
@@ -71,7 +71,8 @@ class FDTD2D:
         n_check: int = 100,
         detectors: Optional[DetectorSet] = None,
         depth: Optional[int] = None,
-        tile: Optional[tuple[int, int]] = None,
+        tile: Optional[int] = None,
+        fast_div: bool = True,
     ) -> None:
         if grid.Nx < 4:
             raise ValueError(f"grid.Nx must be >= 4 for a valid Yee stencil, got {grid.Nx}")
@@ -100,6 +101,7 @@ class FDTD2D:
         self._detectors = detectors
         self._depth = depth
         self._tile = tile
+        self._fast_div = fast_div
         self.steps_completed: int = 0
         self.elapsed_time: float = 0.0
         self.mcells_per_second: float = 0.0
@@ -218,9 +220,10 @@ class FDTD2D:
         runs = self._build_runs()
         stream = self._stream()
         inert = self._src_table.inert if self._src_table is not None else []
-        tile = self._tile or DEFAULT_TILE
+        tile = self._tile or DEFAULT_TILE_ROWS
         depth = self._depth or DEFAULT_DEPTH
-        use_tb = n_steps >= 2 and not inert and all(tb_limits_ok(r, tile, depth) for r in runs)
+        use_tb = n_steps >= 2 and not inert and all(
+            tb_limits_ok(r, tile, d) for r in runs for d in range(1, depth + 1))
         if use_tb:
             first = self.steps_completed
             if split_launches(n_steps, depth) is None:
@@ -246,12 +249,12 @@ class FDTD2D:
             self._second = tuple(torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3))
             self._tb_key = None
         ex2, ey2, hz2 = (self._members(t) for t in self._second)
-        key = (self._cache_key, tile)
+        key = (self._cache_key, tile, self._fast_div)
         if key != self._tb_key:
             for b, r in enumerate(runs):
                 r.second = (ex2[b], ey2[b], hz2[b])
                 r.step = 0  # launches pass the global step as step_offset
-            self._tb_plan = TBPlan(runs, tile, self._fields.Hz.device)
+            self._tb_plan = TBPlan(runs, tile, self._fields.Hz.device, fast_div=self._fast_div)
             self._tb_key = key
         return self._tb_plan
 

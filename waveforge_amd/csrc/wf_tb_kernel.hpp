@@ -1,0 +1,401 @@
+// Temporally blocked, multi-run 2D TM FDTD advance (the throughput path).
+//
+// Work decomposition. One workgroup (4 wavefronts) owns one core tile of one
+// run: TI rows x TJ = 64 - 2D columns, plus a halo of D cells per side, so
+// every row of the staged region is exactly 64 cells = one wavefront. Wave w
+// holds rows w, w+4, w+8, ... of the region in registers (Ex, Ey, Hz and, on
+// the material path, Ca/Cb), one column per lane. The workgroup advances the
+// region D leapfrog steps without touching HBM, shrinking the valid region by
+// one cell per side per step, then writes the core to the second state buffer
+// (ping-pong: no workgroup reads what another writes in the same launch).
+// HBM traffic is ~(20*R + 12)/D bytes per cell-update (R = region/core area)
+// instead of the 32 B of one step per pass.
+//
+// Neighbours: the j-neighbours of a cell live in the adjacent lane of the same
+// wavefront and are fetched with DPP whole-wave shifts; the i-neighbours live
+// in another wavefront and go through LDS, which holds only the Hz and Ey rows
+// (Faraday reads Ey(i+1, j), Ampere reads Hz(i-1, j)).
+//
+// Each step reproduces the reference semantics exactly, in the same fp32
+// operation order (the library is built with -ffp-contract=off):
+//   1. soft sources inside the region, in collection order, added to the
+//      owning lane's registers (src/core/sources.py:1060-1071); LDS keeps the
+//      pre-injection Hz, which is the Mur snapshot (src/core/boundaries.py:237);
+//   2. Faraday on interior cells (src/core/fdtd2d.py:232-235); Mur edge cells
+//      (src/core/boundaries.py:259-274) are written to LDS by the interior cell
+//      whose Faraday value they need; edge lanes reload them after the barrier;
+//   3. Ampere (src/core/fdtd2d.py:244-257); Hz detectors owned by the core are
+//      read from LDS (examples/brain_mimo_imaging.py:196-198).
+//
+// Division by fp32(dx) / fp32(dy): IEEE division, or (FAST) the reciprocal
+// product corrected by one fused residual step, q1 = q0 + (x - q0*d)*r, which
+// the library proves equal to IEEE division for every |x| in [lo, lo+span)
+// by an exhaustive device check per divisor (wf_div_verify); zero uses the
+// fast form too (it is exact for +0), other |x| fall back to IEEE division.
+//
+// Tiles are mapped XCD-aware: the dispatcher deals workgroups round-robin to
+// the 8 XCDs, so block b runs on XCD b%8; we hand each XCD a contiguous band
+// of tiles so neighbouring halos are read through the same L2.
+#pragma once
+#include "wf_common.hpp"
+
+namespace wf_tb {
+
+constexpr int kNT = 256;        // threads per workgroup (4 wavefronts)
+constexpr int kW = 64;          // region width = wavefront size
+constexpr int kMaxGroups = 64;  // source groups per tile region (host checks)
+constexpr int kMaxDets = 64;    // detectors per tile core (host checks)
+
+__device__ __forceinline__ float lane_next(float v) {  // value of lane+1 (0 in lane 63)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_prev(float v) {  // value of lane-1 (0 in lane 0)
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+
+struct Divider {
+    float d, r;
+    unsigned int lo, span;
+};
+
+// x / d. FAST: corrected reciprocal product for every lane; if any lane of
+// the wavefront holds an x outside the verified range (or -0), the whole
+// wavefront takes the IEEE division and those lanes keep its result. The
+// ballot makes the fallback a uniform (scalar) branch.
+template <bool FAST>
+__device__ __forceinline__ float divide(float x, const Divider &v) {
+    if (!FAST) return x / v.d;
+    const float q0 = x * v.r;
+    const float e = __builtin_fmaf(-q0, v.d, x);
+    float q = __builtin_fmaf(e, v.r, q0);
+    const unsigned int bx = __float_as_uint(x);
+    const bool slow = ((bx & 0x7fffffffu) - v.lo >= v.span) && bx != 0u;
+    if (__builtin_expect(__ballot(slow) != 0ull, 0)) {
+        const float qs = x / v.d;
+        q = slow ? qs : q;
+    }
+    return q;
+}
+
+// Rows per lane and the occupancy the register budget is sized for: each lane
+// keeps 5 values per row (Ex, Ey, Hz, Ca, Cb) plus ~64 registers of
+// temporaries; without the cap the scheduler hoists every row's loads and
+// runs at one or two waves per SIMD.
+template <int TI, int D>
+struct TileGeom {
+    static constexpr int NR = (TI + 2 * D + 3) / 4;
+    static constexpr int kRegs = 5 * NR + 64;
+    static constexpr int kWaves = 512 / kRegs < 1 ? 1 : (512 / kRegs > 8 ? 8 : 512 / kRegs);
+};
+
+template <int TI, int D, bool MAT, bool FAST>
+__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(TileGeom<TI, D>::kWaves)))
+k_tb(wf_tb_args a) {
+    constexpr int TJ = kW - 2 * D;
+    constexpr int RI = TI + 2 * D;
+    constexpr int NR = TileGeom<TI, D>::NR;
+    __shared__ float s_hz[RI * kW];
+    __shared__ float s_ey[RI * kW];
+    __shared__ int s_grp[kMaxGroups];
+    __shared__ int s_grp_cell[kMaxGroups];
+    __shared__ int s_det[kMaxDets];
+    __shared__ int s_det_cell[kMaxDets];
+    __shared__ int s_ngrp, s_ndet, s_ey_src;
+    __shared__ unsigned int s_rowmask[4];  // per wavefront: bit k set if row w+4k holds a source
+    __shared__ unsigned int s_max[3];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kW - 1);
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave index, uniform
+    const int per = (a.n_tiles + 7) / 8;
+    const int t = (blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (t >= a.n_tiles) return;
+    const int run = a.tiles[2 * t];
+    const int tile = a.tiles[2 * t + 1];
+    const wf_run2d &R = a.runs[run];
+    const int nx = R.nx, ny = R.ny;
+    const int ntj = (ny + TJ - 1) / TJ;
+    const int I0 = (tile / ntj) * TI;
+    const int J0 = (tile % ntj) * TJ;
+    const int gi0 = I0 - D, gj0 = J0 - D;  // global coordinates of region (0, 0)
+    const int gj = gj0 + lane;
+    const bool col_in = gj >= 0 && gj < ny;
+
+    const float *src_ex = a.parity ? R.ex2 : R.ex;
+    const float *src_ey = a.parity ? R.ey2 : R.ey;
+    const float *src_hz = a.parity ? R.hz2 : R.hz;
+    float *dst_ex = a.parity ? R.ex : R.ex2;
+    float *dst_ey = a.parity ? R.ey : R.ey2;
+    float *dst_hz = a.parity ? R.hz : R.hz2;
+
+    if (tid == 0) {
+        s_ngrp = 0;
+        s_ndet = 0;
+        s_ey_src = 0;
+        s_max[0] = s_max[1] = s_max[2] = 0u;
+        s_rowmask[0] = s_rowmask[1] = s_rowmask[2] = s_rowmask[3] = 0u;
+    }
+
+    float ex[NR], ey[NR], hz[NR], ca[NR], cb[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int li = w + 4 * k;
+        const int gi = gi0 + li;
+        ex[k] = ey[k] = hz[k] = 0.f;
+        ca[k] = cb[k] = 0.f;
+        if (li < RI) {
+            if (gi >= 0 && gi < nx && col_in) {
+                const size_t g = (size_t)gi * ny + gj;
+                ex[k] = src_ex[g];
+                ey[k] = src_ey[g];
+                hz[k] = src_hz[g];
+                if (MAT) {
+                    ca[k] = R.ca[g];
+                    cb[k] = R.cb[g];
+                }
+            }
+            s_hz[li * kW + lane] = hz[k];
+            s_ey[li * kW + lane] = ey[k];
+        }
+    }
+    __syncthreads();
+    for (int g = tid; g < R.n_grp; g += kNT) {
+        const wf_inj2d e0 = R.inj[R.grp_start[g]];
+        const int li = e0.cell / ny - gi0, lj = e0.cell % ny - gj0;
+        if (li >= 0 && li < RI && lj >= 0 && lj < kW) {
+            const int k = atomicAdd(&s_ngrp, 1);
+            if (k < kMaxGroups) {
+                s_grp[k] = g;
+                s_grp_cell[k] = li * kW + lj;
+                atomicOr(&s_rowmask[li & 3], 1u << (li >> 2));
+                if (e0.comp == WF_COMP_EY) s_ey_src = 1;
+            }
+        }
+    }
+    for (int d = tid; d < R.n_det; d += kNT) {
+        const int cell = R.det_cell[d];
+        const int gi = cell / ny, gjd = cell % ny;
+        if (gi >= I0 && gi < I0 + TI && gjd >= J0 && gjd < J0 + TJ) {
+            const int k = atomicAdd(&s_ndet, 1);
+            if (k < kMaxDets) {
+                s_det[k] = d;
+                s_det_cell[k] = (gi - gi0) * kW + (gjd - gj0);
+            }
+        }
+    }
+    __syncthreads();
+    const int ngrp = min(s_ngrp, kMaxGroups);
+    const int ndet = min(s_ndet, kMaxDets);
+    const bool ey_src = s_ey_src != 0;
+    const unsigned int rowmask = s_rowmask[w];
+    const bool edge = (gi0 <= 0) || (gi0 + RI >= nx) || (gj0 <= 0) || (gj0 + kW >= ny);
+    const float dh = R.dh, de = R.de, cx = R.cmx, cy = R.cmy;
+    const Divider divx{R.dx, a.rdx, a.div_lo_x, a.div_span_x};
+    const Divider divy{R.dy, a.rdy, a.div_lo_y, a.div_span_y};
+    (void)de;
+
+#pragma unroll 1
+    for (int s = 1; s <= D; ++s) {
+        const int n = R.step + a.step_offset + s - 1;
+        // 1. Soft sources into the owning lane's registers (rows that hold a
+        // source are flagged per wavefront, so the scan is skipped elsewhere).
+        if (ngrp > 0) {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                if (rowmask & (1u << k)) {
+                    const int li = w + 4 * k;
+                    for (int q = 0; q < ngrp; ++q) {
+                        const int cl = s_grp_cell[q];
+                        if (cl == li * kW + lane) {
+                            const int g = s_grp[q];
+                            const int b0 = R.grp_start[g], b1 = R.grp_start[g + 1];
+                            const int comp = R.inj[b0].comp;
+                            float v = comp == WF_COMP_EX ? ex[k] : (comp == WF_COMP_EY ? ey[k] : hz[k]);
+                            for (int e = b0; e < b1; ++e) {
+                                const float wv = R.inj[e].wave[n] * R.inj[e].amp;
+                                v = v + wv;
+                            }
+                            if (comp == WF_COMP_EX) ex[k] = v;
+                            else if (comp == WF_COMP_EY) { ey[k] = v; s_ey[cl] = v; }
+                            else hz[k] = v;
+                        }
+                    }
+                }
+            }
+            if (ey_src) __syncthreads();
+        }
+        // 2. Faraday + Mur (owner writes through LDS). Row conditions are
+        // wavefront-uniform; per-lane conditions become selects.
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const int li = w + 4 * k;
+            if (li >= s - 1 && li < RI - s) {
+                const int gi = gi0 + li;
+                const float ex_next = lane_next(ex[k]);
+                const int c = li * kW + lane;
+                const float ey_down = s_ey[c + kW];
+                const float av = divide<FAST>(ex_next - ex[k], divy);
+                const float bv = divide<FAST>(ey_down - ey[k], divx);
+                const float F = hz[k] + dh * (av - bv);
+                if (gi >= 1 && gi <= nx - 2) {
+                    const bool lane_int = gj >= 1 && gj <= ny - 2;
+                    if (edge && lane_int) {
+                        const float p_own = s_hz[c];
+                        if (gi == 1 && li >= 1) {
+                            const float p0 = s_hz[c - kW];
+                            const float L = p_own + cx * (F - p0);
+                            if (gj == 1 && lane >= 1) {
+                                const float p00 = s_hz[c - kW - 1];
+                                s_hz[c - kW - 1] = p0 + cy * (L - p00);
+                            }
+                            if (gj == ny - 2 && lane + 1 < kW) {
+                                const float p01 = s_hz[c - kW + 1];
+                                s_hz[c - kW + 1] = p0 + cy * (L - p01);
+                            }
+                            s_hz[c - kW] = L;
+                        }
+                        if (gi == nx - 2 && li + 1 < RI) {
+                            const float p1 = s_hz[c + kW];
+                            const float Rv = p_own + cx * (F - p1);
+                            if (gj == 1 && lane >= 1) {
+                                const float p10 = s_hz[c + kW - 1];
+                                s_hz[c + kW - 1] = p1 + cy * (Rv - p10);
+                            }
+                            if (gj == ny - 2 && lane + 1 < kW) {
+                                const float p11 = s_hz[c + kW + 1];
+                                s_hz[c + kW + 1] = p1 + cy * (Rv - p11);
+                            }
+                            s_hz[c + kW] = Rv;
+                        }
+                        if (gj == 1 && lane >= 1) {
+                            const float pl = s_hz[c - 1];
+                            s_hz[c - 1] = p_own + cy * (F - pl);
+                        }
+                        if (gj == ny - 2 && lane + 1 < kW) {
+                            const float pr = s_hz[c + 1];
+                            s_hz[c + 1] = p_own + cy * (F - pr);
+                        }
+                    }
+                    hz[k] = lane_int ? F : hz[k];
+                    if (lane_int) s_hz[c] = F;
+                }
+            }
+        }
+        __syncthreads();
+        if (edge) {  // Mur values written by their owners
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const int li = w + 4 * k;
+                const int gi = gi0 + li;
+                if (li < RI && gi >= 0 && gi < nx && col_in && (gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1))
+                    hz[k] = s_hz[li * kW + lane];
+            }
+        }
+        // 3. Ampere + detectors.
+        if (tid < ndet) {
+            const int tt = n - R.trace_n0;
+            if (tt >= 0 && tt < R.trace_len) R.traces[s_det[tid] * R.trace_stride + tt] = s_hz[s_det_cell[tid]];
+        }
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const int li = w + 4 * k;
+            if (li >= s && li < RI - s) {
+                const int gi = gi0 + li;
+                const float hz_left = lane_prev(hz[k]);
+                const int c = li * kW + lane;
+                const float hz_up = s_hz[c - kW];
+                const float gyv = divide<FAST>(hz[k] - hz_left, divy);
+                const float gxv = divide<FAST>(hz[k] - hz_up, divx);
+                if (gi >= 0 && gi < nx) {
+                    float nex, ney;
+                    if (MAT) {
+                        const float t1 = ca[k] * ex[k];
+                        const float t2 = cb[k] * gyv;
+                        nex = t1 + t2;
+                        const float u1 = ca[k] * ey[k];
+                        const float u2 = cb[k] * gxv;
+                        ney = u1 - u2;
+                    } else {
+                        const float t2 = de * gyv;
+                        nex = ex[k] + t2;
+                        const float u2 = de * gxv;
+                        ney = ey[k] - u2;
+                    }
+                    ex[k] = (col_in && gj >= 1) ? nex : ex[k];
+                    if (gi >= 1) ey[k] = col_in ? ney : ey[k];
+                    s_ey[c] = ey[k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // Write the core back (and fold the stability maxima if requested).
+    unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
+    const bool lane_core = lane >= D && lane < D + TJ && col_in;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const int li = w + 4 * k;
+        const int gi = gi0 + li;
+        if (li >= D && li < D + TI && gi < nx && lane_core) {
+            const size_t g = (size_t)gi * ny + gj;
+            dst_ex[g] = ex[k];
+            dst_ey[g] = ey[k];
+            dst_hz[g] = hz[k];
+            m0 = max(m0, wf::abs_bits(ex[k]));
+            m1 = max(m1, wf::abs_bits(ey[k]));
+            m2 = max(m2, wf::abs_bits(hz[k]));
+        }
+    }
+    if (a.maxabs_out) {
+        atomicMax(&s_max[0], m0);
+        atomicMax(&s_max[1], m1);
+        atomicMax(&s_max[2], m2);
+        __syncthreads();
+        if (tid < 3) atomicMax(&a.maxabs_out[3 * run + tid], s_max[tid]);
+    }
+}
+
+template <int TI, bool MAT, bool FAST>
+hipError_t launch_depth(const wf_tb_args &a, hipStream_t st) {
+    const unsigned blocks = 8u * (unsigned)((a.n_tiles + 7) / 8);
+    switch (a.depth) {
+#define WF_TB_CASE(DD)                                                                         \
+    case DD:                                                                                   \
+        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST>), dim3(blocks), dim3(kNT), 0, st, a);      \
+        break;
+        WF_TB_CASE(1)
+        WF_TB_CASE(2)
+        WF_TB_CASE(3)
+        WF_TB_CASE(4)
+        WF_TB_CASE(5)
+        WF_TB_CASE(6)
+        WF_TB_CASE(7)
+        WF_TB_CASE(8)
+#undef WF_TB_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+
+}  // namespace wf_tb
+
+// One object file per (materials, division) pair instantiates the kernels for
+// every tile height and depth; the entry point in wf_tb.hip dispatches here.
+#define WF_TB_DECLARE(NAME) hipError_t NAME(const wf_tb_args &a, hipStream_t st)
+WF_TB_DECLARE(wf_tb_launch_mat_fast);
+WF_TB_DECLARE(wf_tb_launch_mat_ieee);
+WF_TB_DECLARE(wf_tb_launch_free_fast);
+WF_TB_DECLARE(wf_tb_launch_free_ieee);
+
+#define WF_TB_DEFINE(NAME, MAT, FAST)                                                   \
+    WF_TB_DECLARE(NAME) {                                                               \
+        switch (a.tile_i) {                                                             \
+            case 16: return wf_tb::launch_depth<16, MAT, FAST>(a, st);                  \
+            case 32: return wf_tb::launch_depth<32, MAT, FAST>(a, st);                  \
+            case 48: return wf_tb::launch_depth<48, MAT, FAST>(a, st);                  \
+            default: return hipErrorInvalidValue;                                       \
+        }                                                                               \
+    }

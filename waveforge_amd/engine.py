@@ -219,14 +219,15 @@ def maxabs(ex: torch.Tensor, ey: torch.Tensor, hz: torch.Tensor, stream: int) ->
     return out
 
 
-def tile_count(nx: int, ny: int, tile: tuple[int, int]) -> int:
-    return ((nx + tile[0] - 1) // tile[0]) * ((ny + tile[1] - 1) // tile[1])
+def tile_count(nx: int, ny: int, tile_i: int, depth: int) -> int:
+    tj = _native.tile_cols(depth)
+    return ((nx + tile_i - 1) // tile_i) * ((ny + tj - 1) // tj)
 
 
-def tb_limits_ok(run: RunState, tile: tuple[int, int], depth: int) -> bool:
+def tb_limits_ok(run: RunState, tile_i: int, depth: int) -> bool:
     """True if every tile region of ``run`` holds at most the per-tile source
     groups and core detectors the temporally blocked kernel can stage."""
-    TI, TJ = tile
+    TI, TJ = tile_i, _native.tile_cols(depth)
     nti = (run.nx + TI - 1) // TI
     ntj = (run.ny + TJ - 1) // TJ
     if run.sources is not None and run.sources.n_grp:
@@ -255,6 +256,46 @@ def tb_limits_ok(run: RunState, tile: tuple[int, int], depth: int) -> bool:
     return True
 
 
+def _f32_bits(x: float) -> int:
+    return int(np.array(x, dtype=np.float32).view(np.uint32))
+
+
+def division_range(d: float) -> tuple[int, int]:
+    """Bit range ``[lo, lo+span)`` of ``|x|`` on which the corrected reciprocal
+    quotient is checked against IEEE ``x/d``: quotients stay normal and finite
+    and the residual ``x - q0*d`` stays normal there."""
+    ed = ((_f32_bits(d) >> 23) & 0xFF) - 127
+    lo_e = max(-99, ed - 122)
+    hi_e = min(127, ed + 125)
+    lo = (lo_e + 127) << 23
+    hi = (hi_e + 127) << 23
+    return lo, hi - lo
+
+
+_DIV_CACHE: dict = {}
+
+
+def verified_division(d: float, device: torch.device) -> Optional[tuple[float, int, int]]:
+    """``(1/d, lo, span)`` if the device proved the fast quotient exact for
+    divisor ``fp32(d)`` on that range (exhaustively, once per process), else None."""
+    df = f32(d)
+    key = (df, str(device))
+    if key in _DIV_CACHE:
+        return _DIV_CACHE[key]
+    if not (df > 0.0) or ((_f32_bits(df) >> 23) & 0xFF) in (0, 0xFF):
+        _DIV_CACHE[key] = None
+        return None
+    r = float(np.float32(1.0) / np.float32(df))
+    lo, span = division_range(df)
+    bad = torch.zeros(1, dtype=torch.int32, device=device)
+    lib = _native.load()
+    stream = _native.stream_handle(device)
+    _native.check(lib.wf_div_verify(df, r, lo, span, bad.data_ptr(), ctypes.c_void_p(stream)), "wf_div_verify")
+    ok = int(bad.item()) == 0
+    _DIV_CACHE[key] = (r, lo, span) if ok else None
+    return _DIV_CACHE[key]
+
+
 def split_launches(n_steps: int, depth: int) -> Optional[list[int]]:
     """Depths (each in ``[1, depth]``) of an even number of launches covering
     ``n_steps`` steps, or ``None`` when no such split exists (odd ``n_steps``
@@ -271,7 +312,7 @@ def split_launches(n_steps: int, depth: int) -> Optional[list[int]]:
 
 
 class TBPlan:
-    """Device run table + tile list for the temporally blocked kernel.
+    """Device run table + per-depth tile lists for the temporally blocked kernel.
 
     ``TBPlan.event_log``: when set to a list, every launch appends
     ``(depth, n_cells, start_event, end_event)`` with HIP events recorded on the
@@ -279,9 +320,10 @@ class TBPlan:
 
     event_log: Optional[list] = None
 
-    def __init__(self, runs: Sequence[RunState], tile: tuple[int, int], device: torch.device) -> None:
-        self.tile = tile
+    def __init__(self, runs: Sequence[RunState], tile_i: int, device: torch.device, fast_div: bool = True) -> None:
+        self.tile_i = int(tile_i)
         self.runs = list(runs)
+        self.device = device
         mats = {r.ca is not None for r in runs}
         if len(mats) != 1:
             raise ValueError("temporally blocked launch needs all runs with or all without materials")
@@ -289,27 +331,46 @@ class TBPlan:
         arr = (WfRun2d * len(runs))(*[r.struct() for r in runs])
         raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
         self.run_table = torch.from_numpy(raw).to(device)
-        tiles = []
-        for k, r in enumerate(runs):
-            nt = tile_count(r.nx, r.ny, tile)
-            tiles.append(np.stack([np.full(nt, k, dtype=np.int32), np.arange(nt, dtype=np.int32)], axis=1))
-        t = np.concatenate(tiles, axis=0)
-        self.n_tiles = int(t.shape[0])
-        self.tile_table = torch.from_numpy(np.ascontiguousarray(t)).to(device)
+        self._tiles: dict[int, tuple[torch.Tensor, int]] = {}
+        self.div = None
+        if fast_div:
+            dxs = {r.coef.dx for r in runs}
+            dys = {r.coef.dy for r in runs}
+            if len(dxs) == 1 and len(dys) == 1:
+                vx = verified_division(dxs.pop(), device)
+                vy = verified_division(dys.pop(), device)
+                if vx is not None and vy is not None:
+                    self.div = (vx, vy)
+        self.n_cells = sum(r.nx * r.ny for r in runs)
+
+    def tiles(self, depth: int) -> tuple[torch.Tensor, int]:
+        if depth not in self._tiles:
+            parts = []
+            for k, r in enumerate(self.runs):
+                nt = tile_count(r.nx, r.ny, self.tile_i, depth)
+                parts.append(np.stack([np.full(nt, k, dtype=np.int32), np.arange(nt, dtype=np.int32)], axis=1))
+            t = np.ascontiguousarray(np.concatenate(parts, axis=0))
+            self._tiles[depth] = (torch.from_numpy(t).to(self.device), int(t.shape[0]))
+        return self._tiles[depth]
 
     def launch(self, depth: int, parity: int, step_offset: int, stream: int,
                maxabs_out: Optional[torch.Tensor] = None) -> None:
+        tiles, n_tiles = self.tiles(depth)
         a = WfTbArgs()
         a.runs = self.run_table.data_ptr()
-        a.tiles = self.tile_table.data_ptr()
+        a.tiles = tiles.data_ptr()
         a.n_runs = len(self.runs)
-        a.n_tiles = self.n_tiles
-        a.tile_i, a.tile_j = self.tile
+        a.n_tiles = n_tiles
+        a.tile_i = self.tile_i
+        a.tile_j = _native.tile_cols(depth)
         a.depth = int(depth)
         a.parity = int(parity)
         a.step_offset = int(step_offset)
         a.has_materials = self.has_materials
         a.maxabs_out = maxabs_out.data_ptr() if maxabs_out is not None else 0
+        if self.div is not None:
+            (a.rdx, a.div_lo_x, a.div_span_x), (a.rdy, a.div_lo_y, a.div_span_y) = self.div
+            a.fast_div = 1
         lib = _native.load()
         log = TBPlan.event_log
         if log is not None:
@@ -319,7 +380,7 @@ class TBPlan:
         _native.check(lib.wf_fdtd2d_tb_advance(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_fdtd2d_tb_advance")
         if log is not None:
             ev1.record()
-            log.append((int(depth), sum(r.nx * r.ny for r in self.runs), ev0, ev1))
+            log.append((int(depth), self.n_cells, ev0, ev1))
 
     def advance(self, n_steps: int, depth: int, stream: int, base_step: int = 0) -> None:
         """Advance every run ``n_steps`` (>= 2) steps, ending in the live buffers.
