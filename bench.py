@@ -50,7 +50,23 @@ def parse():
     p.add_argument("--tile", type=int, default=None, help="tile rows (tile columns are 64 - 2*depth)")
     p.add_argument("--cpu-steps", type=int, default=8, help="timed steps of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
+                        "traffic used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     return p.parse_args()
+
+
+def copy_probe() -> int:
+    import torch
+
+    n = 512 * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "dispatches": 3}))
+    return 0
 
 
 def cpu_baseline(wl, n_steps: int) -> dict:
@@ -84,6 +100,8 @@ def cpu_baseline(wl, n_steps: int) -> dict:
 
 def main() -> int:
     args = parse()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(json.dumps({"error": "waveforge_amd runs on one GPU; multi-GPU scaling is not part of this build",
                           "n_gpus": args.gpus}))

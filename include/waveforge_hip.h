@@ -131,9 +131,21 @@ typedef struct wf_tb_args {
     uint32_t div_lo_y, div_span_y;
     int32_t fast_div;          /* 1: corrected reciprocal division (all runs share dx, dy) */
     int32_t pad;
+    const int32_t *stop;       /* NULL or device n_runs flags: a run whose flag is set is
+                                  copied through unchanged (it failed a stability check) */
 } wf_tb_args;
 
 int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
+
+/* Stability judgement on the device, so the stepping stream never waits for
+ * the host: for each run r, if any of the three maxima maxabs[3r..3r+2]
+ * (uint bit patterns of max|Ex|, max|Ey|, max|Hz|, as folded by
+ * wf_fdtd2d_tb_advance) exceeds `threshold`, set stop[r] = 1; with all != 0,
+ * one failing run sets every flag (a batched FieldSet fails as a whole).
+ * Replaces the `val > threshold` test of FDTD2D._check_stability
+ * (src/core/fdtd2d.py:285-295); NaN maxima never exceed, as in the reference. */
+int wf_fdtd2d_judge(const uint32_t *maxabs, int32_t n_runs, double threshold, int32_t all, int32_t *stop,
+                    hipStream_t stream);
 
 /* Supported tile_i values for wf_tb_args, written into out as (tile_i, 0)
  * pairs (host, 2*count int32); returns count (<= cap). */

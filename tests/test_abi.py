@@ -67,7 +67,7 @@ int main(void) {
   O(wf_tb_args, tile_j) O(wf_tb_args, depth) O(wf_tb_args, parity) O(wf_tb_args, step_offset)
   O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out) O(wf_tb_args, rdx) O(wf_tb_args, rdy)
   O(wf_tb_args, div_lo_x) O(wf_tb_args, div_span_x) O(wf_tb_args, div_lo_y) O(wf_tb_args, div_span_y)
-  O(wf_tb_args, fast_div) O(wf_tb_args, pad)
+  O(wf_tb_args, fast_div) O(wf_tb_args, pad) O(wf_tb_args, stop)
   O(wf_das2d_args, traces) O(wf_das2d_args, ant_x) O(wf_das2d_args, ant_y) O(wf_das2d_args, image)
   O(wf_das2d_args, n_tx) O(wf_das2d_args, n_rx) O(wf_das2d_args, n_t) O(wf_das2d_args, nx_img)
   O(wf_das2d_args, ny_img) O(wf_das2d_args, gate_start) O(wf_das2d_args, clip) O(wf_das2d_args, pad)

@@ -1,0 +1,161 @@
+"""Host-side logic of the product on the CPU (no kernel launches): API
+validation mirrors the reference, waveform tables and material coefficients
+equal the oracle's restatement, launch scheduling invariants, and the product
+refusing to step CPU tensors."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import waveforms_ref as wr
+from oracle.fdtd2d_ref import ca_cb_from_maps, material_ca_cb, stable_dt
+from waveforge_amd import core
+from waveforge_amd.engine import division_range, schedule_launches, split_launches
+
+
+def test_grid_validation_matches_reference():
+    with pytest.raises(ValueError):
+        core.YeeGrid(1, 8, 1e-3, 1e-3, device="cpu")
+    with pytest.raises(ValueError):
+        core.YeeGrid(8, 8, -1e-3, 1e-3, device="cpu")
+    with pytest.raises(ValueError):
+        core.YeeGrid(8, 8, 1e-3, 1e-3, courant=1.0, device="cpu")
+    g = core.YeeGrid(8, 8, 1e-3, 1e-3, device="cpu")
+    with pytest.raises(ValueError):
+        core.YeeGrid(8, 8, 1e-3, 1e-3, dt=2 * g.dt, device="cpu")
+    with pytest.warns(UserWarning):
+        core.YeeGrid(8, 8, 1e-3, 1e-3, dt=0.01 * g.dt, device="cpu")
+    assert g.dt == stable_dt(1e-3, 1e-3)
+    assert g.cell_index(2, 3) == 2 * 8 + 3
+
+
+def test_fdtd2d_construction_checks():
+    g = core.YeeGrid(8, 8, 1e-3, 1e-3, device="cpu")
+    f = core.FieldSet(g)
+    b = core.MurABC(g, f.Hz)
+    with pytest.raises(ValueError):
+        core.FDTD2D(core.YeeGrid(3, 8, 1e-3, 1e-3, device="cpu"), f, b)
+    with pytest.raises(ValueError):
+        core.FDTD2D(g, f, core.MurABC(core.YeeGrid(9, 8, 1e-3, 1e-3, device="cpu"), f.Hz))
+    with pytest.raises(ValueError):
+        core.FDTD2D(g, f, b, Ca=torch.ones(8, 8, 1))
+
+
+def test_product_refuses_cpu_tensors():
+    g = core.YeeGrid(8, 8, 1e-3, 1e-3, device="cpu")
+    f = core.FieldSet(g)
+    sim = core.FDTD2D(g, f, core.MurABC(g, f.Hz))
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        sim.step()
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        sim.run(10)
+
+
+WAVES = [
+    (lambda: core.GaussianPulse(1.3, sigma=7e-12), lambda t: wr.gaussian(t, 1.3, 35e-12, 7e-12)),
+    (lambda: core.SinusoidalSource(0.5, 3e9, 0.0), lambda t: wr.sinusoidal(t, 0.5, 3e9, 0.0)),
+    (lambda: core.RickerWavelet(1.0, 1e9), lambda t: wr.ricker(t, 1.0, 1e9, 1.5e-9)),
+    (lambda: core.ModulatedGaussian(1.0, 1e9, sigma=1 / (2 * math.pi * 0.5e9)),
+     lambda t: wr.modulated_gaussian(t, 1.0, 1e9, 1 / (2 * math.pi * 0.5e9), 5 / (2 * math.pi * 0.5e9))),
+    (lambda: core.Chirp(1.0, 1e8, 1e9, 1e-10, 3e-9), lambda t: wr.chirp(t, 1.0, 1e8, 1e9, 1e-10, 3e-9)),
+    (lambda: core.UWBPulse(1.0, 1e9, 3e9), lambda t: wr.uwb(t, 1.0, 1e9, 3e9)),
+]
+
+
+@pytest.mark.parametrize("k", range(len(WAVES)))
+def test_waveform_tables_equal_oracle(k):
+    make, ref = WAVES[k]
+    dt = stable_dt(1e-3, 1e-3)
+    w = make().build(3000, dt, torch.device("cpu"))
+    r = ref(wr.time_axis(3000, dt))
+    assert w.dtype == torch.float32 and torch.equal(w, r)
+
+
+def test_waveform_closed_forms():
+    """Oracle waveforms against fp64 math-module evaluation (fp32 tolerance) and
+    their defining properties: Gaussian peak value, Ricker zero crossings at
+    t0 +- 1/(sqrt2 pi fp), UWB peak normalised to the amplitude."""
+    dt = stable_dt(1e-3, 1e-3)
+    t = wr.time_axis(4000, dt)
+    g = wr.gaussian(t, 2.0, 100 * dt, 10 * dt)
+    n = 100
+    assert abs(float(g[n]) - 2.0) < 1e-6
+    exact = np.array([2.0 * math.exp(-((k * dt - 100 * dt) ** 2) / (2 * (10 * dt) ** 2)) for k in range(400)])
+    assert np.allclose(g[:400].numpy(), exact, rtol=2e-5, atol=1e-6)
+    fp = 1e9
+    r = wr.ricker(wr.time_axis(4000, dt, torch.float64), 1.0, fp, 1.5 / fp).numpy()
+    zc = 1.5 / fp + 1 / (math.sqrt(2) * math.pi * fp)
+    kz = int(zc / dt)
+    assert r[kz] * r[kz + 1] <= 0.0
+    u = wr.uwb(wr.time_axis(20000, 1e-13, torch.float64), 1.0, 1e9, 3e9).numpy()
+    assert abs(np.abs(u).max() - 1.0) < 1e-4
+
+
+def test_material_map_matches_closed_form():
+    g = core.YeeGrid(20, 16, 1e-3, 1e-3, device="cpu")
+    mm = core.MaterialMap(g)
+    brain = core.Material("brain", 45.0, 0.8)
+    mm.add_rectangle((2, 5), (0, 15), core.Material("slab", 4.0, 0.0))
+    mm.add_circle((10, 8), 4, brain)
+    ca, cb = mm.build()
+    a, b = material_ca_cb(45.0, 0.8, g.dt)
+    assert ca.shape == (20, 16, 1) and ca.dtype == torch.float32
+    assert float(ca[10, 8, 0]) == np.float32(a) and float(cb[10, 8, 0]) == np.float32(b)
+    assert float(ca[3, 0, 0]) == 1.0
+    assert float(cb[0, 0, 0]) == np.float32(material_ca_cb(1.0, 0.0, g.dt)[1])
+    eps = torch.full((20, 16), 45.0, dtype=torch.float64)
+    sig = torch.full((20, 16), 0.8, dtype=torch.float64)
+    c1, c2 = core.coefficients_from_maps(eps, sig, g.dt)
+    o1, o2 = ca_cb_from_maps(eps, sig, g.dt)
+    assert torch.equal(c1.squeeze(2), o1) and torch.equal(c2.squeeze(2), o2)
+    with pytest.raises(ValueError):
+        core.coefficients_from_maps(eps * 0.01, sig, g.dt)
+
+
+def test_source_validation():
+    g = core.YeeGrid(10, 10, 1e-3, 1e-3, device="cpu")
+    w = core.GaussianPulse(1.0, sigma=5e-12)
+    with pytest.raises(ValueError):
+        core.PointSource(w, 10, 0, "Hz", grid=g, N_steps=10)
+    with pytest.raises(ValueError):
+        core.PointSource(w, 0, 0, "Bz", grid=g, N_steps=10)
+    with pytest.raises(ValueError):
+        core.LineSource(w, "x", 3, 5, 5, "Hz", grid=g, N_steps=10)
+    with pytest.raises(ValueError):
+        core.PlaneSource(w, "xz", 10, "Hz", grid=g, N_steps=10)
+
+
+def test_split_launches_even():
+    for n in range(2, 60):
+        for d in range(1, 9):
+            r = split_launches(n, d)
+            if r is None:
+                assert d == 1 and n % 2
+                continue
+            assert sum(r) == n and len(r) % 2 == 0 and max(r) <= d and min(r) >= 1
+
+
+def test_schedule_launches_checks():
+    for start in (0, 5, 99, 100):
+        for n in (2, 3, 50, 101, 333):
+            for d in (1, 3, 6, 8):
+                for ce in (1, 10, 100):
+                    L, checks = schedule_launches(start, n, d, ce)
+                    if L is None:
+                        continue
+                    assert sum(x for x, _ in L) == n and len(L) % 2 == 0
+                    step = start
+                    for dd, slot in L:
+                        step += dd
+                        if slot >= 0:
+                            assert checks[slot] == step and step % ce == 0
+                    assert len({s for _, s in L if s >= 0}) == len(checks)
+
+
+def test_division_range_for_millimetre_cells():
+    """The verified range for dx = 1 mm covers every |x| in [2^-99, 2^115)."""
+    lo, span = division_range(float(np.float32(1e-3)))
+    assert lo == (127 - 99) << 23
+    assert lo + span == (127 + 115) << 23

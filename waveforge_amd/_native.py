@@ -24,6 +24,7 @@ EXPORTED_SYMBOLS = (
     "wf_fdtd2d_step",
     "wf_fdtd2d_maxabs",
     "wf_fdtd2d_tb_advance",
+    "wf_fdtd2d_judge",
     "wf_tb_tile_shapes",
     "wf_div_verify",
     "wf_das2d",
@@ -82,6 +83,7 @@ class WfTbArgs(ctypes.Structure):
         ("div_lo_x", ctypes.c_uint32), ("div_span_x", ctypes.c_uint32),
         ("div_lo_y", ctypes.c_uint32), ("div_span_y", ctypes.c_uint32),
         ("fast_div", _i32), ("pad", _i32),
+        ("stop", _vp),
     ]
 
 
@@ -129,6 +131,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_fdtd2d_tb_advance.restype = ctypes.c_int
         lib.wf_tb_tile_shapes.argtypes = [ctypes.POINTER(_i32), _i32]
         lib.wf_tb_tile_shapes.restype = ctypes.c_int
+        lib.wf_fdtd2d_judge.argtypes = [_vp, _i32, _f64, _i32, _vp, _vp]
+        lib.wf_fdtd2d_judge.restype = ctypes.c_int
         lib.wf_div_verify.argtypes = [_f32, _f32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]
         lib.wf_div_verify.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]

@@ -27,6 +27,8 @@ import torch
 from .. import _native
 from ..engine import (
     RunState,
+    judge,
+    schedule_launches,
     SourceTable,
     StepCoefficients,
     TBPlan,
@@ -216,15 +218,11 @@ class FDTD2D:
         """Advance ``n_steps`` steps starting at ``steps_completed`` (no checks)."""
         if n_steps <= 0:
             return
-        self._device_check()
+        setup = self._tb_setup()
         runs = self._build_runs()
         stream = self._stream()
-        inert = self._src_table.inert if self._src_table is not None else []
-        tile = self._tile or DEFAULT_TILE_ROWS
-        depth = self._depth or DEFAULT_DEPTH
-        use_tb = n_steps >= 2 and not inert and all(
-            tb_limits_ok(r, tile, d) for r in runs for d in range(1, depth + 1))
-        if use_tb:
+        if n_steps >= 2 and setup is not None:
+            _, tile, depth = setup
             first = self.steps_completed
             if split_launches(n_steps, depth) is None:
                 # odd step count at depth 1: one in-place step, then an even split
@@ -279,9 +277,65 @@ class FDTD2D:
 
     def _check_stability(self) -> None:
         """Max-abs check of Ex, Ey, Hz (``src/core/fdtd2d.py:268-295``)."""
-        self.n_stability_checks += 1
         f = self._fields
-        vals = maxabs(f.Ex, f.Ey, f.Hz, self._stream()).cpu().tolist()
+        self._judge_values(maxabs(f.Ex, f.Ey, f.Hz, self._stream()).cpu().tolist())
+
+    def _tb_setup(self):
+        """(runs, tile, depth) if the temporally blocked kernel can run this
+        simulation (no inert-component sources, per-tile limits met)."""
+        self._device_check()
+        runs = self._build_runs()
+        inert = self._src_table.inert if self._src_table is not None else []
+        tile = self._tile or DEFAULT_TILE_ROWS
+        depth = self._depth or DEFAULT_DEPTH
+        key = (self._cache_key, tile, depth)
+        if getattr(self, "_tb_ok_key", None) != key:
+            self._tb_ok = not inert and all(tb_limits_ok(r, tile, d) for r in runs for d in range(1, depth + 1))
+            self._tb_ok_key = key
+        return (runs, tile, depth) if self._tb_ok else None
+
+    def _run_async(self, n_steps: int, setup) -> None:
+        """``n_steps`` steps with the stability checks judged on the device.
+
+        The whole run is enqueued without waiting for the host: at each check
+        step the launch ending there folds max|Ex|, |Ey|, |Hz| into a slot, a
+        one-thread kernel compares it with the threshold and raises a device
+        flag, and every later launch carries the flagged state through
+        unchanged. One synchronisation at the end reads the slots; a failed
+        check raises ``SimulationDivergedError`` with the fields and
+        ``steps_completed`` of that check step, as the reference does."""
+        runs, tile, depth = setup
+        start = self.steps_completed
+        limit = self._wave_limit()
+        n_exec = n_steps if limit is None else max(0, min(n_steps, limit - start))
+        launches, checks = schedule_launches(start, n_exec, depth, self._n_check)
+        if n_exec and launches is None:
+            return self._run_sync(n_steps)
+        stream = self._stream()
+        dev = self._fields.Hz.device
+        if n_exec:
+            plan = self._tb_plan_for(runs, tile)
+            slots = torch.zeros((max(1, len(checks)), len(runs), 3), dtype=torch.int32, device=dev)
+            stop = torch.zeros(len(runs), dtype=torch.int32, device=dev)
+            offset = start
+            for k, (d, slot) in enumerate(launches):
+                plan.launch(d, k % 2, offset, stream, maxabs_out=slots[slot] if slot >= 0 else None, stop=stop)
+                if slot >= 0:
+                    judge(slots[slot], self._threshold, stop, True, stream)
+                offset += d
+            vals = slots.view(torch.float32).amax(dim=1).cpu().tolist()
+            for step_m, (vx, vy, vh) in zip(checks, vals):
+                self.steps_completed = step_m
+                self._judge_values((vx, vy, vh))
+        self.steps_completed = start + n_exec
+        if n_exec < n_steps:
+            raise RuntimeError(
+                f"source waveform table has {limit} samples; cannot inject at step {self.steps_completed} "
+                "(build sources with a larger N_steps)"
+            )
+
+    def _judge_values(self, vals) -> None:
+        self.n_stability_checks += 1
         for name, val in zip(("Ex", "Ey", "Hz"), vals):
             if val > self.last_field_max:
                 self.last_field_max = val
@@ -295,10 +349,32 @@ class FDTD2D:
     def run(self, n_steps: int, *, verbose: bool = False) -> None:
         """Run ``n_steps`` steps (``src/core/fdtd2d.py:301-359``), same telemetry."""
         is_cuda = str(self._grid.device).startswith("cuda")
-        report = max(1, n_steps // 10)
         if is_cuda:
             torch.cuda.synchronize()
         t_start = time.perf_counter()
+        start = self.steps_completed
+        setup = self._tb_setup() if (n_steps >= 2 and not verbose) else None
+        try:
+            if setup is not None:
+                self._run_async(n_steps, setup)
+            else:
+                self._run_sync(n_steps, verbose=verbose, t_start=t_start)
+        finally:
+            if is_cuda:
+                torch.cuda.synchronize()
+            total = time.perf_counter() - t_start
+            self.elapsed_time += total
+            if total > 0.0:
+                self.mcells_per_second = (
+                    (self.steps_completed - start) * self._grid.Nx * self._grid.Ny / total / 1e6
+                )
+
+    def _run_sync(self, n_steps: int, verbose: bool = False, t_start: float | None = None) -> None:
+        """Chunked run with a host-side check at every ``n_check`` step (used
+        for verbose runs and whenever the temporally blocked kernel cannot run)."""
+        is_cuda = str(self._grid.device).startswith("cuda")
+        report = max(1, n_steps // 10)
+        t_start = time.perf_counter() if t_start is None else t_start
         start = self.steps_completed
         done = 0
         while done < n_steps:
@@ -332,12 +408,6 @@ class FDTD2D:
                     f"  {100 * done // n_steps:3d}%  step {self.steps_completed}  "
                     f"field_max={self.last_field_max:.3e}  {self.mcells_per_second:.1f} Mcells/s"
                 )
-        if is_cuda:
-            torch.cuda.synchronize()
-        total = time.perf_counter() - t_start
-        self.elapsed_time += total
-        if total > 0.0:
-            self.mcells_per_second = (self.steps_completed - start) * self._grid.Nx * self._grid.Ny / total / 1e6
 
     def reset(self) -> None:
         self._fields.zero_()

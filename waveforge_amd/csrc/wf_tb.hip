@@ -68,6 +68,39 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
     return wf::check_hip(e, "wf_fdtd2d_tb_advance");
 }
 
+namespace {
+
+__global__ void k_judge(const unsigned int *__restrict__ maxabs, int n_runs, double threshold, int all,
+                        int *__restrict__ stop) {
+    __shared__ int s_any;
+    if (threadIdx.x == 0) s_any = 0;
+    __syncthreads();
+    for (int r = threadIdx.x; r < n_runs; r += blockDim.x) {
+        bool bad = false;
+        for (int c = 0; c < 3; ++c) bad = bad || ((double)__uint_as_float(maxabs[3 * r + c]) > threshold);
+        if (bad) {
+            stop[r] = 1;
+            s_any = 1;
+        }
+    }
+    __syncthreads();
+    if (all && s_any)
+        for (int r = threadIdx.x; r < n_runs; r += blockDim.x) stop[r] = 1;
+}
+
+}  // namespace
+
+extern "C" int wf_fdtd2d_judge(const uint32_t *maxabs, int32_t n_runs, double threshold, int32_t all, int32_t *stop,
+                               hipStream_t stream) {
+    if (!maxabs || !stop || n_runs < 1) {
+        wf::set_error("wf_fdtd2d_judge: bad arguments");
+        return WF_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_judge, dim3(1), dim3(256), 0, stream, reinterpret_cast<const unsigned int *>(maxabs),
+                       n_runs, threshold, all, reinterpret_cast<int *>(stop));
+    return wf::check_launch("wf_fdtd2d_judge");
+}
+
 extern "C" int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, hipStream_t stream) {
     if (!bad || !(d > 0.0f) || span == 0u || lo + span < lo || lo + span > 0x7f800000u) {
         wf::set_error("wf_div_verify: bad arguments");

@@ -128,6 +128,23 @@ k_tb(wf_tb_args a) {
     float *dst_ey = a.parity ? R.ey : R.ey2;
     float *dst_hz = a.parity ? R.hz : R.hz2;
 
+    if (a.stop != nullptr && a.stop[run] != 0) {
+        // The run failed a stability check earlier in this stream: carry its
+        // state through unchanged so it ends in the live buffers as it was at
+        // the failing check.
+        for (int k = w; k < TI; k += 4) {
+            const int gi = I0 + k;
+            const int cj = J0 + lane;
+            if (gi < nx && lane < TJ && cj < ny) {
+                const size_t g = (size_t)gi * ny + cj;
+                dst_ex[g] = src_ex[g];
+                dst_ey[g] = src_ey[g];
+                dst_hz[g] = src_hz[g];
+            }
+        }
+        return;
+    }
+
     if (tid == 0) {
         s_ngrp = 0;
         s_ndet = 0;

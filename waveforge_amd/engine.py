@@ -354,7 +354,7 @@ class TBPlan:
         return self._tiles[depth]
 
     def launch(self, depth: int, parity: int, step_offset: int, stream: int,
-               maxabs_out: Optional[torch.Tensor] = None) -> None:
+               maxabs_out: Optional[torch.Tensor] = None, stop: Optional[torch.Tensor] = None) -> None:
         tiles, n_tiles = self.tiles(depth)
         a = WfTbArgs()
         a.runs = self.run_table.data_ptr()
@@ -368,6 +368,7 @@ class TBPlan:
         a.step_offset = int(step_offset)
         a.has_materials = self.has_materials
         a.maxabs_out = maxabs_out.data_ptr() if maxabs_out is not None else 0
+        a.stop = stop.data_ptr() if stop is not None else 0
         if self.div is not None:
             (a.rdx, a.div_lo_x, a.div_span_x), (a.rdy, a.div_lo_y, a.div_span_y) = self.div
             a.fast_div = 1
@@ -392,3 +393,51 @@ class TBPlan:
         for k, d in enumerate(depths):
             self.launch(d, k % 2, base_step + offset, stream)
             offset += d
+
+
+def judge(maxabs: torch.Tensor, threshold: float, stop: torch.Tensor, all_runs: bool, stream: int) -> None:
+    """Device-side stability judgement of one check slot (``wf_fdtd2d_judge``)."""
+    lib = _native.load()
+    n_runs = stop.numel()
+    _native.check(lib.wf_fdtd2d_judge(maxabs.data_ptr(), n_runs, float(threshold), 1 if all_runs else 0,
+                                      stop.data_ptr(), ctypes.c_void_p(stream)), "wf_fdtd2d_judge")
+
+
+def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optional[int]):
+    """Launch plan for ``n_steps`` steps from step ``start``.
+
+    Returns ``(launches, checks)``: ``launches`` is a list of ``(depth, slot)``
+    where ``slot`` (or -1) is the stability-check slot the launch's final
+    level feeds, ``checks`` the global step of each slot. Segments end at every
+    step ``m`` with ``m % check_every == 0``; the launch count is even (so the
+    state ends in the live buffers), or ``None`` is returned if that is
+    impossible (every launch of depth 1 and an odd count)."""
+    end = start + n_steps
+    checks = []
+    if check_every:
+        m = (start // check_every + 1) * check_every
+        while m <= end:
+            checks.append(m)
+            m += check_every
+    bounds = sorted(set(checks + [end]))
+    launches = []
+    prev = start
+    for b in bounds:
+        seg = b - prev
+        if seg > 0:
+            n_l = -(-seg // depth)
+            base, rem = divmod(seg, n_l)
+            for k in range(n_l):
+                launches.append([base + (1 if k < rem else 0), -1])
+            if b in checks:
+                launches[-1][1] = checks.index(b)
+        prev = b
+    if len(launches) % 2:
+        for k in range(len(launches) - 1, -1, -1):
+            d, slot = launches[k]
+            if d >= 2:
+                launches[k:k + 1] = [[d - d // 2, -1], [d // 2, slot]]
+                break
+        else:
+            return None, checks
+    return [tuple(x) for x in launches], checks
