@@ -159,6 +159,15 @@ def main() -> int:
         "bytes_per_launch_algorithmic": mean_bytes,
         "depths": depths,
     }
+    traffic_file = ROOT / "profiles" / "k_tb_blob4096_traffic.json"
+    tile_rows = sim._tile or 32
+    depth = sim._depth or 6
+    if traffic_file.exists():
+        tinfo = json.loads(traffic_file.read_text())
+        if (tinfo.get("grid"), tinfo.get("tile_rows"), tinfo.get("depth")) == (args.size, tile_rows, depth):
+            roofline["traffic"] = tinfo["hbm_bytes_per_launch"]
+            roofline["traffic_source"] = "profiles/" + tinfo["summary"] + " (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
+            roofline["traffic_source"] += "copy-kernel calibrated, per depth-%d launch)" % depth
     tr = wl.detectors.traces
     finite = bool(torch.isfinite(tr).all()) and bool(torch.isfinite(sim.fields.Hz).all())
     base = None
@@ -183,8 +192,8 @@ def main() -> int:
                         f"1 GHz modulated Gaussian at centre, Mur, 64 detectors",
             "grid": [args.size, args.size],
             "steps": args.steps,
-            "tile_rows": sim._tile or 32,
-            "depth": sim._depth or 4,
+            "tile_rows": tile_rows,
+            "depth": depth,
         },
         "roofline": roofline,
         "cpu_baseline": base,

@@ -47,7 +47,7 @@ EPS0: float = 8.8541878128e-12
 MU0: float = 1.2566370614e-6
 
 DEFAULT_TILE_ROWS = 32
-DEFAULT_DEPTH = 4
+DEFAULT_DEPTH = 6
 
 
 class SimulationDivergedError(RuntimeError):
