@@ -147,6 +147,12 @@ int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
 int wf_fdtd2d_judge(const uint32_t *maxabs, int32_t n_runs, double threshold, int32_t all, int32_t *stop,
                     hipStream_t stream);
 
+/* runs[r].step += delta for the n_runs device descriptors. A stepping loop
+ * captured in a HIP graph ends with this kernel, so each replay advances the
+ * step index its sources and detectors use. No reference counterpart: it
+ * serves the graph-captured form of the run loop (src/core/fdtd2d.py:328-329). */
+int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, hipStream_t stream);
+
 /* Supported tile_i values for wf_tb_args, written into out as (tile_i, 0)
  * pairs (host, 2*count int32); returns count (<= cap). */
 int wf_tb_tile_shapes(int32_t *out, int32_t cap);

@@ -1,0 +1,147 @@
+"""Batched multi-run driver vs the oracle, run by run, bit for bit.
+
+Runs of different grid sizes, media, sources and receivers advance together
+in one launch sequence; each run must equal its own standalone oracle run
+(alone-vs-in-batch identity). Also: a diverging run is isolated, graph
+replays equal eager stepping, and ``run_mimo`` equals the oracle per TX.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from cases import assert_bitwise, waveform_table
+from oracle.fdtd2d_ref import OracleDivergedError, OracleFDTD2D, OracleSource, stable_dt
+
+pytestmark = pytest.mark.gpu
+
+DX = 1e-3
+
+
+def _media(nx, ny, seed, dt):
+    g = torch.Generator().manual_seed(seed)
+    eps = 1.0 + 40.0 * torch.rand((nx, ny), generator=g, dtype=torch.float64)
+    sig = 1.5 * torch.rand((nx, ny), generator=g, dtype=torch.float64)
+    alpha = sig * dt / (2.0 * 8.8541878128e-12 * eps)
+    ca = ((1.0 - alpha) / (1.0 + alpha)).float()
+    cb = ((dt / (8.8541878128e-12 * eps)) / (1.0 + alpha)).float()
+    return ca, cb
+
+
+def _specs(device, steps, dt, sizes=((37, 41), (64, 64), (100, 53), (20, 90), (150, 150)), amps=None):
+    from waveforge_amd.mimo import RunSpec
+
+    specs, oracles = [], []
+    for k, (nx, ny) in enumerate(sizes):
+        ca, cb = _media(nx, ny, k, dt)
+        kinds = ["gauss", "dgauss", "sin"]
+        a0 = 1.0 if amps is None else amps[k]
+        src = [("Hz", nx // 2, ny // 3, kinds[k % 3], a0), ("Ey", 2, ny - 3, kinds[(k + 1) % 3], 0.5 * a0)]
+        rx = [(1, 1), (nx - 2, ny // 2), (nx // 2, ny // 3)]
+        waves = {kd: waveform_table(kd, steps + 4, dt) for kd in kinds}
+        specs.append(RunSpec(nx, ny, ca.to(device), cb.to(device),
+                             sources=[(c, i, j, waves[w].to(device), a) for c, i, j, w, a in src],
+                             receivers=rx, n_record=steps))
+        osrc = [OracleSource(c, waves[w], torch.tensor([i]), torch.tensor([j]), torch.tensor([a]))
+                for c, i, j, w, a in src]
+        oracles.append(OracleFDTD2D(nx, ny, DX, DX, dt, Ca=ca.unsqueeze(2), Cb=cb.unsqueeze(2), sources=osrc,
+                                    detectors=rx, n_check=10 ** 9))
+    return specs, oracles
+
+
+@pytest.mark.parametrize("depth,tile", [(6, 32), (4, 16), (8, 48), (3, 32)])
+def test_batched_runs_each_equal_standalone_oracle(device, depth, tile):
+    from waveforge_amd.mimo import BatchedRuns
+
+    steps = 70
+    dt = stable_dt(DX, DX)
+    specs, oracles = _specs(device, steps, dt)
+    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=depth, tile_rows=tile, n_check=25)
+    br.advance(30)
+    br.advance(steps - 30)
+    st = br.status()
+    assert not any(st["diverged"]) and st["checks"] == 2
+    for k, o in enumerate(oracles):
+        print(f"[batched] run={k} grid={o.Nx}x{o.Ny} depth={depth} tile={tile}", flush=True)
+        o.run(steps)
+        ex, ey, hz = br.fields(k)
+        assert_bitwise(ex, o.Ex.reshape(o.Nx, o.Ny), f"run {k} Ex")
+        assert_bitwise(ey, o.Ey.reshape(o.Nx, o.Ny), f"run {k} Ey")
+        assert_bitwise(hz, o.Hz.reshape(o.Nx, o.Ny), f"run {k} Hz")
+        assert_bitwise(br.traces(k), o.trace_array(), f"run {k} traces")
+
+
+def test_diverging_run_is_isolated(device):
+    from waveforge_amd.mimo import BatchedRuns
+
+    steps = 60
+    dt = stable_dt(DX, DX)
+    specs, oracles = _specs(device, steps, dt, sizes=((30, 30), (40, 35), (25, 50)), amps=(1e-3, 1e6, 1e-3))
+    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=5, n_check=10,
+                     stability_threshold=10.0)
+    br.advance(steps)
+    st = br.status()
+    for k, o in enumerate(oracles):
+        o.n_check, o.threshold = 10, 10.0
+        diverged = False
+        try:
+            o.run(steps)
+        except OracleDivergedError:
+            diverged = True
+        assert st["diverged"][k] == diverged, k
+        ex, ey, hz = br.fields(k)
+        # a diverged run keeps the fields of its failing check step
+        assert_bitwise(hz, o.Hz.reshape(o.Nx, o.Ny), f"run {k} Hz")
+        assert_bitwise(ex, o.Ex.reshape(o.Nx, o.Ny), f"run {k} Ex")
+    assert st["diverged"] == [False, True, False]
+
+
+def test_graph_replay_equals_eager(device):
+    """A captured 24-step chunk replayed several times, with eager launches and
+    synchronisations between replays, equals eager stepping after every replay."""
+    from waveforge_amd.mimo import BatchedRuns
+
+    dt = stable_dt(DX, DX)
+    steps = 24 * 4 + 12
+    specs_a, _ = _specs(device, steps, dt, sizes=((40, 33), (64, 70)))
+    specs_b, _ = _specs(device, steps, dt, sizes=((40, 33), (64, 70)))
+    a = BatchedRuns(specs_a, dx=DX, dy=DX, dt=dt, device=device, depth=6, n_check=10 ** 9)
+    b = BatchedRuns(specs_b, dx=DX, dy=DX, dt=dt, device=device, depth=6, n_check=10 ** 9)
+    a.capture(24)
+    a.replay(1)
+    b.advance(24)
+    a.advance(12)  # eager work between replays
+    b.advance(12)
+    torch.cuda.synchronize()
+    for r in range(3):
+        a.replay(1)
+        torch.cuda.synchronize()
+        b.advance(24)
+        for k in range(2):
+            for fa, fb, nm in zip(a.fields(k), b.fields(k), ("Ex", "Ey", "Hz")):
+                assert torch.equal(fa, fb), f"replay {r} run {k} {nm}"
+            assert torch.equal(a.traces(k), b.traces(k)), f"replay {r} run {k} traces"
+    assert a.steps_completed == b.steps_completed == steps
+
+
+def test_run_mimo_matches_oracle_per_tx(device):
+    from waveforge_amd import core
+    from waveforge_amd.mimo import run_mimo
+
+    n, steps = 60, 150
+    g = core.YeeGrid(n, n, DX, DX, device=device)
+    mm = core.MaterialMap(g, default=core.Material("coupling", 20.0, 0.1))
+    mm.add_circle((30, 30), 20, core.Material("brain", 45.0, 0.8))
+    Ca, Cb = mm.build()
+    ants = core.get_antenna_positions(6, 26, (30, 30), g)
+    S = run_mimo(g, Ca, Cb, ants, steps)
+    assert S.shape == (6, 6, steps) and S.dtype == np.float32
+    wave = core.RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5e-9).build(steps, g.dt, torch.device("cpu"))
+    for tx, (i, j) in enumerate(ants):
+        o = OracleFDTD2D(n, n, DX, DX, g.dt, Ca=Ca.cpu(), Cb=Cb.cpu(),
+                         sources=[OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0]))],
+                         detectors=ants, n_check=500)
+        o.run(steps)
+        np.testing.assert_array_equal(S[tx], o.trace_array().numpy())

@@ -25,6 +25,7 @@ EXPORTED_SYMBOLS = (
     "wf_fdtd2d_maxabs",
     "wf_fdtd2d_tb_advance",
     "wf_fdtd2d_judge",
+    "wf_runs_add_step",
     "wf_tb_tile_shapes",
     "wf_div_verify",
     "wf_das2d",
@@ -133,6 +134,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_tb_tile_shapes.restype = ctypes.c_int
         lib.wf_fdtd2d_judge.argtypes = [_vp, _i32, _f64, _i32, _vp, _vp]
         lib.wf_fdtd2d_judge.restype = ctypes.c_int
+        lib.wf_runs_add_step.argtypes = [_vp, _i32, _i32, _vp]
+        lib.wf_runs_add_step.restype = ctypes.c_int
         lib.wf_div_verify.argtypes = [_f32, _f32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]
         lib.wf_div_verify.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]

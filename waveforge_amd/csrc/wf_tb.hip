@@ -101,6 +101,21 @@ extern "C" int wf_fdtd2d_judge(const uint32_t *maxabs, int32_t n_runs, double th
     return wf::check_launch("wf_fdtd2d_judge");
 }
 
+namespace {
+__global__ void k_add_step(wf_run2d *runs, int n_runs, int delta) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n_runs; r += gridDim.x * blockDim.x) runs[r].step += delta;
+}
+}  // namespace
+
+extern "C" int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, hipStream_t stream) {
+    if (!runs || n_runs < 1) {
+        wf::set_error("wf_runs_add_step: bad arguments");
+        return WF_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_add_step, dim3((n_runs + 255) / 256), dim3(256), 0, stream, runs, n_runs, delta);
+    return wf::check_launch("wf_runs_add_step");
+}
+
 extern "C" int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, hipStream_t stream) {
     if (!bad || !(d > 0.0f) || span == 0u || lo + span < lo || lo + span > 0x7f800000u) {
         wf::set_error("wf_div_verify: bad arguments");

@@ -81,6 +81,18 @@ class StepCoefficients:
         return cls(f32(dt / MU0), f32(dt / EPS0), f32(dx), f32(dy), f32(cmx), f32(cmy))
 
 
+class CellSource:
+    """Minimal soft source ``field[i, j] += wave[n] * amp`` for drivers that do
+    not build reference-style source objects (same attributes the table reads)."""
+
+    def __init__(self, component: str, wave: torch.Tensor, i: int, j: int, amp: float = 1.0) -> None:
+        self.component = component
+        self._waveform_tensor = wave
+        self._i_idx = torch.tensor([int(i)])
+        self._j_idx = torch.tensor([int(j)])
+        self._amp = torch.tensor([float(amp)], dtype=torch.float32)
+
+
 class SourceTable:
     """Flattened soft-source contributions of a source list, on the device.
 
@@ -441,3 +453,11 @@ def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optiona
         else:
             return None, checks
     return [tuple(x) for x in launches], checks
+
+
+def runs_add_step(run_table: torch.Tensor, n_runs: int, delta: int, stream: int) -> None:
+    """Add ``delta`` to every device run descriptor's ``step`` (``wf_runs_add_step``);
+    lets a captured step loop be replayed for later step ranges."""
+    lib = _native.load()
+    _native.check(lib.wf_runs_add_step(run_table.data_ptr(), int(n_runs), int(delta), ctypes.c_void_p(stream)),
+                  "wf_runs_add_step")

@@ -1,0 +1,239 @@
+"""Batched multi-run driver: many independent 2D TM runs in one launch.
+
+The reference images a phantom by looping over transmitters and running one
+``FDTD2D`` per TX with a per-step Python loop and a detector gather
+(``run_mimo``, ``examples/brain_mimo_imaging.py:149-204``). ``BatchedRuns``
+advances any number of runs — different grid sizes, materials, sources and
+receivers — together: every launch of the temporally blocked kernel covers
+the tiles of all runs, so small grids fill the GPU and the per-step launch
+overhead disappears. Each run behaves exactly as if it were stepped alone
+(the kernel reads only that run's descriptor and tiles), which the tests
+check bit for bit against the oracle.
+
+Stability checks are judged on the device per run every ``n_check`` steps:
+a diverged run is flagged and carried through unchanged while the others
+continue (failure isolation), and ``status()`` reports it.
+
+``use_graph=True`` captures ``chunk`` steps of launches (ending with a
+kernel that advances the runs' step index) in a HIP graph through
+``torch.cuda.CUDAGraph`` and replays it.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native
+from .engine import (
+    CellSource,
+    RunState,
+    SourceTable,
+    StepCoefficients,
+    TBPlan,
+    judge,
+    runs_add_step,
+    schedule_launches,
+    tb_limits_ok,
+)
+
+
+@dataclass
+class RunSpec:
+    """One run: grid size, optional per-cell Ca/Cb ``(nx, ny)`` on the device,
+    soft point sources ``(component, i, j, wave, amp)`` and Hz receivers."""
+
+    nx: int
+    ny: int
+    ca: Optional[torch.Tensor]
+    cb: Optional[torch.Tensor]
+    sources: Sequence[tuple] = ()
+    receivers: Sequence[tuple[int, int]] = ()
+    n_record: int = 0
+    tag: object = None
+
+
+class BatchedRuns:
+    """Device state and launch plan of a set of independent runs."""
+
+    def __init__(self, specs: Sequence[RunSpec], *, dx: float, dy: float, dt: float, device=None,
+                 tile_rows: int = 32, depth: int = 6, n_check: int = 500, stability_threshold: float = 1e10,
+                 fast_div: bool = True) -> None:
+        if not specs:
+            raise ValueError("no runs")
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        if self.device.type != "cuda":
+            raise RuntimeError("waveforge_amd: batched runs need the ROCm GPU; there is no CPU path")
+        _native.load()
+        self.specs = list(specs)
+        self.dt, self.dx, self.dy = dt, dx, dy
+        self.tile_rows, self.depth = tile_rows, depth
+        self.n_check, self.threshold = n_check, stability_threshold
+        coef = StepCoefficients.from_grid(dt, dx, dy)
+        self.runs: list[RunState] = []
+        for k, sp in enumerate(self.specs):
+            dev = self.device
+            ex, ey, hz, ex2, ey2, hz2 = (torch.zeros((sp.nx, sp.ny), dtype=torch.float32, device=dev)
+                                         for _ in range(6))
+            srcs = [CellSource(c, w, i, j, a) for (c, i, j, w, a) in sp.sources]
+            table = SourceTable(srcs, sp.nx, sp.ny, dev)
+            rs = RunState(ex, ey, hz, sp.ca, sp.cb, sp.nx, sp.ny, coef, sources=table)
+            rs.second = (ex2, ey2, hz2)
+            if sp.receivers:
+                cells = [i * sp.ny + j for i, j in sp.receivers]
+                rs.det_cell = torch.tensor(cells, dtype=torch.int32, device=dev)
+                rs.traces = torch.zeros((len(cells), max(1, sp.n_record)), dtype=torch.float32, device=dev)
+                rs.trace_n0 = 0
+            rs.step = 0
+            self.runs.append(rs)
+            for d in range(1, depth + 1):
+                if not tb_limits_ok(rs, tile_rows, d):
+                    raise ValueError(f"run {k}: too many sources or receivers per tile for the batched kernel")
+        self.plan = TBPlan(self.runs, tile_rows, self.device, fast_div=fast_div)
+        self.steps_completed = 0
+        self._dev_step = 0  # value of every descriptor's device-side `step` field
+        self.stop = torch.zeros(len(self.runs), dtype=torch.int32, device=self.device)
+        self.check_log: list[tuple[int, torch.Tensor]] = []
+        self._graph = None
+
+    # ------------------------------------------------------------------
+    @property
+    def n_cells(self) -> int:
+        return sum(r.nx * r.ny for r in self.runs)
+
+    def traces(self, k: int) -> torch.Tensor:
+        return self.runs[k].traces
+
+    def fields(self, k: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        r = self.runs[k]
+        return r.ex, r.ey, r.hz
+
+    def _stream(self) -> int:
+        return _native.stream_handle(self.device)
+
+    def _check_waves(self, n_steps: int) -> None:
+        end = self.steps_completed + n_steps
+        for k, r in enumerate(self.runs):
+            t = r.sources
+            if t is not None and t.min_wave_len is not None and end > t.min_wave_len:
+                raise RuntimeError(f"run {k}: source waveform has {t.min_wave_len} samples, cannot reach step {end}")
+
+    def advance(self, n_steps: int) -> None:
+        """Enqueue ``n_steps`` steps of every run (asynchronous)."""
+        if n_steps <= 0:
+            return
+        self._check_waves(n_steps)
+        launches, checks = schedule_launches(self.steps_completed, n_steps, self.depth, self.n_check)
+        if launches is None:
+            raise ValueError("batched advance needs an even launch split (use n_steps >= 2 or depth >= 2)")
+        stream = self._stream()
+        slots = None
+        if checks:
+            slots = torch.zeros((len(checks), len(self.runs), 3), dtype=torch.int32, device=self.device)
+        offset = self.steps_completed - self._dev_step
+        for k, (d, slot) in enumerate(launches):
+            self.plan.launch(d, k % 2, offset, stream, maxabs_out=slots[slot] if slot >= 0 else None,
+                             stop=self.stop)
+            if slot >= 0:
+                judge(slots[slot], self.threshold, self.stop, False, stream)
+            offset += d
+        for m, step in enumerate(checks):
+            self.check_log.append((step, slots[m]))
+        self.steps_completed += n_steps
+
+    # ------------------------------------------------------------------
+    def capture(self, chunk: int) -> None:
+        """Capture ``chunk`` steps (no stability checks inside) in a HIP graph."""
+        launches, _ = schedule_launches(0, chunk, self.depth, None)
+        if launches is None:
+            raise ValueError("graph chunk needs an even launch split")
+        for d, _ in launches:
+            self.plan.tiles(d)  # upload tile tables before capture
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(s):
+            # warm the launch path (table uploads) outside the capture
+            torch.cuda.synchronize(self.device)
+            with torch.cuda.graph(g, stream=s):
+                stream = _native.stream_handle(self.device)
+                off = 0
+                for k, (d, _) in enumerate(launches):
+                    self.plan.launch(d, k % 2, off, stream, stop=self.stop)
+                    off += d
+                runs_add_step(self.plan.run_table, len(self.runs), chunk, stream)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graph = (g, chunk)
+
+    def replay(self, n_chunks: int) -> None:
+        """Replay the captured chunk ``n_chunks`` times (device step index advances)."""
+        if self._graph is None:
+            raise RuntimeError("capture() first")
+        g, chunk = self._graph
+        self._check_waves(n_chunks * chunk)
+        if self.steps_completed != self._dev_step:
+            # bring the device step index up to date after eager advances
+            runs_add_step(self.plan.run_table, len(self.runs), self.steps_completed - self._dev_step, self._stream())
+            self._dev_step = self.steps_completed
+        for _ in range(n_chunks):
+            g.replay()
+        self.steps_completed += n_chunks * chunk
+        self._dev_step += n_chunks * chunk
+
+    # ------------------------------------------------------------------
+    def status(self) -> dict:
+        """Synchronise and report per-run stability: ``{"diverged": [...],
+        "checks": n, "field_max": [...]}`` (field_max is the largest checked
+        max|E|, |H| per run, as ``last_field_max``)."""
+        stop = self.stop.cpu().numpy().astype(bool)
+        fmax = np.full(len(self.runs), -1.0)
+        for _, slot in self.check_log:
+            v = slot.view(torch.float32).cpu().numpy().max(axis=1)
+            fmax = np.maximum(fmax, np.where(np.isnan(v), -1.0, v))
+        return {"diverged": stop.tolist(), "checks": len(self.check_log), "field_max": fmax.tolist()}
+
+
+# ---------------------------------------------------------------------------
+# MIMO imaging on top of the batched runs
+# ---------------------------------------------------------------------------
+
+
+def mimo_specs(ca: torch.Tensor, cb: torch.Tensor, antennas, wave: torch.Tensor, n_steps: int, tag=None):
+    """One RunSpec per TX antenna: Hz point source ``wave`` at the TX, all
+    antennas record Hz (``examples/brain_mimo_imaging.py:176-201``)."""
+    nx, ny = ca.shape[0], ca.shape[1]
+    ca2, cb2 = ca.reshape(nx, ny), cb.reshape(nx, ny)
+    return [RunSpec(nx, ny, ca2, cb2, sources=[("Hz", i, j, wave, 1.0)], receivers=list(antennas),
+                    n_record=n_steps, tag=(tag, t)) for t, (i, j) in enumerate(antennas)]
+
+
+def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, depth: int = 6,
+             tile_rows: int = 32, n_check: int = 500, return_tensor: bool = False):
+    """All TX runs of one array in one batched launch sequence.
+
+    Same inputs and output as the reference's ``run_mimo``: returns the
+    ``(N_tx, N_rx, n_steps)`` float32 signals (NumPy unless
+    ``return_tensor``). The default waveform is the reference's 1 GHz Ricker
+    with ``t0 = 1.5/f`` built for ``n_steps`` samples.
+    """
+    from .core.sources import RickerWavelet
+
+    if waveform is None:
+        waveform = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9)
+    wave = waveform.build(n_steps, grid.dt, grid.device, torch.float32)
+    specs = mimo_specs(Ca, Cb, antenna_positions, wave, n_steps)
+    br = BatchedRuns(specs, dx=grid.dx, dy=grid.dy, dt=grid.dt, device=grid.device, tile_rows=tile_rows,
+                     depth=depth, n_check=n_check)
+    br.advance(n_steps)
+    st = br.status()
+    if any(st["diverged"]):
+        from .core.fdtd2d import SimulationDivergedError
+
+        bad = [k for k, v in enumerate(st["diverged"]) if v]
+        raise SimulationDivergedError(f"TX runs {bad} diverged")
+    out = torch.stack([br.traces(k) for k in range(len(specs))])
+    return out if return_tensor else out.cpu().numpy()
