@@ -237,3 +237,52 @@ def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, de
         raise SimulationDivergedError(f"TX runs {bad} diverged")
     out = torch.stack([br.traces(k) for k in range(len(specs))])
     return out if return_tensor else out.cpu().numpy()
+
+
+def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, device, depth: int = 6,
+                   tile_rows: int = 32, n_check: int = 500, v_medium: float | None = None, gate_start: int = 0,
+                   use_graph: bool = False, chunk: int = 240):
+    """Batched MIMO imaging of many phantoms: all TX runs of all phantoms in
+    one batched launch sequence, then one DAS image per phantom (raw traces).
+
+    ``phantoms`` is a list of ``(eps_r, sigma, antennas)`` (NumPy maps in
+    cells, antenna cells). Returns ``(images, traces, batched_runs)`` with
+    ``images[p]`` a device fp64 ``(n, n)`` image and ``traces[p]`` the
+    ``(N_tx, N_rx, n_steps)`` device signals.
+    """
+    from .core.imaging import run_das
+    from .core.materials import coefficients_from_maps
+    from .core.sources import RickerWavelet
+
+    wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(n_steps, dt, device, torch.float32)
+    specs, owners = [], []
+    for p, (eps, sig, ants) in enumerate(phantoms):
+        ca, cb = coefficients_from_maps(eps, sig, dt, device=device)
+        sp = mimo_specs(ca, cb, ants, wave, n_steps, tag=p)
+        specs += sp
+        owners += [p] * len(sp)
+    br = BatchedRuns(specs, dx=dx, dy=dx, dt=dt, device=device, tile_rows=tile_rows, depth=depth,
+                     n_check=n_check)
+    if use_graph and n_steps >= chunk:
+        br.capture(chunk)
+        br.replay(n_steps // chunk)
+        br.advance(n_steps - (n_steps // chunk) * chunk)
+    else:
+        br.advance(n_steps)
+    images, traces = [], []
+    k = 0
+    if v_medium is None:
+        v_medium = 299_792_458.0 / math.sqrt(40.0)
+    for p, (eps, sig, ants) in enumerate(phantoms):
+        n_ant = len(ants)
+        S = torch.stack([br.traces(k + t) for t in range(n_ant)])
+        k += n_ant
+        nx, ny = eps.shape
+        ax = torch.tensor([a[0] * dx for a in ants], dtype=torch.float64, device=device)
+        ay = torch.tensor([a[1] * dx for a in ants], dtype=torch.float64, device=device)
+        img = torch.empty((nx, ny), dtype=torch.float64, device=device)
+        run_das(S, ax, ay, img, pitch_x=nx * dx / nx, pitch_y=ny * dx / ny, v_medium=v_medium, dt=dt,
+                gate_start=gate_start, clip=True)
+        images.append(img)
+        traces.append(S)
+    return images, traces, br

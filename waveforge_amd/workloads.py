@@ -181,3 +181,18 @@ def head_phantom(n: int, n_elements: int = 16, *, seed: int = 0, clot: bool = Tr
 def ricker_1ghz() -> RickerWavelet:
     """1 GHz Ricker pulse with t0 = 1.5/f (``examples/brain_mimo_imaging.py:184-185``)."""
     return RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9)
+
+
+def phantom_set(n_phantoms: int = 32, sizes=(150, 200, 256, 384), n_elements: int = 16, seed: int = 0):
+    """Seeded head phantoms cycling through ``sizes`` (8 of each for 32): the
+    batched-imaging workload. Returns ``[(eps_r, sigma, antennas), ...]``."""
+    out = []
+    for p in range(n_phantoms):
+        ph = head_phantom(sizes[p % len(sizes)], n_elements, seed=seed * 1000 + p, clot=True)
+        out.append((ph.eps, ph.sigma, ph.antennas))
+    return out
+
+
+def imaging_dt(dx: float = 1e-3) -> float:
+    """dt = 0.99 dx / (c sqrt 2), the time step of the imaging workloads."""
+    return 0.99 * dx / (C0 * math.sqrt(2.0))
