@@ -50,6 +50,9 @@ def parse():
     p.add_argument("--tile", type=int, default=None, help="tile rows (tile columns are 64 - 2*depth)")
     p.add_argument("--cpu-steps", type=int, default=8, help="timed steps of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-imaging", action="store_true", help="skip the batched-imaging section")
+    p.add_argument("--no-serving", action="store_true", help="skip the serving section")
+    p.add_argument("--serving-requests", type=int, default=48)
     p.add_argument("--copy-probe", action="store_true",
                    help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
                         "traffic used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -95,6 +98,121 @@ def cpu_baseline(wl, n_steps: int) -> dict:
         "kind": "port",
         "sample": f"{g.Nx}x{g.Ny} blob medium, steps 3-{2 + n_steps} ({n_steps} timed steps) of the same run, "
                   f"oracle torch-CPU restatement, {torch.get_num_threads()} threads",
+    }
+
+
+def imaging_suite(dev) -> dict:
+    """Batched MIMO imaging: 32 seeded head phantoms (8 each of 150^2, 200^2,
+    256^2, 384^2), 16 TX x 16 RX, 2000 steps, all 512 runs in one batched
+    launch sequence (HIP-graph replay of 240-step chunks), then one DAS image
+    per phantom. Compared with the one-run-per-launch GPU paths on the same
+    runs: FDTD2D.run per TX (temporally blocked, one run per launch) and
+    FDTD2D.step per TX (three launches per step), each timed on a sample of
+    2 runs per grid size and scaled to all 512 runs."""
+    import torch
+
+    from waveforge_amd import core, mimo, workloads
+
+    dt = workloads.imaging_dt()
+    steps = 2000
+    phantoms = workloads.phantom_set(32)
+    mimo.image_phantoms(phantoms[:4], 240, dt=dt, device=dev)  # warm-up (division check, code objects)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    images, traces, br = mimo.image_phantoms(phantoms, steps, dt=dt, device=dev, use_graph=True, chunk=240)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t0
+    cu = br.n_cells * steps
+    st = br.status()
+    # one-run-per-launch GPU paths on a sample, and alone == in-batch on the sampled runs
+    run_s, step_s, same = [], [], True
+    wave = core.RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5e-9)
+    for p in range(4):
+        eps, sig, ants = phantoms[p]
+        n = eps.shape[0]
+        g = core.YeeGrid(n, n, 1e-3, 1e-3, courant=0.99, device=dev)
+        Ca, Cb = core.coefficients_from_maps(eps, sig, dt, device=dev)
+        for tx in range(2):
+            for mode in ("run", "step"):
+                f = core.FieldSet(g)
+                det = core.DetectorSet(ants, steps, grid=g)
+                src = core.PointSource(wave, ants[tx][0], ants[tx][1], "Hz", grid=g, N_steps=steps)
+                sim = core.FDTD2D(g, f, core.MurABC(g, f.Hz), core.SourceCollection([src]), Ca=Ca, Cb=Cb,
+                                  detectors=det, n_check=500)
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                if mode == "run":
+                    sim.run(steps)
+                else:
+                    for _ in range(steps):
+                        sim.step()
+                torch.cuda.synchronize()
+                el = time.perf_counter() - t1
+                (run_s if mode == "run" else step_s).append((n * n, el))
+                same = same and torch.equal(det.traces, traces[p][tx])
+    def projected(samples):
+        per_size = {}
+        for cells, el in samples:
+            per_size.setdefault(cells, []).append(el)
+        return sum(sum(v) / len(v) for v in per_size.values()) * 8 * 16  # 8 phantoms per size, 16 TX each
+    plain_run = projected(run_s)
+    plain_step = projected(step_s)
+    peak = [tuple(int(x) for x in divmod(int(img.argmax()), img.shape[1])) for img in images[:4]]
+    return {
+        "workload": "32 head phantoms (8 x {150,200,256,384}^2), 16 TX x 16 RX, 2000 steps, 512 runs + 32 DAS images",
+        "cell_updates": cu,
+        "batched_s": round(batched_s, 4),
+        "batched_mcell_updates_per_s": round(cu / batched_s / 1e6, 1),
+        "images_per_s": round(32 / batched_s, 2),
+        "per_image_latency_ms": round(batched_s * 1e3, 1),
+        "plain_run_per_tx_projected_s": round(plain_run, 3),
+        "plain_step_per_tx_projected_s": round(plain_step, 3),
+        "speedup_vs_run_per_tx": round(plain_run / batched_s, 2),
+        "speedup_vs_step_per_tx": round(plain_step / batched_s, 2),
+        "diverged_runs": int(sum(st["diverged"])),
+        "alone_equals_in_batch": bool(same),
+        "sampled_runs": len(run_s),
+        "peak_pixels_first4": peak,
+    }
+
+
+def serving_suite(dev, n_requests: int) -> dict:
+    """Continuous-batching server on a Poisson stream of imaging requests
+    (grid log-uniform in 128..512, E in {8,16,32}, steps = 8 x diagonal),
+    arrival rate set for ~85 % of the batched server's measured capacity; the
+    same stream served one request at a time for comparison."""
+    import numpy as np
+    import torch
+
+    from waveforge_amd import serving, workloads
+
+    dt = workloads.imaging_dt()
+    reqs = serving.poisson_requests(n_requests, 1.0, n_min=128, n_max=512, seed=7)
+    work = sum(r.cell_updates for r in reqs)
+    capacity = 2.0e11  # cell-updates/s of the batched kernel on this mix (conservative)
+    span = work / capacity / 0.85
+    last = reqs[-1].arrival
+    for r in reqs:
+        r.arrival *= span / last
+    warm = serving.poisson_requests(2, 100.0, n_min=64, n_max=96, seed=1)
+    serving.ImagingServer(dt=dt, device=dev).serve(warm, realtime=False)
+    torch.cuda.synchronize()
+    srv = serving.ImagingServer(dt=dt, device=dev, hbm_budget=96e9, quantum=96)
+    st = srv.serve(reqs, realtime=True)
+    one = serving.ImagingServer(dt=dt, device=dev, hbm_budget=96e9, quantum=96, max_requests=1)
+    reqs1 = serving.poisson_requests(n_requests, 1.0, n_min=128, n_max=512, seed=7)
+    for r in reqs1:
+        r.arrival *= span / last
+    st1 = one.serve(reqs1, realtime=True)
+    same = all(torch.equal(a.image, b.image) for a, b in zip(sorted(srv.done, key=lambda r: r.rid),
+                                                              sorted(one.done, key=lambda r: r.rid)))
+    return {
+        "workload": f"{n_requests} Poisson requests, grid 128..512 log-uniform, E in {{8,16,32}}, steps = 8*diag",
+        "offered_load": 0.85,
+        "continuous": st,
+        "one_at_a_time": st1,
+        "throughput_ratio": round(st["mcell_updates_per_s"] / st1["mcell_updates_per_s"], 2),
+        "images_equal_alone_vs_mixed": bool(same),
     }
 
 
@@ -174,6 +292,13 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
         base = cpu_baseline(wl, args.cpu_steps)
+    imaging = serving_res = None
+    if not args.no_imaging:
+        print("[bench] batched imaging section", file=sys.stderr, flush=True)
+        imaging = imaging_suite(dev)
+    if not args.no_serving:
+        print("[bench] serving section", file=sys.stderr, flush=True)
+        serving_res = serving_suite(dev, args.serving_requests)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -198,6 +323,8 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": base,
         "fields_finite": finite,
+        "imaging": imaging,
+        "serving": serving_res,
     }
     print(json.dumps(out))
     return 0

@@ -1,0 +1,305 @@
+"""Continuous-batching imaging server.
+
+A request is one MIMO imaging job: a phantom (eps_r, sigma maps), an array of
+E antennas (every element transmits once, all receive Hz) and a step count;
+its answer is the DAS image of its E x E x T traces. The reference serves
+nothing: it images one phantom at a time with a sequential per-TX Python
+loop (``examples/brain_mimo_imaging.py:149-204, 211-266``).
+
+The server keeps every admitted request's field state on the device and
+advances all active runs together, one quantum of steps per scheduling round,
+through the temporally blocked multi-run kernel: requests that arrive while
+others are on the GPU join at the next round once their HBM footprint fits
+the budget (and leave as soon as their runs reach their step count), so work
+of different requests overlaps inside the same launches. Each run reads only
+its own descriptor and tiles, so a request's result does not depend on what
+else is in flight (the tests check alone == in the mix, bit for bit).
+Stability is judged on the device per run at every round; a diverged request
+is reported failed without stopping the others.
+"""
+
+from __future__ import annotations
+
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native
+from .core.imaging import run_das
+from .core.materials import coefficients_from_maps
+from .core.sources import RickerWavelet
+from .engine import (
+    CellSource,
+    RunState,
+    SourceTable,
+    StepCoefficients,
+    TBPlan,
+    judge,
+    runs_add_step,
+    tb_limits_ok,
+)
+
+C0 = 299_792_458.0
+
+
+@dataclass
+class ImagingRequest:
+    """One imaging job. ``arrival`` is in seconds from the start of the stream."""
+
+    rid: int
+    arrival: float
+    eps: np.ndarray
+    sigma: np.ndarray
+    antennas: Sequence[tuple[int, int]]
+    n_steps: int
+    # filled in by the server
+    admitted: Optional[float] = None
+    finished: Optional[float] = None
+    image: Optional[torch.Tensor] = None
+    traces: Optional[torch.Tensor] = None
+    failed: bool = False
+    _event: object = None
+    _runs: list = field(default_factory=list)
+    _bytes: int = 0
+
+    @property
+    def latency(self) -> Optional[float]:
+        return None if self.finished is None else self.finished - self.arrival
+
+    @property
+    def cell_updates(self) -> int:
+        n = self.eps.shape[0] * self.eps.shape[1]
+        return n * self.n_steps * len(self.antennas)
+
+
+def request_footprint(nx: int, ny: int, n_ant: int, n_steps: int, quantum: int) -> int:
+    """Device bytes a request holds while active: two field states per TX run,
+    shared Ca/Cb, the traces, the waveform and the image."""
+    cells = nx * ny
+    return (6 * cells * n_ant + 2 * cells) * 4 + n_ant * n_ant * n_steps * 4 + (n_steps + quantum) * 4 + cells * 8
+
+
+class ImagingServer:
+    """Admit, co-schedule and retire imaging requests on one GPU."""
+
+    def __init__(self, *, dt: float, dx: float = 1e-3, device=None, hbm_budget: float = 64e9, quantum: int = 96,
+                 depth: int = 6, tile_rows: int = 32, max_requests: Optional[int] = None,
+                 stability_threshold: float = 1e10, v_medium: Optional[float] = None) -> None:
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        if self.device.type != "cuda":
+            raise RuntimeError("waveforge_amd: the imaging server needs the ROCm GPU; there is no CPU path")
+        _native.load()
+        self.dt, self.dx = dt, dx
+        self.budget = hbm_budget
+        self.quantum = quantum
+        self.depth, self.tile_rows = depth, tile_rows
+        self.max_requests = max_requests
+        self.threshold = stability_threshold
+        self.v_medium = v_medium if v_medium is not None else C0 / math.sqrt(40.0)
+        self.coef = StepCoefficients.from_grid(dt, dx, dx)
+        self.active: list[ImagingRequest] = []
+        self.retiring: list[ImagingRequest] = []
+        self.done: list[ImagingRequest] = []
+        self.used = 0
+        self._plan: Optional[TBPlan] = None
+        self._plan_runs: list = []
+        self._stop: Optional[torch.Tensor] = None
+        self.rounds = 0
+        if quantum % 2 or quantum < 2 * depth:
+            raise ValueError("quantum must be even and at least 2*depth steps")
+
+    # ------------------------------------------------------------------
+    def footprint(self, req: ImagingRequest) -> int:
+        nx, ny = req.eps.shape
+        return request_footprint(nx, ny, len(req.antennas), req.n_steps, self.quantum)
+
+    def can_admit(self, req: ImagingRequest) -> bool:
+        if self.max_requests is not None and len(self.active) >= self.max_requests:
+            return False
+        fp = self.footprint(req)
+        if fp > self.budget:
+            raise ValueError(f"request {req.rid} needs {fp/1e9:.2f} GB, above the {self.budget/1e9:.2f} GB budget")
+        return self.used + fp <= self.budget
+
+    def admit(self, req: ImagingRequest, now: float) -> None:
+        dev = self.device
+        nx, ny = req.eps.shape
+        ca, cb = coefficients_from_maps(req.eps, req.sigma, self.dt, device=dev)
+        ca, cb = ca.reshape(nx, ny), cb.reshape(nx, ny)
+        wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(
+            req.n_steps + self.quantum, self.dt, dev, torch.float32)
+        cells = torch.tensor([i * ny + j for i, j in req.antennas], dtype=torch.int32, device=dev)
+        n_ant = len(req.antennas)
+        req.traces = torch.zeros((n_ant, n_ant, req.n_steps), dtype=torch.float32, device=dev)
+        runs = []
+        for t, (i, j) in enumerate(req.antennas):
+            f = [torch.zeros((nx, ny), dtype=torch.float32, device=dev) for _ in range(6)]
+            table = SourceTable([CellSource("Hz", wave, i, j, 1.0)], nx, ny, dev)
+            rs = RunState(f[0], f[1], f[2], ca, cb, nx, ny, self.coef, sources=table, det_cell=cells,
+                          traces=req.traces[t], trace_n0=0, step=0)
+            rs.second = (f[3], f[4], f[5])
+            rs.extra["req"] = req
+            rs.extra["stop"] = 0
+            for d in range(1, self.depth + 1):
+                if not tb_limits_ok(rs, self.tile_rows, d):
+                    raise ValueError(f"request {req.rid}: receivers too dense for the batched kernel")
+            runs.append(rs)
+        req._runs = runs
+        req._bytes = self.footprint(req)
+        req.admitted = now
+        self.used += req._bytes
+        self._sync_flags()
+        self.active.append(req)
+        self._plan = None
+
+    # ------------------------------------------------------------------
+    def _sync_flags(self) -> None:
+        """Copy the device stop flags of the current table back to the runs."""
+        if self._stop is not None and self._plan_runs:
+            for r, fl in zip(self._plan_runs, self._stop.cpu().tolist()):
+                r.extra["stop"] = fl
+
+    def _build_plan(self) -> None:
+        self._plan_runs = [r for req in self.active for r in req._runs]
+        if not self._plan_runs:
+            self._plan = self._stop = None
+            return
+        self._plan = TBPlan(self._plan_runs, self.tile_rows, self.device)
+        self._stop = torch.tensor([r.extra["stop"] for r in self._plan_runs], dtype=torch.int32,
+                                  device=self.device)
+
+    def advance_round(self) -> None:
+        """Advance every active run by one quantum (asynchronous) and judge it."""
+        if not self.active:
+            return
+        if self._plan is None:
+            self._build_plan()
+        stream = _native.stream_handle(self.device)
+        q, d = self.quantum, self.depth
+        n_l = -(-q // d)
+        if n_l % 2:
+            n_l += 1
+        base, rem = divmod(q, n_l)
+        depths = [base + (1 if k < rem else 0) for k in range(n_l)]
+        slot = torch.zeros((len(self._plan_runs), 3), dtype=torch.int32, device=self.device)
+        off = 0
+        for k, dd in enumerate(depths):
+            last = k == len(depths) - 1
+            self._plan.launch(dd, k % 2, off, stream, maxabs_out=slot if last else None, stop=self._stop)
+            off += dd
+        judge(slot, self.threshold, self._stop, False, stream)
+        runs_add_step(self._plan.run_table, len(self._plan_runs), q, stream)
+        for r in self._plan_runs:
+            r.step += q
+        self.rounds += 1
+
+    def retire_finished(self) -> None:
+        """Requests whose runs reached their step count leave the active set;
+        their DAS image is enqueued and completion is tracked by an event."""
+        finished = [req for req in self.active if req._runs[0].step >= req.n_steps]
+        if not finished:
+            return
+        self._sync_flags()
+        for req in finished:
+            self._finish(req)
+        self.active = [req for req in self.active if req not in finished]
+        self._plan = None
+        self._build_plan()
+
+    def _finish(self, req: ImagingRequest) -> None:
+        dev = self.device
+        nx, ny = req.eps.shape
+        ax = torch.tensor([a[0] * self.dx for a in req.antennas], dtype=torch.float64, device=dev)
+        ay = torch.tensor([a[1] * self.dx for a in req.antennas], dtype=torch.float64, device=dev)
+        req.image = torch.empty((nx, ny), dtype=torch.float64, device=dev)
+        run_das(req.traces, ax, ay, req.image, pitch_x=self.dx, pitch_y=self.dx, v_medium=self.v_medium,
+                dt=self.dt, clip=True)
+        req.failed = any(r.extra.get("stop", 0) for r in req._runs)
+        ev = torch.cuda.Event()
+        ev.record()
+        req._event = ev
+        self.retiring.append(req)
+
+    def poll(self, now_fn) -> None:
+        """Complete retiring requests whose image is ready; free their memory."""
+        still = []
+        for req in self.retiring:
+            if req._event.query():
+                req.finished = now_fn()
+                self.used -= req._bytes
+                for r in req._runs:
+                    r.ex = r.ey = r.hz = r.second = None
+                req._runs = []
+                self.done.append(req)
+            else:
+                still.append(req)
+        self.retiring = still
+
+    # ------------------------------------------------------------------
+    def serve(self, requests: Sequence[ImagingRequest], *, realtime: bool = True) -> dict:
+        """Serve a stream of requests (sorted by arrival). With ``realtime``
+        the arrivals follow the wall clock from the call; returns latency and
+        throughput statistics."""
+        pending = sorted(requests, key=lambda r: r.arrival)
+        t0 = time.perf_counter()
+        now = lambda: time.perf_counter() - t0  # noqa: E731
+        k = 0
+        while k < len(pending) or self.active or self.retiring:
+            t = now()
+            while k < len(pending) and (pending[k].arrival <= t or not realtime) and self.can_admit(pending[k]):
+                self.admit(pending[k], t)
+                k += 1
+            if self.active:
+                self.advance_round()
+                self.retire_finished()
+                # keep the host at most a couple of rounds ahead of the device
+                if len(self.active) and self.rounds % 2 == 0:
+                    torch.cuda.current_stream(self.device).synchronize()
+            elif self.retiring:
+                torch.cuda.current_stream(self.device).synchronize()
+            elif k < len(pending):
+                wait = pending[k].arrival - now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+            self.poll(now)
+        torch.cuda.synchronize(self.device)
+        self.poll(now)
+        total = now()
+        lat = np.array([r.latency for r in self.done if r.latency is not None])
+        cu = sum(r.cell_updates for r in self.done)
+        return {
+            "requests": len(self.done),
+            "failed": int(sum(r.failed for r in self.done)),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
+            "mean_ms": float(lat.mean() * 1e3) if lat.size else None,
+            "wall_s": total,
+            "images_per_s": len(self.done) / total if total > 0 else None,
+            "mcell_updates_per_s": cu / total / 1e6 if total > 0 else None,
+            "rounds": self.rounds,
+        }
+
+
+def poisson_requests(n: int, rate_hz: float, *, n_min: int = 128, n_max: int = 1024, elements=(8, 16, 32),
+                     seed: int = 0, steps_per_diag: float = 8.0, dx: float = 1e-3) -> list[ImagingRequest]:
+    """The serving stream: Poisson arrivals at ``rate_hz``; grid size
+    log-uniform in ``[n_min, n_max]``, E from ``elements``, steps =
+    ``steps_per_diag`` x grid diagonal in cells; head phantoms with a clot."""
+    from .workloads import head_phantom
+
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate_hz)
+        n_cells = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        e = int(rng.choice(elements))
+        ph = head_phantom(n_cells, e, seed=seed * 100003 + rid, clot=True, dx=dx)
+        steps = int(math.floor(steps_per_diag * n_cells * math.sqrt(2.0)))
+        out.append(ImagingRequest(rid, t, ph.eps, ph.sigma, ph.antennas, steps))
+    return out

@@ -85,8 +85,8 @@ class Simulation:
 
 def slab_256(device="cuda", n: int = 256, n_steps: int = 1000, **sim_kw) -> Simulation:
     dx = 1e-3
-    dt = 0.99 * dx / (C0 * math.sqrt(2.0))
-    grid = YeeGrid(n, n, dx, dx, dt=dt, device=device)
+    grid = YeeGrid(n, n, dx, dx, courant=0.99, device=device)  # dt = 0.99 dx / (c sqrt 2)
+    dt = grid.dt
     fields = FieldSet(grid)
     mm = MaterialMap(grid)
     mm.add_rectangle((150, 169), (0, n - 1), Material("slab", eps_r=4.0, sigma=0.0))
@@ -194,5 +194,8 @@ def phantom_set(n_phantoms: int = 32, sizes=(150, 200, 256, 384), n_elements: in
 
 
 def imaging_dt(dx: float = 1e-3) -> float:
-    """dt = 0.99 dx / (c sqrt 2), the time step of the imaging workloads."""
-    return 0.99 * dx / (C0 * math.sqrt(2.0))
+    """dt = 0.99 dx / (c sqrt 2), the time step of the imaging workloads, computed
+    as ``YeeGrid(..., courant=0.99)`` computes it."""
+    from .core.grid import compute_stable_dt
+
+    return compute_stable_dt(dx, dx, courant=0.99)
