@@ -1,0 +1,127 @@
+"""Write the reference fixtures (run in the build container only).
+
+The reference ships no golden vectors. This script imports the reference's
+own ``core`` package and imaging example functions from /root/reference (pure
+Python) and records small input/output vectors for what the oracle's
+mathematical checks cannot pin at the bit level: the exact eager-op
+trajectory of ``FDTD2D.step`` (fields and detector traces), the ``Waveform.build``
+tables, and the DAS images of ``examples/brain_mimo_imaging.py`` and
+``examples/09_brain_clot_dataset_sample.py``. Everything runs on the CPU in
+fp32, as the reference's tests do. Output: ``tests/golden/reference_fixtures.npz``.
+"""
+
+from __future__ import annotations
+
+import importlib.util
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+import torch
+
+REF = Path("/root/reference")
+OUT = Path(__file__).resolve().parent / "reference_fixtures.npz"
+
+
+def load_module(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def main() -> None:
+    sys.path.insert(0, str(REF / "src"))
+    from core.boundaries import MurABC
+    from core.fdtd2d import FDTD2D
+    from core.fields import FieldSet
+    from core.grid import YeeGrid
+    from core.materials import Material, MaterialMap
+    from core import sources as S
+
+    out = {}
+    torch.manual_seed(0)
+
+    # 1. slab trajectory: 48x48, courant 0.99, eps_r=4 slab, Gaussian Hz point source, 2 detectors, 150 steps
+    g = YeeGrid(48, 48, 1e-3, 1e-3, courant=0.99, device="cpu")
+    f = FieldSet(g)
+    mm = MaterialMap(g)
+    mm.add_rectangle((28, 33), (0, 47), Material("slab", 4.0, 0.0))
+    Ca, Cb = mm.build()
+    src = S.PointSource(S.GaussianPulse(1.0, sigma=8 * g.dt), 12, 24, "Hz", grid=g, N_steps=200)
+    sim = FDTD2D(g, f, MurABC(g, f.Hz), S.SourceCollection([src]), Ca=Ca, Cb=Cb, n_check=50)
+    tr = []
+    for _ in range(150):
+        sim.step()
+        tr.append([float(f.Hz[20, 24, 0]), float(f.Hz[40, 24, 0])])
+    out["slab_dt"] = np.array(g.dt)
+    out["slab_Ca"], out["slab_Cb"] = Ca.numpy(), Cb.numpy()
+    out["slab_wave"] = src._waveform_tensor.numpy()
+    out["slab_Ex"], out["slab_Ey"], out["slab_Hz"] = f.Ex.numpy(), f.Ey.numpy(), f.Hz.numpy()
+    out["slab_traces"] = np.array(tr, dtype=np.float32).T
+    out["slab_last_field_max"] = np.array(sim.last_field_max)
+
+    # 2. lossy media + mixed sources, free-space boundary values, 40x36, default courant, 80 steps
+    g = YeeGrid(40, 36, 1e-3, 1e-3, device="cpu")
+    f = FieldSet(g)
+    mm = MaterialMap(g, default=Material("coupling", 20.0, 0.1))
+    mm.add_circle((20, 18), 12, Material("brain", 45.0, 0.8))
+    mm.add_circle((24, 15), 4, Material("clot", 60.0, 1.2))
+    Ca, Cb = mm.build()
+    srcs = [
+        S.PointSource(S.RickerWavelet(1.0, 20e9), 20, 18, "Hz", grid=g, N_steps=100),
+        S.PointSource(S.SinusoidalSource(0.3, 30e9), 3, 4, "Ex", grid=g, N_steps=100),
+        S.LineSource(S.ModulatedGaussian(0.5, 25e9, sigma=2e-11), "x", 30, 5, 35, "Ey", grid=g, N_steps=100,
+                     amplitude_profile=[0.5 + 0.02 * k for k in range(30)]),
+        S.PlaneSource(S.GaussianPulse(0.2, sigma=5 * g.dt), "xz", 2, "Hz", grid=g, N_steps=100),
+        S.PointSource(S.UWBPulse(1.0, 10e9, 40e9), 20, 18, "Hz", grid=g, N_steps=100),
+    ]
+    sim = FDTD2D(g, f, MurABC(g, f.Hz), S.SourceCollection(srcs), Ca=Ca, Cb=Cb, n_check=1000)
+    for _ in range(80):
+        sim.step()
+    out["lossy_Ca"], out["lossy_Cb"] = Ca.numpy(), Cb.numpy()
+    for k, s_ in enumerate(srcs):
+        out[f"lossy_wave{k}"] = s_._waveform_tensor.numpy()
+    out["lossy_Ex"], out["lossy_Ey"], out["lossy_Hz"] = f.Ex.numpy(), f.Ey.numpy(), f.Hz.numpy()
+
+    # 3. waveform tables at dt = default 1 mm grid dt, 600 samples
+    dt = YeeGrid(8, 8, 1e-3, 1e-3, device="cpu").dt
+    waves = {
+        "gaussian": S.GaussianPulse(1.3, sigma=7e-12),
+        "gaussian_freq": S.GaussianPulse(0.5, freq=30e9),
+        "sinusoidal": S.SinusoidalSource(0.5, 3e9, 0.0),
+        "ricker": S.RickerWavelet(1.0, 1e9),
+        "modulated": S.ModulatedGaussian(1.0, 1e9, sigma=1 / (2 * math.pi * 0.5e9)),
+        "chirp": S.Chirp(1.0, 1e8, 1e9, 1e-10, 1.2e-9),
+        "uwb": S.UWBPulse(1.0, 1e9, 3e9),
+    }
+    out["wave_dt"] = np.array(dt)
+    for name, w in waves.items():
+        out[f"wave_{name}"] = w.build(600, dt, torch.device("cpu"), torch.float32).numpy()
+
+    # 4. DAS images (brain example; example 09 time-gated)
+    rng = np.random.default_rng(5)
+    bm = load_module("ref_brain_mimo", REF / "examples" / "brain_mimo_imaging.py")
+    g = YeeGrid(30, 30, 1e-3, 1e-3, device="cpu")
+    ants = bm.get_antenna_positions(6, 12, (15, 15), g)
+    Sx = rng.standard_normal((6, 6, 200)).astype(np.float32)
+    out["das_traces"] = Sx
+    out["das_antennas"] = np.array(ants, dtype=np.int64)
+    out["das_dt"] = np.array(g.dt)
+    out["das_image"] = bm.das_backprojection(Sx, ants, g, 200)
+    out["das_image_20x25"] = bm.das_backprojection(Sx, ants, g, 200, image_shape=(20, 25))
+    ex9 = load_module("ref_ex09", REF / "examples" / "09_brain_clot_dataset_sample.py")
+    g9 = YeeGrid(40, 40, 1e-3, 1e-3, device="cpu")
+    ants9 = ex9.get_antenna_positions(5, 16, (20, 20), g9) if hasattr(ex9, "get_antenna_positions") else \
+        bm.get_antenna_positions(5, 16, (20, 20), g9)
+    S9 = rng.standard_normal((5, 5, 500)).astype(np.float32)
+    out["das09_traces"], out["das09_antennas"] = S9, np.array(ants9, dtype=np.int64)
+    out["das09_dt"] = np.array(g9.dt)
+    out["das09_image"] = ex9.das_backprojection(S9, ants9, g9, 500)
+    np.savez_compressed(OUT, **out)
+    print(f"wrote {OUT} ({OUT.stat().st_size} bytes, {len(out)} arrays)")
+
+
+if __name__ == "__main__":
+    main()
