@@ -165,6 +165,33 @@ int wf_tb_tile_shapes(int32_t *out, int32_t cap);
  * wf_fdtd2d_tb_advance, which replaces the reference's tensor / dx. */
 int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, hipStream_t stream);
 
+/* One discrete adjoint step of the leapfrog update for n_runs runs that share
+ * the grid and the coefficients (one phantom, several TX). Given the adjoint
+ * state of u_{n+1} in (lex, ley, lhz), it adds dL/dd_n at the receivers, applies
+ * the transposed stages of step n in reverse order (Ampere, Mur top/bottom/
+ * right/left, Faraday, snapshot) leaving the adjoint of u_n in (lex, ley, lhz),
+ * and accumulates dL/dCa and dL/dCb (fp64) per run. ex_prev/ey_prev are the
+ * fields before step n, hz_next the Hz after it. Sources are affine and drop
+ * out. No reference code counterpart: the reference's differentiable-FDTD
+ * plan is design prose (docs/architecture/DIFFERENTIABLE_PHYSICS.md:33); the
+ * forward it differentiates is FDTD2D.step (src/core/fdtd2d.py:187-266). */
+typedef struct wf_adj_args {
+    const float *ca, *cb;            /* (nx, ny), shared by all runs */
+    const float *ex_prev, *ey_prev;  /* run r at r*state_stride */
+    const float *hz_next;
+    float *lex, *ley, *lhz;          /* adjoint state, run r at r*run_stride */
+    float *lex2, *ley2, *lhz2, *lp;  /* scratch, same layout; lp zero on entry and exit */
+    double *gca, *gcb;               /* gradient accumulators, run r at r*run_stride */
+    const float *dl_dd;              /* dL/d trace[r][k][n] at r*dd_run_stride + k*dd_rx_stride + n */
+    const int32_t *rx_cell;          /* n_rx receiver cells */
+    int64_t run_stride, state_stride, dd_run_stride, dd_rx_stride;
+    int32_t nx, ny, n_runs, n_rx, step;
+    float dh, dx, dy, cmx, cmy;
+    int32_t pad;
+} wf_adj_args;
+
+int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream);
+
 /* Delay-and-sum backprojection of one trace set onto a pixel grid.
  * Replaces das_backprojection of examples/brain_mimo_imaging.py:211-266 and
  * its variants (time gate: examples/09_brain_clot_dataset_sample.py:176-178;

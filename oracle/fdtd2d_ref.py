@@ -138,6 +138,7 @@ class OracleFDTD2D:
         dtype: torch.dtype = torch.float32,
         n_check: int = 100,
         stability_threshold: float = 1e10,
+        functional: bool = False,
     ) -> None:
         if Nx < 4 or Ny < 4:
             raise ValueError("grid must be at least 4x4")
@@ -161,6 +162,7 @@ class OracleFDTD2D:
         self.steps_completed = 0
         self.last_field_max = -1.0
         self.n_stability_checks = 0
+        self.functional = functional
 
     # --- convenience accessors -------------------------------------------
     @property
@@ -196,6 +198,8 @@ class OracleFDTD2D:
 
     def step(self) -> None:
         """One leapfrog step (``src/core/fdtd2d.py:187-266``)."""
+        if self.functional:
+            return self._step_functional()
         n = self.steps_completed
         Ex, Ey, Hz = self.fields["Ex"], self.fields["Ey"], self.fields["Hz"]
         Dh, De, dx, dy = self.Dh, self.De, self.dx, self.dy
@@ -226,6 +230,48 @@ class OracleFDTD2D:
             self.traces.append(Hz[..., ii, jj, 0].clone())
         if self.steps_completed % self.n_check == 0:
             self.check_stability()
+
+    def _step_functional(self) -> None:
+        """The same step written without in-place updates, so torch autograd
+        can differentiate a whole run (used in fp64 as the gradient reference).
+        Same formulas and order; adding the Faraday term as zero-padded full
+        arrays and rebuilding the Mur edges by concatenation."""
+        n = self.steps_completed
+        Ex, Ey, Hz = (self.fields[k][..., 0] for k in ("Ex", "Ey", "Hz"))
+        Dh, De, dx, dy = self.Dh, self.De, self.dx, self.dy
+        P = Hz
+        comp = {"Ex": Ex, "Ey": Ey, "Hz": Hz}
+        for src in self.sources:
+            t = comp[src.component]
+            if src.kind != "cells":
+                raise ValueError("functional oracle supports point and line sources only")
+            vals = src.wave[n:n + 1] * src.amp
+            comp[src.component] = t.index_put((src.i_idx, src.j_idx), vals, accumulate=True)
+        Ex, Ey, H1 = comp["Ex"], comp["Ey"], comp["Hz"]
+        dF = Dh * ((Ex[:-1, 1:] - Ex[:-1, :-1]) / dy - (Ey[1:, :-1] - Ey[:-1, :-1]) / dx)
+        H2 = H1 + torch.nn.functional.pad(dF, (0, 1, 0, 1))
+        cx, cy = self.cx, self.cy
+        L = P[1, :] + cx * (H2[1, :] - P[0, :])
+        R = P[-2, :] + cx * (H2[-2, :] - P[-1, :])
+        H3 = torch.cat([L[None], H2[1:-1], R[None]], 0)
+        B = P[:, 1] + cy * (H3[:, 1] - P[:, 0])
+        T = P[:, -2] + cy * (H3[:, -2] - P[:, -1])
+        H4 = torch.cat([B[:, None], H3[:, 1:-1], T[:, None]], 1)
+        gy = (H4[:, 1:] - H4[:, :-1]) / dy
+        gx = (H4[1:, :] - H4[:-1, :]) / dx
+        if self.Ca is None:
+            Ex2 = torch.cat([Ex[:, :1], Ex[:, 1:] + De * gy], 1)
+            Ey2 = torch.cat([Ey[:1, :], Ey[1:, :] - De * gx], 0)
+        else:
+            Ca, Cb = self.Ca[..., 0], self.Cb[..., 0]
+            Ex2 = torch.cat([Ex[:, :1], Ca[:, 1:] * Ex[:, 1:] + Cb[:, 1:] * gy], 1)
+            Ey2 = torch.cat([Ey[:1, :], Ca[1:, :] * Ey[1:, :] - Cb[1:, :] * gx], 0)
+        self.fields["Ex"], self.fields["Ey"], self.fields["Hz"] = Ex2[..., None], Ey2[..., None], H4[..., None]
+        self.steps_completed += 1
+        if self.detectors:
+            ii = torch.tensor([d[0] for d in self.detectors])
+            jj = torch.tensor([d[1] for d in self.detectors])
+            self.traces.append(H4[ii, jj])
 
     def check_stability(self) -> None:
         """``_check_stability`` (``src/core/fdtd2d.py:268-295``)."""

@@ -72,6 +72,10 @@ int main(void) {
   O(wf_das2d_args, n_tx) O(wf_das2d_args, n_rx) O(wf_das2d_args, n_t) O(wf_das2d_args, nx_img)
   O(wf_das2d_args, ny_img) O(wf_das2d_args, gate_start) O(wf_das2d_args, clip) O(wf_das2d_args, pad)
   O(wf_das2d_args, pitch_x) O(wf_das2d_args, pitch_y) O(wf_das2d_args, v_medium) O(wf_das2d_args, dt)
+  printf("wf_adj_args size %zu\n", sizeof(wf_adj_args));
+  O(wf_adj_args, ca) O(wf_adj_args, hz_next) O(wf_adj_args, lp) O(wf_adj_args, gcb) O(wf_adj_args, dl_dd)
+  O(wf_adj_args, rx_cell) O(wf_adj_args, run_stride) O(wf_adj_args, dd_rx_stride) O(wf_adj_args, nx)
+  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, pad)
   return 0;
 }
 """
@@ -88,7 +92,7 @@ def test_struct_layout_matches_header(tmp_path):
         pytest.skip(f"cannot compile layout probe: {exc}")
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     types = {"wf_inj2d": _native.WfInj2d, "wf_run2d": _native.WfRun2d, "wf_tb_args": _native.WfTbArgs,
-             "wf_das2d_args": _native.WfDas2dArgs}
+             "wf_das2d_args": _native.WfDas2dArgs, "wf_adj_args": _native.WfAdjArgs}
     checked = 0
     for line in out:
         if not line.strip():

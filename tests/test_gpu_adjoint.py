@@ -1,0 +1,107 @@
+"""Hand-written adjoint vs autograd through the oracle's restatement (fp64)
+and vs central differences of the HIP forward.
+
+The loss is 0.5 ||d_sim - d_obs||^2 over all TX x RX traces; gradients are
+taken with respect to per-cell eps_r and sigma through Ca/Cb. Checkpoints are
+placed so the last segment is shorter than the others.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle.fdtd2d_ref import OracleFDTD2D, OracleSource, stable_dt
+
+pytestmark = pytest.mark.gpu
+
+EPS0 = 8.8541878128e-12
+
+
+def _media(nx, ny, seed):
+    g = torch.Generator().manual_seed(seed)
+    eps = 2.0 + 20.0 * torch.rand((nx, ny), generator=g, dtype=torch.float64)
+    sig = 0.5 * torch.rand((nx, ny), generator=g, dtype=torch.float64)
+    return eps, sig
+
+
+def _setup(device, nx=28, ny=24, steps=90, ck=25, depth=5, tx=((6, 5), (20, 17)), rx=((3, 20), (24, 4), (14, 12))):
+    from waveforge_amd.adjoint import TraceProblem
+    from waveforge_amd.core.sources import RickerWavelet
+
+    dt = stable_dt(1e-3, 1e-3)
+    wave = RickerWavelet(1.0, 30e9).build(steps + 4, dt, torch.device("cpu"))
+    prob = TraceProblem(nx, ny, 1e-3, dt, list(tx), list(rx), wave.to(device), steps, checkpoint=ck, depth=depth)
+    return prob, wave, dt
+
+
+def test_adjoint_matches_autograd_through_oracle(device):
+    from waveforge_amd.adjoint import coefficients, misfit_gradient
+
+    prob, wave, dt = _setup(device)
+    eps, sig = _media(prob.nx, prob.ny, 0)
+    g = torch.Generator().manual_seed(9)
+    d_obs = 1e-3 * torch.randn((len(prob.tx), len(prob.rx), prob.n_steps), generator=g, dtype=torch.float64)
+    loss, ge, gs = misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device))
+
+    e = eps.clone().requires_grad_(True)
+    s = sig.clone().requires_grad_(True)
+    ca, cb = coefficients(e, s, dt)
+    d = []
+    for (i, j) in prob.tx:
+        src = OracleSource("Hz", wave.double(), torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0], dtype=torch.float64))
+        o = OracleFDTD2D(prob.nx, prob.ny, 1e-3, 1e-3, dt, Ca=ca.unsqueeze(2), Cb=cb.unsqueeze(2), sources=[src],
+                         detectors=list(prob.rx), dtype=torch.float64, n_check=10 ** 9, functional=True)
+        o.run(prob.n_steps)
+        d.append(o.trace_array())
+    d = torch.stack(d)
+    ref_loss = 0.5 * ((d - d_obs) ** 2).sum()
+    ref_loss.backward()
+    assert abs(loss - float(ref_loss)) / float(ref_loss) < 1e-5
+    for got, ref, nm in ((ge, e.grad, "eps"), (gs, s.grad, "sigma")):
+        got = got.detach().cpu().double()
+        err = float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref))
+        print(f"[adjoint] d loss / d {nm}: rel L2 vs fp64 autograd = {err:.3e}", flush=True)
+        assert err < 1e-4, (nm, err)
+
+
+def test_adjoint_directional_derivative_vs_central_differences(device):
+    """For single-cell eps_r perturbations of +-1e-3 eps_r, (L(+) - L(-)) equals
+    sum(dL/dCa * dCa + dL/dCb * dCb) over the fp32 coefficients the forward
+    actually used."""
+    from waveforge_amd.adjoint import coefficients
+
+    prob, wave, dt = _setup(device, nx=64, ny=60, steps=260, ck=100, depth=6,
+                            tx=((10, 10), (50, 45), (30, 52)), rx=((5, 30), (58, 30), (32, 5), (32, 55)))
+    eps, sig = _media(prob.nx, prob.ny, 3)
+
+    def run(e):
+        ca64, cb64 = coefficients(e, sig, dt)
+        ca, cb = ca64.float().to(device).contiguous(), cb64.float().to(device).contiguous()
+        tr, ck = prob.forward(ca, cb)
+        return tr, ck, ca, cb
+
+    base, ckpts, ca, cb = run(eps)
+    d_obs = base.double() * 0.8
+    resid = (base.double() - d_obs).float()
+    gca, gcb = prob.backward(ca, cb, ckpts, resid)
+    rng = np.random.default_rng(4)
+    cells = [(int(rng.integers(8, 56)), int(rng.integers(8, 52))) for _ in range(8)]
+    worst = 0.0
+    for (i, j) in cells:
+        h = 1e-3 * float(eps[i, j])
+        Lpm, dca, dcb = [], [], []
+        for sgn in (1.0, -1.0):
+            e2 = eps.clone()
+            e2[i, j] += sgn * h
+            tr, _, ca2, cb2 = run(e2)
+            Lpm.append(0.5 * float(((tr.double() - d_obs) ** 2).sum()))
+            dca.append(float(ca2[i, j]))
+            dcb.append(float(cb2[i, j]))
+        fd = Lpm[0] - Lpm[1]
+        pred = float(gca[i, j]) * (dca[0] - dca[1]) + float(gcb[i, j]) * (dcb[0] - dcb[1])
+        rel = abs(fd - pred) / max(abs(fd), 1e-300)
+        worst = max(worst, rel)
+        print(f"[adjoint] cell ({i},{j}): FD {fd:.6e} adjoint {pred:.6e} rel {rel:.2e}", flush=True)
+    assert worst < 2e-2

@@ -29,6 +29,7 @@ EXPORTED_SYMBOLS = (
     "wf_tb_tile_shapes",
     "wf_div_verify",
     "wf_das2d",
+    "wf_adjoint_step",
 )
 
 _vp = ctypes.c_void_p
@@ -101,6 +102,16 @@ class WfDas2dArgs(ctypes.Structure):
     ]
 
 
+class WfAdjArgs(ctypes.Structure):
+    """``wf_adj_args``."""
+
+    _fields_ = [(n, _vp) for n in ("ca", "cb", "ex_prev", "ey_prev", "hz_next", "lex", "ley", "lhz", "lex2",
+                                   "ley2", "lhz2", "lp", "gca", "gcb", "dl_dd", "rx_cell")] + \
+        [(n, _i64) for n in ("run_stride", "state_stride", "dd_run_stride", "dd_rx_stride")] + \
+        [(n, _i32) for n in ("nx", "ny", "n_runs", "n_rx", "step")] + \
+        [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("pad", _i32)]
+
+
 _lock = threading.Lock()
 _lib = None
 
@@ -138,6 +149,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_runs_add_step.restype = ctypes.c_int
         lib.wf_div_verify.argtypes = [_f32, _f32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]
         lib.wf_div_verify.restype = ctypes.c_int
+        lib.wf_adjoint_step.argtypes = [ctypes.POINTER(WfAdjArgs), _vp]
+        lib.wf_adjoint_step.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]
         lib.wf_das2d.restype = ctypes.c_int
         ver = lib.wf_abi_version()
