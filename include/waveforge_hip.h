@@ -137,6 +137,54 @@ typedef struct wf_tb_args {
 
 int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
 
+/* fp64 counterparts of wf_inj2d / wf_run2d for the double-precision check
+ * path: the same step in fp64 arithmetic, as the reference runs when its grid
+ * dtype is float64 (src/core/fields.py:156-168 allocates in the grid dtype;
+ * the coefficients are then Python doubles, src/core/fdtd2d.py:158-159, and
+ * MurABC's, boundaries.py:177-178). It serves as the FP64 reference run that
+ * per-result errors of the fp32 path are stated against, and as the forward
+ * of central-difference gradient checks. Same layout and ownership rules as
+ * the fp32 descriptors; no temporally blocked variant. */
+typedef struct wf_inj2d_f64 {
+    const double *wave;
+    int32_t wave_len;
+    int32_t comp;
+    int32_t cell;
+    int32_t pad;
+    double amp;
+} wf_inj2d_f64;
+
+typedef struct wf_run2d_f64 {
+    double *ex, *ey, *hz;
+    const double *ca, *cb;        /* NULL => free space */
+    double *pstrip;               /* 4*(nx+ny) doubles */
+    const wf_inj2d_f64 *inj;
+    const int32_t *grp_start;
+    const int32_t *det_cell;
+    double *traces;
+    int64_t trace_stride;
+    int32_t n_grp, n_inj;
+    int32_t n_det, trace_n0, trace_len;
+    int32_t nx, ny;
+    int32_t step;
+    double dh, de, dx, dy;        /* dt/MU0, dt/EPS0, dx, dy in fp64 */
+    double cmx, cmy;
+} wf_run2d_f64;
+
+typedef struct wf_f64_args {
+    const wf_run2d_f64 *runs;     /* device array of n_runs descriptors */
+    int32_t n_runs;
+    int32_t max_nx, max_ny;       /* bounds of every run's nx, ny */
+    int32_t n_steps;
+    int32_t step_offset;          /* step index = runs[r].step + step_offset + s */
+    int32_t pad;
+} wf_f64_args;
+
+/* Advance every run `n_steps` fp64 steps in place, three launches per step
+ * covering all runs (the per-step semantics of wf_fdtd2d_step). Replaces
+ * FDTD2D.step() on a float64 grid (src/core/fdtd2d.py:187-266). */
+int wf_fdtd2d_step_f64(const wf_f64_args *args, hipStream_t stream);
+
 /* Stability judgement on the device, so the stepping stream never waits for
  * the host: for each run r, if any of the three maxima maxabs[3r..3r+2]
  * (uint bit patterns of max|Ex|, max|Ey|, max|Hz|, as folded by

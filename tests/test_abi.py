@@ -76,6 +76,12 @@ int main(void) {
   O(wf_adj_args, ca) O(wf_adj_args, hz_next) O(wf_adj_args, lp) O(wf_adj_args, gcb) O(wf_adj_args, dl_dd)
   O(wf_adj_args, rx_cell) O(wf_adj_args, run_stride) O(wf_adj_args, dd_rx_stride) O(wf_adj_args, nx)
   O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, pad)
+  printf("wf_inj2d_f64 size %zu\nwf_run2d_f64 size %zu\nwf_f64_args size %zu\n",
+         sizeof(wf_inj2d_f64), sizeof(wf_run2d_f64), sizeof(wf_f64_args));
+  O(wf_inj2d_f64, wave) O(wf_inj2d_f64, cell) O(wf_inj2d_f64, amp)
+  O(wf_run2d_f64, ex) O(wf_run2d_f64, pstrip) O(wf_run2d_f64, traces) O(wf_run2d_f64, trace_stride)
+  O(wf_run2d_f64, n_grp) O(wf_run2d_f64, step) O(wf_run2d_f64, dh) O(wf_run2d_f64, cmy)
+  O(wf_f64_args, runs) O(wf_f64_args, n_runs) O(wf_f64_args, step_offset) O(wf_f64_args, pad)
   return 0;
 }
 """
@@ -92,7 +98,9 @@ def test_struct_layout_matches_header(tmp_path):
         pytest.skip(f"cannot compile layout probe: {exc}")
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     types = {"wf_inj2d": _native.WfInj2d, "wf_run2d": _native.WfRun2d, "wf_tb_args": _native.WfTbArgs,
-             "wf_das2d_args": _native.WfDas2dArgs, "wf_adj_args": _native.WfAdjArgs}
+             "wf_das2d_args": _native.WfDas2dArgs, "wf_adj_args": _native.WfAdjArgs,
+             "wf_inj2d_f64": _native.WfInj2dF64, "wf_run2d_f64": _native.WfRun2dF64,
+             "wf_f64_args": _native.WfF64Args}
     checked = 0
     for line in out:
         if not line.strip():

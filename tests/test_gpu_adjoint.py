@@ -66,42 +66,38 @@ def test_adjoint_matches_autograd_through_oracle(device):
         assert err < 1e-4, (nm, err)
 
 
-def test_adjoint_directional_derivative_vs_central_differences(device):
-    """For single-cell eps_r perturbations of +-1e-3 eps_r, (L(+) - L(-)) equals
-    sum(dL/dCa * dCa + dL/dCb * dCb) over the fp32 coefficients the forward
-    actually used."""
-    from waveforge_amd.adjoint import coefficients
+def test_adjoint_vs_central_differences_of_the_fp64_forward(device):
+    """dL/d eps_r at single cells vs (L(eps+h) - L(eps-h)) / 2h, h = 1e-3 eps_r,
+    with L evaluated by the fp64 GPU forward (fp32 rounding would swamp the
+    quotient at weakly sensitive cells). The adjoint differentiates the fp32
+    forward, so agreement is to fp32 accuracy relative to the gradient's scale."""
+    from waveforge_amd.adjoint import coefficients, misfit_gradient
 
     prob, wave, dt = _setup(device, nx=64, ny=60, steps=260, ck=100, depth=6,
                             tx=((10, 10), (50, 45), (30, 52)), rx=((5, 30), (58, 30), (32, 5), (32, 55)))
     eps, sig = _media(prob.nx, prob.ny, 3)
+    eps_d, sig_d = eps.to(device), sig.to(device)
+    base = prob.forward_fp64(*coefficients(eps_d, sig_d, dt))
+    d_obs = base * 0.8
+    _, ge, gs = misfit_gradient(eps_d, sig_d, prob, d_obs)
+    ge = ge.double().cpu()
+    scale = float(ge.abs().max())
 
-    def run(e):
-        ca64, cb64 = coefficients(e, sig, dt)
-        ca, cb = ca64.float().to(device).contiguous(), cb64.float().to(device).contiguous()
-        tr, ck = prob.forward(ca, cb)
-        return tr, ck, ca, cb
+    def loss(e):
+        d = prob.forward_fp64(*coefficients(e, sig_d, dt))
+        return 0.5 * float(((d - d_obs) ** 2).sum())
 
-    base, ckpts, ca, cb = run(eps)
-    d_obs = base.double() * 0.8
-    resid = (base.double() - d_obs).float()
-    gca, gcb = prob.backward(ca, cb, ckpts, resid)
     rng = np.random.default_rng(4)
     cells = [(int(rng.integers(8, 56)), int(rng.integers(8, 52))) for _ in range(8)]
-    worst = 0.0
+    cells.append(tuple(int(v) for v in np.unravel_index(int(ge.abs().argmax()), ge.shape)))
     for (i, j) in cells:
         h = 1e-3 * float(eps[i, j])
-        Lpm, dca, dcb = [], [], []
-        for sgn in (1.0, -1.0):
-            e2 = eps.clone()
-            e2[i, j] += sgn * h
-            tr, _, ca2, cb2 = run(e2)
-            Lpm.append(0.5 * float(((tr.double() - d_obs) ** 2).sum()))
-            dca.append(float(ca2[i, j]))
-            dcb.append(float(cb2[i, j]))
-        fd = Lpm[0] - Lpm[1]
-        pred = float(gca[i, j]) * (dca[0] - dca[1]) + float(gcb[i, j]) * (dcb[0] - dcb[1])
-        rel = abs(fd - pred) / max(abs(fd), 1e-300)
-        worst = max(worst, rel)
-        print(f"[adjoint] cell ({i},{j}): FD {fd:.6e} adjoint {pred:.6e} rel {rel:.2e}", flush=True)
-    assert worst < 2e-2
+        ep, em = eps_d.clone(), eps_d.clone()
+        ep[i, j] += h
+        em[i, j] -= h
+        fd = (loss(ep) - loss(em)) / (2 * h)
+        adj = float(ge[i, j])
+        tol = 2e-3 * abs(fd) + 1e-5 * scale
+        print(f"[adjoint] cell ({i},{j}): FD {fd:.6e} adjoint {adj:.6e} |diff| {abs(fd - adj):.2e} tol {tol:.2e}",
+              flush=True)
+        assert abs(fd - adj) <= tol, (i, j, fd, adj)

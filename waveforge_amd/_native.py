@@ -30,6 +30,7 @@ EXPORTED_SYMBOLS = (
     "wf_div_verify",
     "wf_das2d",
     "wf_adjoint_step",
+    "wf_fdtd2d_step_f64",
 )
 
 _vp = ctypes.c_void_p
@@ -112,6 +113,29 @@ class WfAdjArgs(ctypes.Structure):
         [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("pad", _i32)]
 
 
+class WfInj2dF64(ctypes.Structure):
+    """``wf_inj2d_f64``."""
+
+    _fields_ = [("wave", _vp), ("wave_len", _i32), ("comp", _i32), ("cell", _i32), ("pad", _i32), ("amp", _f64)]
+
+
+class WfRun2dF64(ctypes.Structure):
+    """``wf_run2d_f64``."""
+
+    _fields_ = [(n, _vp) for n in ("ex", "ey", "hz", "ca", "cb", "pstrip", "inj", "grp_start", "det_cell",
+                                   "traces")] + \
+        [("trace_stride", _i64)] + \
+        [(n, _i32) for n in ("n_grp", "n_inj", "n_det", "trace_n0", "trace_len", "nx", "ny", "step")] + \
+        [(n, _f64) for n in ("dh", "de", "dx", "dy", "cmx", "cmy")]
+
+
+class WfF64Args(ctypes.Structure):
+    """``wf_f64_args``."""
+
+    _fields_ = [("runs", _vp)] + [(n, _i32) for n in ("n_runs", "max_nx", "max_ny", "n_steps", "step_offset",
+                                                       "pad")]
+
+
 _lock = threading.Lock()
 _lib = None
 
@@ -151,6 +175,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_div_verify.restype = ctypes.c_int
         lib.wf_adjoint_step.argtypes = [ctypes.POINTER(WfAdjArgs), _vp]
         lib.wf_adjoint_step.restype = ctypes.c_int
+        lib.wf_fdtd2d_step_f64.argtypes = [ctypes.POINTER(WfF64Args), _vp]
+        lib.wf_fdtd2d_step_f64.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]
         lib.wf_das2d.restype = ctypes.c_int
         ver = lib.wf_abi_version()

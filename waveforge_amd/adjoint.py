@@ -88,6 +88,18 @@ class TraceProblem:
         self._tables = [r.sources for r in br.runs]
         return traces, ckpts
 
+    def forward_fp64(self, ca64: torch.Tensor, cb64: torch.Tensor) -> torch.Tensor:
+        """fp64 traces ``(n_tx, n_rx, n_steps)`` for fp64 coefficients (the
+        double-precision step, no checkpoints): the forward of central-difference
+        gradient checks, whose quotients fp32 rounding would swamp."""
+        from .fp64 import Fp64Runs
+
+        specs = [RunSpec(self.nx, self.ny, ca64, cb64, sources=[("Hz", i, j, self.wave, 1.0)],
+                         receivers=list(self.rx), n_record=self.n_steps) for (i, j) in self.tx]
+        runs = Fp64Runs(specs, dx=self.dx, dy=self.dx, dt=self.dt, device=self.device)
+        runs.advance(self.n_steps)
+        return torch.stack([runs.traces(k) for k in range(len(specs))])
+
     # ------------------------------------------------------------------
     def _recompute(self, ca, cb, start: torch.Tensor, s0: int, L: int) -> torch.Tensor:
         """States ``H[k] = u_{s0+k}``, k = 0..L, from checkpoint ``start``: one
@@ -195,4 +207,4 @@ def misfit_gradient(eps, sigma, problem: TraceProblem, d_obs: torch.Tensor):
     d = traces(e, s, problem)
     loss = 0.5 * ((d.double() - d_obs.double()) ** 2).sum()
     loss.backward()
-    return float(loss), e.grad, s.grad
+    return float(loss.detach()), e.grad, s.grad

@@ -21,19 +21,32 @@
 // CPU `tensor / python_float` is.
 #include "wf_common.hpp"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int kBX = 64;
 constexpr int kBY = 4;
 
-__device__ __forceinline__ float prow(const wf_run2d &r, int s, int j) {
+// The cell bodies are templates over the run descriptor, so the fp32 path
+// (wf_run2d) and the fp64 check path (wf_run2d_f64) share one statement of
+// the step; T is the descriptor's field type.
+template <class R>
+using real_t = typename std::remove_cv<typename std::remove_pointer<decltype(R::ex)>::type>::type;
+
+template <class R>
+__device__ __forceinline__ real_t<R> prow(const R &r, int s, int j) {
     return r.pstrip[s * r.ny + j];  // s: 0 -> row 0, 1 -> row 1, 2 -> row nx-2, 3 -> row nx-1
 }
-__device__ __forceinline__ float pcol(const wf_run2d &r, int s, int i) {
+template <class R>
+__device__ __forceinline__ real_t<R> pcol(const R &r, int s, int i) {
     return r.pstrip[4 * r.ny + s * r.nx + i];  // s: 0 -> col 0, 1 -> col 1, 2 -> col ny-2, 3 -> col ny-1
 }
 
-__global__ void __launch_bounds__(1024) k_pre(wf_run2d r, int n) {
+// Snapshot strips then soft sources of step n; one workgroup per run.
+template <class R>
+__device__ __forceinline__ void pre_body(const R &r, int n) {
+    using T = real_t<R>;
     const int nx = r.nx, ny = r.ny;
     const int tid = threadIdx.x;
     for (int t = tid; t < 4 * ny; t += blockDim.x) {
@@ -49,83 +62,130 @@ __global__ void __launch_bounds__(1024) k_pre(wf_run2d r, int n) {
     __syncthreads();
     for (int g = tid; g < r.n_grp; g += blockDim.x) {
         const int a = r.grp_start[g], b = r.grp_start[g + 1];
-        const wf_inj2d e0 = r.inj[a];
-        float *f = e0.comp == WF_COMP_EX ? r.ex : (e0.comp == WF_COMP_EY ? r.ey : r.hz);
-        float v = f[e0.cell];
+        const int comp = r.inj[a].comp, cell = r.inj[a].cell;
+        T *f = comp == WF_COMP_EX ? r.ex : (comp == WF_COMP_EY ? r.ey : r.hz);
+        T v = f[cell];
         for (int c = a; c < b; ++c) {
-            const float w = r.inj[c].wave[n] * r.inj[c].amp;  // values = w_now * amp
-            v = v + w;                                        // target[idx] += values
+            const T w = r.inj[c].wave[n] * r.inj[c].amp;  // values = w_now * amp
+            v = v + w;                                    // target[idx] += values
         }
-        f[e0.cell] = v;
+        f[cell] = v;
     }
 }
 
-__global__ void __launch_bounds__(kBX * kBY) k_h(wf_run2d r) {
+// Faraday at interior cell (i, j) plus the Mur edge cells it owns.
+template <class R>
+__device__ __forceinline__ void h_body(const R &r, int i, int j) {
+    using T = real_t<R>;
     const int nx = r.nx, ny = r.ny;
-    const int j = 1 + blockIdx.x * kBX + threadIdx.x;
-    const int i = 1 + blockIdx.y * kBY + threadIdx.y;
-    if (i > nx - 2 || j > ny - 2) return;
     const size_t c = (size_t)i * ny + j;
-    const float h1 = r.hz[c];
-    const float a = (r.ex[c + 1] - r.ex[c]) / r.dy;
-    const float b = (r.ey[c + ny] - r.ey[c]) / r.dx;
-    const float F = h1 + r.dh * (a - b);
+    const T h1 = r.hz[c];
+    const T a = (r.ex[c + 1] - r.ex[c]) / r.dy;
+    const T b = (r.ey[c + ny] - r.ey[c]) / r.dx;
+    const T F = h1 + r.dh * (a - b);
     r.hz[c] = F;
-    const float cx = r.cmx, cy = r.cmy;
+    const T cx = r.cmx, cy = r.cmy;
     if (i == 1) {  // left edge (0, j), then the corners that use it
-        const float L = prow(r, 1, j) + cx * (F - prow(r, 0, j));
+        const T L = prow(r, 1, j) + cx * (F - prow(r, 0, j));
         r.hz[j] = L;
         if (j == 1) r.hz[0] = prow(r, 0, 1) + cy * (L - prow(r, 0, 0));
         if (j == ny - 2) r.hz[ny - 1] = prow(r, 0, ny - 2) + cy * (L - prow(r, 0, ny - 1));
     }
     if (i == nx - 2) {  // right edge (nx-1, j)
-        const float R = prow(r, 2, j) + cx * (F - prow(r, 3, j));
-        r.hz[(size_t)(nx - 1) * ny + j] = R;
-        if (j == 1) r.hz[(size_t)(nx - 1) * ny] = prow(r, 3, 1) + cy * (R - prow(r, 3, 0));
+        const T R_ = prow(r, 2, j) + cx * (F - prow(r, 3, j));
+        r.hz[(size_t)(nx - 1) * ny + j] = R_;
+        if (j == 1) r.hz[(size_t)(nx - 1) * ny] = prow(r, 3, 1) + cy * (R_ - prow(r, 3, 0));
         if (j == ny - 2)
-            r.hz[(size_t)(nx - 1) * ny + ny - 1] = prow(r, 3, ny - 2) + cy * (R - prow(r, 3, ny - 1));
+            r.hz[(size_t)(nx - 1) * ny + ny - 1] = prow(r, 3, ny - 2) + cy * (R_ - prow(r, 3, ny - 1));
     }
     if (j == 1) r.hz[(size_t)i * ny] = pcol(r, 1, i) + cy * (F - pcol(r, 0, i));
     if (j == ny - 2) r.hz[(size_t)i * ny + ny - 1] = pcol(r, 2, i) + cy * (F - pcol(r, 3, i));
 }
 
-template <bool kMat>
-__global__ void __launch_bounds__(kBX * kBY) k_e(wf_run2d r, int n) {
-    const int nx = r.nx, ny = r.ny;
-    const int j = blockIdx.x * kBX + threadIdx.x;
-    const int i = blockIdx.y * kBY + threadIdx.y;
-    if (blockIdx.x == 0 && blockIdx.y == 0) {
-        const int t = n - r.trace_n0;
-        if (t >= 0 && t < r.trace_len) {
-            for (int d = threadIdx.y * kBX + threadIdx.x; d < r.n_det; d += kBX * kBY)
-                r.traces[d * r.trace_stride + t] = r.hz[r.det_cell[d]];
-        }
+// Detector gather of step n (Hz after the step); threads of one workgroup.
+template <class R>
+__device__ __forceinline__ void record_body(const R &r, int n) {
+    const int t = n - r.trace_n0;
+    if (t >= 0 && t < r.trace_len) {
+        for (int d = threadIdx.y * kBX + threadIdx.x; d < r.n_det; d += kBX * kBY)
+            r.traces[d * r.trace_stride + t] = r.hz[r.det_cell[d]];
     }
-    if (i >= nx || j >= ny) return;
+}
+
+// Ampere at cell (i, j).
+template <bool kMat, class R>
+__device__ __forceinline__ void e_body(const R &r, int i, int j) {
+    using T = real_t<R>;
+    const int ny = r.ny;
     const size_t c = (size_t)i * ny + j;
-    const float h = r.hz[c];
+    const T h = r.hz[c];
     if (j >= 1) {
-        const float g = (h - r.hz[c - 1]) / r.dy;
+        const T g = (h - r.hz[c - 1]) / r.dy;
         if (kMat) {
-            const float t1 = r.ca[c] * r.ex[c];
-            const float t2 = r.cb[c] * g;
+            const T t1 = r.ca[c] * r.ex[c];
+            const T t2 = r.cb[c] * g;
             r.ex[c] = t1 + t2;
         } else {
-            const float t2 = r.de * g;
+            const T t2 = r.de * g;
             r.ex[c] = r.ex[c] + t2;
         }
     }
     if (i >= 1) {
-        const float g = (h - r.hz[c - ny]) / r.dx;
+        const T g = (h - r.hz[c - ny]) / r.dx;
         if (kMat) {
-            const float t1 = r.ca[c] * r.ey[c];
-            const float t2 = r.cb[c] * g;
+            const T t1 = r.ca[c] * r.ey[c];
+            const T t2 = r.cb[c] * g;
             r.ey[c] = t1 - t2;
         } else {
-            const float t2 = r.de * g;
+            const T t2 = r.de * g;
             r.ey[c] = r.ey[c] - t2;
         }
     }
+}
+
+__global__ void __launch_bounds__(1024) k_pre(wf_run2d r, int n) { pre_body(r, n); }
+
+__global__ void __launch_bounds__(kBX * kBY) k_h(wf_run2d r) {
+    const int j = 1 + blockIdx.x * kBX + threadIdx.x;
+    const int i = 1 + blockIdx.y * kBY + threadIdx.y;
+    if (i > r.nx - 2 || j > r.ny - 2) return;
+    h_body(r, i, j);
+}
+
+template <bool kMat>
+__global__ void __launch_bounds__(kBX * kBY) k_e(wf_run2d r, int n) {
+    const int j = blockIdx.x * kBX + threadIdx.x;
+    const int i = blockIdx.y * kBY + threadIdx.y;
+    if (blockIdx.x == 0 && blockIdx.y == 0) record_body(r, n);
+    if (i >= r.nx || j >= r.ny) return;
+    e_body<kMat>(r, i, j);
+}
+
+// fp64 check path: one launch per stage covers every run (blockIdx.z or
+// blockIdx.x is the run); descriptors are read from the device table.
+__global__ void __launch_bounds__(1024) k_pre64(const wf_run2d_f64 *runs, int n_off) {
+    const wf_run2d_f64 &r = runs[blockIdx.x];
+    pre_body(r, r.step + n_off);
+}
+
+__global__ void __launch_bounds__(kBX * kBY) k_h64(const wf_run2d_f64 *runs) {
+    const wf_run2d_f64 &r = runs[blockIdx.z];
+    const int j = 1 + blockIdx.x * kBX + threadIdx.x;
+    const int i = 1 + blockIdx.y * kBY + threadIdx.y;
+    if (i > r.nx - 2 || j > r.ny - 2) return;
+    h_body(r, i, j);
+}
+
+__global__ void __launch_bounds__(kBX * kBY) k_e64(const wf_run2d_f64 *runs, int n_off) {
+    const wf_run2d_f64 &r = runs[blockIdx.z];
+    const int j = blockIdx.x * kBX + threadIdx.x;
+    const int i = blockIdx.y * kBY + threadIdx.y;
+    if (blockIdx.x == 0 && blockIdx.y == 0) record_body(r, r.step + n_off);
+    if (i >= r.nx || j >= r.ny) return;
+    if (r.ca)
+        e_body<true>(r, i, j);
+    else
+        e_body<false>(r, i, j);
 }
 
 __global__ void __launch_bounds__(256) k_maxabs_partial(const float *__restrict__ ex,
@@ -215,4 +275,26 @@ extern "C" int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *h
     hipLaunchKernelGGL(k_maxabs_partial, dim3(n_partial), dim3(256), 0, stream, ex, ey, hz, n, p);
     hipLaunchKernelGGL(k_maxabs_final, dim3(1), dim3(256), 0, stream, p, n_partial, out3);
     return wf::check_launch("wf_fdtd2d_maxabs");
+}
+
+extern "C" int wf_fdtd2d_step_f64(const wf_f64_args *args, hipStream_t stream) {
+    if (!args || !args->runs || args->n_runs < 1 || args->n_runs > 65535 || args->n_steps < 0 ||
+        args->max_nx < 4 || args->max_ny < 4) {
+        wf::set_error("wf_fdtd2d_step_f64: bad arguments");
+        return WF_ERR_ARG;
+    }
+    const wf_f64_args &a = *args;
+    dim3 gh((a.max_ny - 2 + kBX - 1) / kBX, (a.max_nx - 2 + kBY - 1) / kBY, a.n_runs);
+    dim3 ge((a.max_ny + kBX - 1) / kBX, (a.max_nx + kBY - 1) / kBY, a.n_runs);
+    for (int s = 0; s < a.n_steps; ++s) {
+        const int off = a.step_offset + s;
+        hipLaunchKernelGGL(k_pre64, dim3(a.n_runs), dim3(1024), 0, stream, a.runs, off);
+        hipLaunchKernelGGL(k_h64, gh, dim3(kBX, kBY), 0, stream, a.runs);
+        hipLaunchKernelGGL(k_e64, ge, dim3(kBX, kBY), 0, stream, a.runs, off);
+        if ((s & 255) == 255 || s + 1 == a.n_steps) {
+            const int rc = wf::check_launch("wf_fdtd2d_step_f64");
+            if (rc) return rc;
+        }
+    }
+    return WF_OK;
 }

@@ -1,0 +1,138 @@
+"""Double-precision runs on the GPU: the same leapfrog step in fp64.
+
+The reference runs in float64 when its grid dtype is float64
+(``src/core/fields.py:156-168`` allocates the fields in the grid dtype; the
+update coefficients are Python doubles, ``src/core/fdtd2d.py:158-159``, and
+so are the Mur coefficients, ``src/core/boundaries.py:177-178``). This
+module drives ``wf_fdtd2d_step_f64`` for a set of runs that may differ in
+grid size: three launches per step cover every run. It is the FP64
+reference run against which fp32 results state their error at full size,
+and the forward of central-difference gradient checks, where fp32 rounding
+would swamp the difference quotient.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _native
+from ._native import WfF64Args, WfRun2dF64
+from .engine import C0, EPS0, MU0
+
+INJ64_DTYPE = np.dtype([("wave", np.uint64), ("wave_len", np.int32), ("comp", np.int32), ("cell", np.int32),
+                        ("pad", np.int32), ("amp", np.float64)])
+assert INJ64_DTYPE.itemsize == ctypes.sizeof(_native.WfInj2dF64)
+
+
+class Fp64Runs:
+    """fp64 state, sources and receivers of independent runs.
+
+    ``specs`` are ``mimo.RunSpec``-like objects: ``nx, ny, ca, cb`` (any
+    float tensors or None), ``sources`` as ``(comp, i, j, wave, amp)`` tuples
+    (wave: a device tensor, used in fp64), ``receivers`` as ``(i, j)`` Hz
+    cells and ``n_record``. Contributions to one (component, cell) are applied
+    in list order, as ``SourceCollection`` does.
+    """
+
+    def __init__(self, specs: Sequence, *, dx: float, dy: float, dt: float, device=None) -> None:
+        if not specs:
+            raise ValueError("no runs")
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        if self.device.type != "cuda":
+            raise RuntimeError("waveforge_amd: fp64 runs need the ROCm GPU; there is no CPU path")
+        _native.load()
+        self.dx, self.dy, self.dt = float(dx), float(dy), float(dt)
+        cmx = (C0 * dt - dx) / (C0 * dt + dx)
+        cmy = (C0 * dt - dy) / (C0 * dt + dy)
+        dev = self.device
+        self._keep = []
+        self.state, self.trace_bufs = [], []
+        structs = []
+        self.min_wave_len = math.inf
+        for k, sp in enumerate(specs):
+            nx, ny = int(sp.nx), int(sp.ny)
+            if nx < 4 or ny < 4:
+                raise ValueError(f"run {k}: grid must be at least 4x4")
+            ex, ey, hz = (torch.zeros((nx, ny), dtype=torch.float64, device=dev) for _ in range(3))
+            pstrip = torch.zeros(4 * (nx + ny), dtype=torch.float64, device=dev)
+            r = WfRun2dF64()
+            r.ex, r.ey, r.hz, r.pstrip = ex.data_ptr(), ey.data_ptr(), hz.data_ptr(), pstrip.data_ptr()
+            self._keep += [pstrip]
+            if (sp.ca is None) != (sp.cb is None):
+                raise ValueError(f"run {k}: ca and cb must both be given or both None")
+            if sp.ca is not None:
+                ca = sp.ca.to(device=dev, dtype=torch.float64).reshape(nx, ny).contiguous()
+                cb = sp.cb.to(device=dev, dtype=torch.float64).reshape(nx, ny).contiguous()
+                self._keep += [ca, cb]
+                r.ca, r.cb = ca.data_ptr(), cb.data_ptr()
+            self._source_table(r, sp.sources, nx, ny, k)
+            if sp.receivers:
+                if any(not (0 <= i < nx and 0 <= j < ny) for i, j in sp.receivers):
+                    raise ValueError(f"run {k}: receiver outside the grid")
+                cells =torch.tensor([i * ny + j for i, j in sp.receivers], dtype=torch.int32, device=dev)
+                tr = torch.zeros((len(sp.receivers), max(1, int(sp.n_record))), dtype=torch.float64, device=dev)
+                self._keep.append(cells)
+                r.det_cell, r.traces = cells.data_ptr(), tr.data_ptr()
+                r.trace_stride, r.n_det, r.trace_len = tr.stride(0), len(sp.receivers), tr.shape[1]
+                self.trace_bufs.append(tr)
+            else:
+                self.trace_bufs.append(None)
+            r.nx, r.ny, r.step = nx, ny, 0
+            r.dh, r.de, r.dx, r.dy, r.cmx, r.cmy = dt / MU0, dt / EPS0, self.dx, self.dy, cmx, cmy
+            structs.append(r)
+            self.state.append((ex, ey, hz))
+        arr = (WfRun2dF64 * len(structs))(*structs)
+        self._table = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(dev)
+        self.max_nx = max(int(s.nx) for s in specs)
+        self.max_ny = max(int(s.ny) for s in specs)
+        self.steps_completed = 0
+
+    def _source_table(self, r: WfRun2dF64, sources, nx: int, ny: int, k: int) -> None:
+        if not sources:
+            return
+        rec = np.zeros(len(sources), dtype=INJ64_DTYPE)
+        key = np.zeros(len(sources), dtype=np.int64)
+        for m, (comp, i, j, wave, amp) in enumerate(sources):
+            if comp not in _native.WF_COMP:
+                raise ValueError(f"run {k}: component {comp!r} is not a 2D TM field")
+            if not (0 <= i < nx and 0 <= j < ny):
+                raise ValueError(f"run {k}: source cell ({i}, {j}) outside the grid")
+            w = wave.to(device=self.device, dtype=torch.float64).contiguous()
+            self._keep.append(w)
+            self.min_wave_len = min(self.min_wave_len, w.numel())
+            rec[m] = (w.data_ptr(), w.numel(), _native.WF_COMP[comp], i * ny + j, 0, float(amp))
+            key[m] = _native.WF_COMP[comp] * nx * ny + i * ny + j
+        perm = np.argsort(key, kind="stable")
+        rec, key = rec[perm], key[perm]
+        starts = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+        grp = torch.from_numpy(np.r_[starts, len(rec)].astype(np.int32)).to(self.device)
+        inj = torch.from_numpy(rec.view(np.uint8).copy()).to(self.device)
+        self._keep += [grp, inj]
+        r.inj, r.grp_start, r.n_grp, r.n_inj = inj.data_ptr(), grp.data_ptr(), len(starts), len(rec)
+
+    def advance(self, n_steps: int) -> None:
+        """Enqueue ``n_steps`` fp64 steps of every run (asynchronous)."""
+        if n_steps <= 0:
+            return
+        if self.steps_completed + n_steps > self.min_wave_len:
+            raise RuntimeError(f"a source waveform has {self.min_wave_len} samples, "
+                               f"cannot reach step {self.steps_completed + n_steps}")
+        lib = _native.load()
+        a = WfF64Args()
+        a.runs, a.n_runs = self._table.data_ptr(), len(self.state)
+        a.max_nx, a.max_ny = self.max_nx, self.max_ny
+        a.n_steps, a.step_offset = int(n_steps), self.steps_completed
+        _native.check(lib.wf_fdtd2d_step_f64(ctypes.byref(a), ctypes.c_void_p(_native.stream_handle(self.device))),
+                      "wf_fdtd2d_step_f64")
+        self.steps_completed += n_steps
+
+    def fields(self, k: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        return self.state[k]
+
+    def traces(self, k: int) -> torch.Tensor:
+        return self.trace_bufs[k]
