@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--no-imaging", action="store_true", help="skip the batched-imaging section")
     p.add_argument("--no-serving", action="store_true", help="skip the serving section")
     p.add_argument("--serving-requests", type=int, default=48)
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
+    p.add_argument("--fd-cells", type=int, default=32, help="seeded cells of the central-difference check")
     p.add_argument("--copy-probe", action="store_true",
                    help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
                         "traffic used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
@@ -216,6 +218,93 @@ def serving_suite(dev, n_requests: int) -> dict:
     }
 
 
+def adjoint_suite(dev, n_fd: int) -> dict:
+    """Trace-misfit gradient on the 256^2 head phantom, 16 TX x 16 RX, 3000
+    steps, checkpoints every 100 steps: d_obs from the clot phantom, the model
+    is the clot-free twin. Times the fp32 forward and forward + adjoint
+    backward; states the fp32 traces' error against the fp64 GPU run; checks
+    dL/d eps_r at ``n_fd`` seeded cells (and dL/d sigma at the first 8)
+    against central differences (h = 1e-3 eps_r, 1e-3 sigma) of the fp64 loss."""
+    import numpy as np
+    import torch
+
+    from waveforge_amd import workloads
+    from waveforge_amd.adjoint import coefficients, misfit_gradient
+
+    case = workloads.adjoint_case(dev)
+    prob = case.problem
+    dt = prob.dt
+    eps, sig, d_obs = case.eps, case.sigma, case.d_obs
+    n_tx, n = len(prob.tx), prob.nx
+    cu = n_tx * n * n * prob.n_steps
+    misfit_gradient(eps, sig, prob, d_obs)  # warm-up
+    ca64, cb64 = coefficients(eps, sig, dt)
+    ca, cb = ca64.float().contiguous(), cb64.float().contiguous()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d32, _ = prob.forward(ca, cb)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss, ge, gs = misfit_gradient(eps, sig, prob, d_obs)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss64, ge64, gs64 = misfit_gradient(eps, sig, prob, d_obs, fp64_residual=True)
+    torch.cuda.synchronize()
+    t_grad64 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    d64 = prob.forward_fp64(ca64, cb64)
+    torch.cuda.synchronize()
+    t_f64 = time.perf_counter() - t0
+    fp64_err = float(torch.linalg.norm(d32.double() - d64) / torch.linalg.norm(d64))
+    ge, gs = ge.double().cpu(), gs.double().cpu()
+    ge64, gs64 = ge64.double().cpu(), gs64.double().cpu()
+    resid_rel = float(torch.linalg.norm(d64 - d_obs.double()) / torch.linalg.norm(d64))
+    d_obs64 = d_obs.double()
+
+    def fd_loss(e, s):
+        d = prob.forward_fp64(*coefficients(e, s, dt))
+        return 0.5 * float(((d - d_obs64) ** 2).sum())
+
+    rng = np.random.default_rng(12345)
+    cells = [(int(rng.integers(2, n - 2)), int(rng.integers(2, n - 2))) for _ in range(n_fd)]
+    checks = []
+    t0 = time.perf_counter()
+    for k, (i, j) in enumerate(cells):
+        for name, base in (("eps", eps), ("sigma", sig)):
+            if name == "sigma" and k >= 8:
+                continue
+            h = 1e-3 * float(base[i, j])
+            p, m = base.clone(), base.clone()
+            p[i, j] += h
+            m[i, j] -= h
+            lp = fd_loss(p, sig) if name == "eps" else fd_loss(eps, p)
+            lm = fd_loss(m, sig) if name == "eps" else fd_loss(eps, m)
+            checks.append((name, i, j, (lp - lm) / (2 * h)))
+    t_fd = time.perf_counter() - t0
+    out = {"workload": "256^2 head phantom (clot-free model, clot data), 16 TX x 16 RX, 3000 steps, "
+                       "checkpoint every 100, loss 0.5||d_sim - d_obs||^2",
+           "loss": loss, "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+           "forward_mcell_updates_per_s": round(cu / t_fwd / 1e6, 1),
+           "gradient_cell_updates": 3 * cu, "gradient_mcell_updates_per_s": round(3 * cu / t_grad / 1e6, 1),
+           "traces_rel_l2_vs_fp64": fp64_err, "residual_rel_l2": resid_rel,
+           "gradient_fp64_residual_s": round(t_grad64, 4), "loss_fp64_residual": loss64,
+           "fp64_forward_s": round(t_f64, 3), "fd_s": round(t_fd, 2)}
+    for tag, grads in (("fp32", (ge, gs)), ("fp64_residual", (ge64, gs64))):
+        for name, grad in zip(("eps", "sigma"), grads):
+            rows = [(fd, float(grad[i, j])) for (nm, i, j, fd) in checks if nm == name]
+            scale = float(grad.abs().max())
+            rel = [abs(fd - ad) / max(abs(fd), 1e-300) for fd, ad in rows]
+            out[f"fd_{name}_{tag}"] = {
+                "cells": len(rows),
+                "max_rel_err": max(rel),
+                "median_rel_err": float(np.median(rel)),
+                "max_abs_diff_over_grad_max": max(abs(fd - ad) for fd, ad in rows) / scale,
+            }
+    return out
+
+
 def main() -> int:
     args = parse()
     if args.copy_probe:
@@ -292,13 +381,16 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
         base = cpu_baseline(wl, args.cpu_steps)
-    imaging = serving_res = None
+    imaging = serving_res = adjoint = None
     if not args.no_imaging:
         print("[bench] batched imaging section", file=sys.stderr, flush=True)
         imaging = imaging_suite(dev)
     if not args.no_serving:
         print("[bench] serving section", file=sys.stderr, flush=True)
         serving_res = serving_suite(dev, args.serving_requests)
+    if not args.no_adjoint:
+        print("[bench] adjoint section", file=sys.stderr, flush=True)
+        adjoint = adjoint_suite(dev, args.fd_cells)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -325,6 +417,7 @@ def main() -> int:
         "fields_finite": finite,
         "imaging": imaging,
         "serving": serving_res,
+        "adjoint": adjoint,
     }
     print(json.dumps(out))
     return 0

@@ -178,3 +178,25 @@ def rel_l2(a, b) -> float:
     b = torch.as_tensor(b, dtype=torch.float64).reshape(-1)
     den = float(torch.linalg.norm(b))
     return float(torch.linalg.norm(a - b)) / (den if den > 0 else 1.0)
+
+
+def oracle_from_sim(sim, dtype=torch.float32, n_check=10 ** 9):
+    """The oracle of a product ``FDTD2D`` built from the same inputs: Ca/Cb as
+    stored, point/line sources with the product's fp32 waveform tables (fp32)
+    or the tables rebuilt in fp64 by the same ``Waveform`` (fp64, as the
+    reference builds them on a float64 grid), detectors at the same cells."""
+    g = sim.grid
+    srcs = []
+    for src in sim._source_list():
+        if dtype == torch.float32:
+            w = src._waveform_tensor.detach().cpu()
+        else:
+            w = src._waveform.build(src._waveform_tensor.numel(), g.dt, torch.device("cpu"), dtype)
+        srcs.append(OracleSource(src.component, w, src._i_idx.cpu(), src._j_idx.cpu(),
+                                 src._amp.detach().cpu().to(dtype)))
+    det = sim._detectors
+    cells = [] if det is None else [tuple(c) for c in det.cells]
+    ca = None if sim._Ca is None else sim._Ca.detach().cpu().reshape(g.Nx, g.Ny, 1)
+    cb = None if sim._Cb is None else sim._Cb.detach().cpu().reshape(g.Nx, g.Ny, 1)
+    return OracleFDTD2D(g.Nx, g.Ny, g.dx, g.dy, g.dt, Ca=ca, Cb=cb, sources=srcs, detectors=cells, dtype=dtype,
+                        n_check=n_check)

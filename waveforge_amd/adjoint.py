@@ -101,36 +101,55 @@ class TraceProblem:
         return torch.stack([runs.traces(k) for k in range(len(specs))])
 
     # ------------------------------------------------------------------
-    def _recompute(self, ca, cb, start: torch.Tensor, s0: int, L: int) -> torch.Tensor:
-        """States ``H[k] = u_{s0+k}``, k = 0..L, from checkpoint ``start``: one
-        depth-1 launch per step, reading H[k] and writing H[k+1]."""
-        n_tx, nx, ny = len(self.tx), self.nx, self.ny
-        H = torch.empty((L + 1, n_tx, 3, nx, ny), dtype=torch.float32, device=self.device)
-        H[0].copy_(start)
-        structs = []
-        for k in range(L):
-            for r in range(n_tx):
-                rs = RunState(H[k, r, 0], H[k, r, 1], H[k, r, 2], ca, cb, nx, ny, self.coef,
-                              sources=self._tables[r], step=s0 + k)
-                rs.second = (H[k + 1, r, 0], H[k + 1, r, 1], H[k + 1, r, 2])
-                structs.append(rs.struct())
-        arr = (WfRun2d * len(structs))(*structs)
-        table = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(self.device)
-        tile_i = 32
-        tiles = np.concatenate([np.stack([np.full(tile_count(nx, ny, tile_i, 1), r, dtype=np.int32),
-                                          np.arange(tile_count(nx, ny, tile_i, 1), dtype=np.int32)], axis=1)
-                                for r in range(n_tx)])
+    def _history(self, ca, cb):
+        """State history buffer ``H[k] = u_{s0+k}`` (k = 0..K) for one segment
+        and the run table that makes depth-1 launch k read H[k] and write
+        H[k+1] (descriptor step = k; the launch's step_offset adds s0). Built
+        once per backward: descriptors differ only in their state pointers."""
+        n_tx, nx, ny, K = len(self.tx), self.nx, self.ny, self.checkpoint
+        H = torch.empty((K + 1, n_tx, 3, nx, ny), dtype=torch.float32, device=self.device)
+        base = []
+        for r in range(n_tx):
+            rs = RunState(H[0, r, 0], H[0, r, 1], H[0, r, 2], ca, cb, nx, ny, self.coef,
+                          sources=self._tables[r], step=0)
+            rs.second = (H[1, r, 0], H[1, r, 1], H[1, r, 2])
+            base.append(rs.struct())
+        sz = ctypes.sizeof(WfRun2d)
+        tbl = np.tile(np.frombuffer(bytes((WfRun2d * n_tx)(*base)), dtype=np.uint8).reshape(1, n_tx, sz), (K, 1, 1))
+        ptr = tbl.view(np.uint64)  # (K, n_tx, sz/8)
+        cell_bytes = nx * ny * 4
+        k = np.arange(K, dtype=np.uint64)[:, None]
+        rr = np.arange(n_tx, dtype=np.uint64)[None, :]
+        h0 = np.uint64(H.data_ptr())
+        for c, (f_lo, f_hi) in enumerate((("ex", "ex2"), ("ey", "ey2"), ("hz", "hz2"))):
+            src = h0 + ((k * np.uint64(n_tx) + rr) * np.uint64(3) + np.uint64(c)) * np.uint64(cell_bytes)
+            ptr[:, :, getattr(WfRun2d, f_lo).offset // 8] = src
+            ptr[:, :, getattr(WfRun2d, f_hi).offset // 8] = src + np.uint64(3 * n_tx * cell_bytes)
+        tbl.view(np.int32)[:, :, WfRun2d.step.offset // 4] = np.arange(K, dtype=np.int32)[:, None]
+        table = torch.from_numpy(np.ascontiguousarray(tbl).reshape(-1)).to(self.device)
+        nt = tile_count(nx, ny, self.recompute_tile_rows, 1)
+        tiles = np.stack([np.repeat(np.arange(n_tx, dtype=np.int32), nt),
+                          np.tile(np.arange(nt, dtype=np.int32), n_tx)], axis=1)
         tiles_t = torch.from_numpy(np.ascontiguousarray(tiles)).to(self.device)
+        return H, table, tiles_t
+
+    recompute_tile_rows = 32
+
+    def _recompute(self, hist, start: torch.Tensor, s0: int, L: int) -> torch.Tensor:
+        """Fill ``H[0..L]`` with ``u_{s0}..u_{s0+L}`` from checkpoint ``start``."""
+        H, table, tiles_t = hist
+        n_tx = len(self.tx)
+        H[0].copy_(start)
         lib = _native.load()
         stream = _native.stream_handle(self.device)
         sz = ctypes.sizeof(WfRun2d)
+        a = WfTbArgs()
+        a.tiles = tiles_t.data_ptr()
+        a.n_runs, a.n_tiles = n_tx, int(tiles_t.shape[0])
+        a.tile_i, a.tile_j = self.recompute_tile_rows, _native.tile_cols(1)
+        a.depth, a.parity, a.step_offset, a.has_materials = 1, 0, s0, 1
         for k in range(L):
-            a = WfTbArgs()
             a.runs = table.data_ptr() + k * n_tx * sz
-            a.tiles = tiles_t.data_ptr()
-            a.n_runs, a.n_tiles = n_tx, int(tiles.shape[0])
-            a.tile_i, a.tile_j = tile_i, _native.tile_cols(1)
-            a.depth, a.parity, a.step_offset, a.has_materials = 1, 0, 0, 1
             _native.check(lib.wf_fdtd2d_tb_advance(ctypes.byref(a), ctypes.c_void_p(stream)), "recompute")
         return H
 
@@ -157,27 +176,33 @@ class TraceProblem:
         a.nx, a.ny, a.n_runs, a.n_rx = nx, ny, n_tx, len(self.rx)
         c = self.coef
         a.dh, a.dx, a.dy, a.cmx, a.cmy = c.dh, c.dx, c.dy, c.cmx, c.cmy
+        hist = self._history(ca, cb)
+        H = hist[0]
+        step_bytes = n_tx * 3 * nx * ny * 4
+        h0 = H.data_ptr()
         for seg in range(len(ckpts) - 1, -1, -1):
             s0 = seg * K
             L = min(K, self.n_steps - s0)
-            H = self._recompute(ca, cb, ckpts[seg], s0, L)
+            self._recompute(hist, ckpts[seg], s0, L)
             for k in range(L - 1, -1, -1):
                 a.step = s0 + k
-                a.ex_prev = H[k, :, 0].data_ptr()
-                a.ey_prev = H[k, :, 1].data_ptr()
-                a.hz_next = H[k + 1, :, 2].data_ptr()
+                a.ex_prev = h0 + k * step_bytes
+                a.ey_prev = h0 + k * step_bytes + nx * ny * 4
+                a.hz_next = h0 + (k + 1) * step_bytes + 2 * nx * ny * 4
                 _native.check(lib.wf_adjoint_step(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_adjoint_step")
-            del H
+        del hist, H
         return gca.sum(0), gcb.sum(0)
 
 
 class _Traces(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, eps, sigma, problem):
+    def forward(ctx, eps, sigma, problem, fp64):
         with torch.no_grad():
             ca64, cb64 = coefficients(eps.double(), sigma.double(), problem.dt)
             ca, cb = ca64.float().contiguous(), cb64.float().contiguous()
             traces, ckpts = problem.forward(ca, cb)
+            if fp64:
+                traces = problem.forward_fp64(ca64, cb64)
         ctx.problem, ctx.ca, ctx.cb, ctx.ckpts = problem, ca, cb, ckpts
         ctx.save_for_backward(eps, sigma)
         return traces
@@ -191,20 +216,26 @@ class _Traces(torch.autograd.Function):
             s = sigma.detach().double().requires_grad_(True)
             ca64, cb64 = coefficients(e, s, ctx.problem.dt)
             ge, gs = torch.autograd.grad([ca64, cb64], [e, s], [gca, gcb])
-        return ge.to(eps.dtype), gs.to(sigma.dtype), None
+        return ge.to(eps.dtype), gs.to(sigma.dtype), None, None
 
 
-def traces(eps: torch.Tensor, sigma: torch.Tensor, problem: TraceProblem) -> torch.Tensor:
+def traces(eps: torch.Tensor, sigma: torch.Tensor, problem: TraceProblem, fp64: bool = False) -> torch.Tensor:
     """Differentiable ``(n_tx, n_rx, n_steps)`` Hz traces of ``problem`` for
-    per-cell ``eps`` and ``sigma`` maps ``(nx, ny)`` on the device."""
-    return _Traces.apply(eps, sigma, problem)
+    per-cell ``eps`` and ``sigma`` maps ``(nx, ny)`` on the device.
+
+    With ``fp64`` the returned traces come from the double-precision GPU step
+    (fp64 coefficients), so a misfit whose residual is much smaller than the
+    traces themselves is not dominated by fp32 rounding; checkpoints and the
+    adjoint still run in fp32 (their rounding enters the gradient only
+    relative to the states, not to the residual)."""
+    return _Traces.apply(eps, sigma, problem, fp64)
 
 
-def misfit_gradient(eps, sigma, problem: TraceProblem, d_obs: torch.Tensor):
+def misfit_gradient(eps, sigma, problem: TraceProblem, d_obs: torch.Tensor, fp64_residual: bool = False):
     """``L = 0.5 ||d_sim - d_obs||^2`` (fp64) and its gradient w.r.t. eps, sigma."""
     e = eps.detach().clone().requires_grad_(True)
     s = sigma.detach().clone().requires_grad_(True)
-    d = traces(e, s, problem)
+    d = traces(e, s, problem, fp64_residual)
     loss = 0.5 * ((d.double() - d_obs.double()) ** 2).sum()
     loss.backward()
     return float(loss.detach()), e.grad, s.grad

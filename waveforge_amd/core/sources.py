@@ -326,6 +326,7 @@ class PlaneSource:
             raise ValueError(f"position={position} out of range [0, {limit}) for plane={plane!r}")
         self._component, self._plane, self._position = component, plane, position
         self._N_steps, self._n_cells = N_steps, n_cells
+        self._waveform = waveform
         self._waveform_tensor = waveform.build(N_steps, grid.dt, grid.device, grid.dtype)
 
     def step(self, fields_dict, n: int) -> None:
@@ -372,6 +373,7 @@ class PointSource:
         if not (0 <= k < grid.Nz):
             raise ValueError(f"k={k} out of range [0, {grid.Nz})")
         self._component, self._N_steps = component, N_steps
+        self._waveform = waveform
         self._waveform_tensor = waveform.build(N_steps, grid.dt, grid.device, grid.dtype)
         dev = grid.device
         self._i_idx = torch.tensor([i], dtype=torch.long, device=dev)
@@ -424,6 +426,7 @@ class LineSource:
             i_vals = torch.full((n_cells,), position, dtype=torch.long, device=dev)
             j_vals = torch.arange(start, stop, dtype=torch.long, device=dev)
         self._component, self._N_steps, self._n_cells = component, N_steps, n_cells
+        self._waveform = waveform
         self._waveform_tensor = waveform.build(N_steps, grid.dt, grid.device, grid.dtype)
         self._i_idx, self._j_idx = i_vals, j_vals
         self._k_idx = torch.zeros(n_cells, dtype=torch.long, device=dev)

@@ -71,10 +71,11 @@ class Fp64Runs:
                 self._keep += [ca, cb]
                 r.ca, r.cb = ca.data_ptr(), cb.data_ptr()
             self._source_table(r, sp.sources, nx, ny, k)
+            r.trace_n0 = int(getattr(sp, "trace_n0", 0))
             if sp.receivers:
                 if any(not (0 <= i < nx and 0 <= j < ny) for i, j in sp.receivers):
                     raise ValueError(f"run {k}: receiver outside the grid")
-                cells =torch.tensor([i * ny + j for i, j in sp.receivers], dtype=torch.int32, device=dev)
+                cells = torch.tensor([i * ny + j for i, j in sp.receivers], dtype=torch.int32, device=dev)
                 tr = torch.zeros((len(sp.receivers), max(1, int(sp.n_record))), dtype=torch.float64, device=dev)
                 self._keep.append(cells)
                 r.det_cell, r.traces = cells.data_ptr(), tr.data_ptr()
@@ -136,3 +137,56 @@ class Fp64Runs:
 
     def traces(self, k: int) -> torch.Tensor:
         return self.trace_bufs[k]
+
+
+class _TwinSpec:
+    def __init__(self, nx, ny, ca, cb, sources, receivers, n_record, trace_n0):
+        self.nx, self.ny, self.ca, self.cb = nx, ny, ca, cb
+        self.sources, self.receivers, self.n_record, self.trace_n0 = sources, receivers, n_record, trace_n0
+
+
+def fp64_twin(sim) -> Fp64Runs:
+    """The same simulation as an fp32 ``core.FDTD2D`` (one batch member),
+    set up as the reference sets it up on a float64 grid: waveform tables
+    rebuilt in fp64 (``Waveform.build`` with the grid dtype,
+    ``src/core/sources.py:112-138``), Ca/Cb as stored (the reference computes
+    them in fp32 and converts, ``src/core/materials.py:218-250``), fp64 update
+    and Mur coefficients, the same sources, cells and detectors, zero initial
+    fields and step 0. Advance it with ``advance`` and compare."""
+    from .core.sources import PlaneSource
+
+    g = sim.grid
+    nx, ny = g.Nx, g.Ny
+    if sim.fields.Hz.dim() == 4:
+        raise ValueError("fp64_twin: batched FieldSets are not supported")
+    srcs = []
+    for src in sim._source_list():
+        comp = src.component
+        if comp not in _native.WF_COMP:
+            continue  # Ez, Hx, Hy never enter the 2D TM update
+        wf = getattr(src, "_waveform", None)
+        n = src._waveform_tensor.numel()
+        wave = wf.build(n, g.dt, g.device, torch.float64) if wf is not None else src._waveform_tensor.double()
+        if isinstance(src, PlaneSource):
+            ii, jj, aa = src.cells(nx, ny)
+        else:
+            ii, jj, aa = src._i_idx.cpu(), src._j_idx.cpu(), src._amp.detach().double().cpu()
+        for i, j, a in zip(ii.tolist(), jj.tolist(), aa.tolist()):
+            srcs.append((comp, int(i), int(j), wave, float(a)))
+    det = sim._detectors
+    receivers, n_record, n0 = [], 0, 0
+    if det is not None and len(det):
+        receivers = [(int(c) // ny, int(c) % ny) for c in det.cell_index.cpu().tolist()]
+        n_record, n0 = det.traces.shape[-1], det.start_step
+    ca = None if sim._Ca is None else sim._Ca.reshape(nx, ny)
+    cb = None if sim._Cb is None else sim._Cb.reshape(nx, ny)
+    spec = _TwinSpec(nx, ny, ca, cb, srcs, receivers, n_record, n0)
+    return Fp64Runs([spec], dx=g.dx, dy=g.dy, dt=g.dt, device=g.device)
+
+
+def rel_l2(a: torch.Tensor, ref: torch.Tensor) -> float:
+    """``||a - ref|| / ||ref||`` in fp64 (0 if both are zero)."""
+    a, ref = a.double(), ref.double()
+    den = float(torch.linalg.norm(ref))
+    num = float(torch.linalg.norm(a - ref))
+    return num / den if den > 0 else num

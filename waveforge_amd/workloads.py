@@ -199,3 +199,37 @@ def imaging_dt(dx: float = 1e-3) -> float:
     from .core.grid import compute_stable_dt
 
     return compute_stable_dt(dx, dx, courant=0.99)
+
+
+# ---------------------------------------------------------------------------
+# Adjoint workload: trace-misfit gradient on a head phantom
+# ---------------------------------------------------------------------------
+
+@dataclass
+class AdjointCase:
+    problem: object             # adjoint.TraceProblem
+    eps: torch.Tensor           # clot-free eps_r, fp64 on the device (the model being differentiated)
+    sigma: torch.Tensor
+    d_obs: torch.Tensor         # fp32 traces of the clot phantom (the "observed" data)
+    phantom: HeadPhantom        # the clot phantom
+
+
+def adjoint_case(device="cuda", n: int = 256, n_steps: int = 3000, seed: int = 0, checkpoint: int = 100,
+                 depth: int = 6, n_elements: int = 16) -> AdjointCase:
+    """256^2 head phantom, 16 TX x 16 RX, 3000 steps: d_obs from the clot
+    phantom, the model is its clot-free twin (same seed, same head), loss
+    0.5 ||d_sim - d_obs||^2, checkpoints every ``checkpoint`` steps."""
+    from .adjoint import TraceProblem, coefficients
+
+    dev = torch.device(device)
+    dt = imaging_dt()
+    ph = head_phantom(n, n_elements, seed=seed, clot=True)
+    free = head_phantom(n, n_elements, seed=seed, clot=False)
+    wave = ricker_1ghz().build(n_steps + 1, dt, dev)
+    prob = TraceProblem(n, n, 1e-3, dt, list(ph.antennas), list(ph.antennas), wave, n_steps,
+                        checkpoint=checkpoint, depth=depth)
+    e_c = torch.from_numpy(ph.eps).to(dev)
+    s_c = torch.from_numpy(ph.sigma).to(dev)
+    ca, cb = coefficients(e_c, s_c, dt)
+    d_obs, _ = prob.forward(ca.float().contiguous(), cb.float().contiguous())
+    return AdjointCase(prob, torch.from_numpy(free.eps).to(dev), torch.from_numpy(free.sigma).to(dev), d_obs, ph)
