@@ -269,20 +269,75 @@ class ImagingServer:
             self.poll(now)
         torch.cuda.synchronize(self.device)
         self.poll(now)
-        total = now()
-        lat = np.array([r.latency for r in self.done if r.latency is not None])
-        cu = sum(r.cell_updates for r in self.done)
-        return {
-            "requests": len(self.done),
-            "failed": int(sum(r.failed for r in self.done)),
-            "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
-            "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
-            "mean_ms": float(lat.mean() * 1e3) if lat.size else None,
-            "wall_s": total,
-            "images_per_s": len(self.done) / total if total > 0 else None,
-            "mcell_updates_per_s": cu / total / 1e6 if total > 0 else None,
-            "rounds": self.rounds,
-        }
+        return _stats(self.done, now(), self.rounds)
+
+
+def _stats(done, total: float, rounds: int = 0) -> dict:
+    lat = np.array([r.latency for r in done if r.latency is not None])
+    cu = sum(r.cell_updates for r in done)
+    return {
+        "requests": len(done),
+        "failed": int(sum(r.failed for r in done)),
+        "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
+        "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
+        "mean_ms": float(lat.mean() * 1e3) if lat.size else None,
+        "wall_s": total,
+        "images_per_s": len(done) / total if total > 0 else None,
+        "mcell_updates_per_s": cu / total / 1e6 if total > 0 else None,
+        "rounds": rounds,
+    }
+
+
+def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e-3, device=None,
+                realtime: bool = True, v_medium: Optional[float] = None, n_check: int = 100) -> dict:
+    """The plain one-run-per-launch GPU path as a server, for comparison.
+
+    Requests are taken one at a time in arrival order; every TX of a request
+    is its own ``core.FDTD2D`` run (the temporally blocked kernel over that
+    one grid, one run per launch sequence, stability checked every
+    ``n_check`` steps), as a user gets by moving the reference's per-TX
+    ``run_mimo`` loop (``examples/brain_mimo_imaging.py:149-204``) onto the
+    GPU; then DAS. Fills ``image``, ``traces``, ``finished`` of each request
+    and returns the same statistics as ``ImagingServer.serve``."""
+    from .core import DetectorSet, FDTD2D, FieldSet, MurABC, PointSource, SourceCollection, YeeGrid
+    from .core.fdtd2d import SimulationDivergedError
+
+    dev = torch.device(device) if device is not None else torch.device("cuda")
+    v = v_medium if v_medium is not None else C0 / math.sqrt(40.0)
+    wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9)
+    pending = sorted(requests, key=lambda r: r.arrival)
+    t0 = time.perf_counter()
+    now = lambda: time.perf_counter() - t0  # noqa: E731
+    for req in pending:
+        if realtime:
+            wait = req.arrival - now()
+            if wait > 0:
+                time.sleep(wait)
+        req.admitted = now()
+        nx, ny = req.eps.shape
+        grid = YeeGrid(nx, ny, dx, dx, courant=0.99, dt=dt, device=dev)
+        ca, cb = coefficients_from_maps(req.eps, req.sigma, dt, device=dev)
+        n_ant = len(req.antennas)
+        req.traces = torch.empty((n_ant, n_ant, req.n_steps), dtype=torch.float32, device=dev)
+        req.failed = False
+        for t, (i, j) in enumerate(req.antennas):
+            f = FieldSet(grid)
+            det = DetectorSet(req.antennas, req.n_steps, grid=grid)
+            src = PointSource(wave, i, j, "Hz", grid=grid, N_steps=req.n_steps)
+            sim = FDTD2D(grid, f, MurABC(grid, f.Hz), SourceCollection([src]), Ca=ca, Cb=cb, detectors=det,
+                         n_check=n_check)
+            try:
+                sim.run(req.n_steps)
+            except SimulationDivergedError:
+                req.failed = True
+            req.traces[t].copy_(det.traces)
+        ax = torch.tensor([a[0] * dx for a in req.antennas], dtype=torch.float64, device=dev)
+        ay = torch.tensor([a[1] * dx for a in req.antennas], dtype=torch.float64, device=dev)
+        req.image = torch.empty((nx, ny), dtype=torch.float64, device=dev)
+        run_das(req.traces, ax, ay, req.image, pitch_x=dx, pitch_y=dx, v_medium=v, dt=dt, clip=True)
+        torch.cuda.current_stream(dev).synchronize()
+        req.finished = now()
+    return _stats(pending, now())
 
 
 def poisson_requests(n: int, rate_hz: float, *, n_min: int = 128, n_max: int = 1024, elements=(8, 16, 32),
