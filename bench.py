@@ -11,7 +11,9 @@ cell-updates per second over the timed steps, with all inputs resident in HBM.
 ``roofline``: the dominant kernel is the temporally blocked stepper. Its
 algorithmic traffic is 32 B per cell-update (read Ex, Ey, Hz, Ca, Cb, write
 Ex, Ey, Hz); ``achieved`` = algorithmic bytes per launch / mean launch time,
-both measured live with HIP events on the launch stream over the timed region,
+both measured live with HIP events on the launch stream around every 4th
+launch of the timed region (a uniform sample that keeps the event records'
+own cost out of the timed stream; the warm-up runs with the same sampling),
 against the 8 TB/s HBM peak. With temporal blocking the kernel moves fewer HBM
 bytes than the algorithm nominally needs, so ``frac`` may exceed 1.
 
@@ -38,6 +40,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Mcell-updates/s + %HBM roofline (2D TM fp32); p50/p99 per-image latency"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE = 32  # material path: read Ex, Ey, Hz, Ca, Cb; write Ex, Ey, Hz
+EVENT_STRIDE = 4  # launches of the timed region bracketed by HIP events: every 4th
 
 
 def parse():
@@ -48,6 +51,7 @@ def parse():
     p.add_argument("--size", type=int, default=4096)
     p.add_argument("--depth", type=int, default=None)
     p.add_argument("--tile", type=int, default=None, help="tile rows (tile columns are 64 - 2*depth)")
+    p.add_argument("--autotune", action="store_true", help="let the autotuner pick tile rows and depth")
     p.add_argument("--cpu-steps", type=int, default=8, help="timed steps of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-imaging", action="store_true", help="skip the batched-imaging section")
@@ -344,15 +348,26 @@ def main() -> int:
         kw["depth"] = args.depth
     if args.tile:
         kw["tile"] = args.tile
+    if args.autotune:
+        kw["depth"], kw["tile"] = "auto", "auto"
     total = args.warmup + args.steps
     print(f"[bench] building {args.size}x{args.size} blob medium ({total} steps)", file=sys.stderr, flush=True)
     wl = workloads.blob_medium(dev, n=args.size, n_steps=total, **kw)
     sim = wl.sim
+    # the timing events of the timed region are created and recorded once
+    # before it (the first record of a fresh HIP event costs far more than a
+    # reuse; hundreds of them stalled the first timed launch by tens of ms)
+    n_events = 2 * (args.steps // EVENT_STRIDE + 64)
+    pool = [torch.cuda.Event(enable_timing=True) for _ in range(n_events)]
+    for ev in pool:
+        ev.record()
+    TBPlan.event_log, TBPlan.event_stride = [], EVENT_STRIDE
     sim.run(args.warmup)
     torch.cuda.synchronize()
+    TBPlan.event_pool = pool
     print(f"[bench] warm-up {args.warmup} steps done; timing {args.steps}", file=sys.stderr, flush=True)
     log: list = []
-    TBPlan.event_log = log
+    TBPlan.event_log, TBPlan._event_count = log, 0
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -360,7 +375,8 @@ def main() -> int:
     sim.run(args.steps)
     end.record()
     torch.cuda.synchronize()
-    TBPlan.event_log = None
+    TBPlan.event_log, TBPlan.event_stride, TBPlan.event_pool = None, 1, None
+    first_launch_ms = start.elapsed_time(log[0][2]) if log else None
     ms = start.elapsed_time(end)
     cells = args.size * args.size
     value = args.steps * cells / (ms / 1e3) / 1e6
@@ -378,14 +394,18 @@ def main() -> int:
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
         "kernel": "k_tb (temporally blocked stepper)",
-        "launches": len(log),
+        "launches_timed": len(log),
+        "launch_sample": f"every {EVENT_STRIDE}th launch of the timed region",
         "mean_launch_us": round(mean_ms * 1e3, 2),
+        "first_launch_offset_ms": None if first_launch_ms is None else round(first_launch_ms, 3),
         "bytes_per_launch_algorithmic": mean_bytes,
         "depths": depths,
     }
     traffic_file = ROOT / "profiles" / "k_tb_blob4096_traffic.json"
-    tile_rows = sim._tile or 32
-    depth = sim._depth or 6
+    tile_rows, depth = sim.launch_shape
+    from waveforge_amd import autotune
+
+    tuned = autotune.tune_runs(sim._build_runs(), fast_div=sim._fast_div)
     if traffic_file.exists():
         tinfo = json.loads(traffic_file.read_text())
         if (tinfo.get("grid"), tinfo.get("tile_rows"), tinfo.get("depth")) == (args.size, tile_rows, depth):
@@ -430,6 +450,8 @@ def main() -> int:
             "depth": depth,
         },
         "roofline": roofline,
+        "autotune": {"choice": [tuned.tile_rows, tuned.depth], "used": [tile_rows, depth],
+                     "rates_mcell_per_s": tuned.rates},
         "cpu_baseline": base,
         "fields_finite": finite,
         "imaging": imaging,

@@ -43,6 +43,17 @@ def test_fdtd2d_construction_checks():
         core.FDTD2D(g, f, b, Ca=torch.ones(8, 8, 1))
 
 
+def test_launch_shape_arguments_are_checked():
+    g = core.YeeGrid(16, 16, 1e-3, 1e-3, device="cpu")
+    f = core.FieldSet(g)
+    b = core.MurABC(g, f.Hz)
+    for bad in (dict(depth=0), dict(depth=9), dict(depth="deep"), dict(tile=2.5)):
+        with pytest.raises(ValueError):
+            core.FDTD2D(g, f, b, None, **bad)
+    core.FDTD2D(g, f, b, None, depth="auto", tile="auto")
+    core.FDTD2D(g, f, b, None, depth=8, tile=48)
+
+
 def test_product_refuses_cpu_tensors():
     g = core.YeeGrid(8, 8, 1e-3, 1e-3, device="cpu")
     f = core.FieldSet(g)

@@ -1,0 +1,125 @@
+"""Launch-shape autotuner for the temporally blocked stepper.
+
+The stepper's two free parameters are the core tile height (``tile_rows``,
+one of the shapes the library reports through ``wf_tb_tile_shapes``) and the
+temporal depth (steps per launch, 1..8; the tile width follows as
+64 - 2*depth). Their best values depend on the set of grids in flight (a
+4096^2 grid and 512 small grids fill the 256 CUs differently) and on whether
+the runs carry per-cell coefficients. ``tune_runs`` times every admissible
+pair on scratch copies of the caller's runs (same shapes, same Ca/Cb, fresh
+field buffers, no sources or detectors), with HIP events around each launch
+on the launch stream, and returns the fastest. Results are cached per
+(device, grid shapes, materials, division mode) for the process, and can be
+saved to and loaded from a JSON file.
+
+Admissible means the caller's own runs meet the kernel's per-tile source and
+detector limits at every depth up to the candidate's, since a run of ``n``
+steps at depth ``d`` also launches shallower blocks.
+"""
+
+from __future__ import annotations
+
+import json
+from collections import Counter
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional, Sequence
+
+import torch
+
+from . import _native
+from .engine import RunState, TBPlan, tb_limits_ok
+
+DEPTHS = tuple(range(1, _native.TB_MAX_DEPTH + 1))
+
+
+@dataclass
+class TuneResult:
+    tile_rows: int
+    depth: int
+    rates: dict = field(default_factory=dict)   # "tile_rows,depth" -> Mcell-updates/s
+
+    def as_dict(self) -> dict:
+        return {"tile_rows": self.tile_rows, "depth": self.depth, "rates": self.rates}
+
+
+_CACHE: dict = {}
+
+
+def _key(runs: Sequence[RunState], device, fast_div: bool) -> str:
+    shapes = sorted(Counter((r.nx, r.ny) for r in runs).items())
+    mats = int(runs[0].ca is not None)
+    c = runs[0].coef
+    name = torch.cuda.get_device_name(device)
+    return json.dumps([name, shapes, mats, bool(fast_div), c.dx, c.dy])
+
+
+def _scratch(runs: Sequence[RunState], device) -> list[RunState]:
+    g = torch.Generator(device=device).manual_seed(0)
+    out = []
+    for r in runs:
+        f = [(torch.rand((r.nx, r.ny), generator=g, device=device) - 0.5) * 1e-2 for _ in range(6)]
+        s = RunState(f[0], f[1], f[2], r.ca, r.cb, r.nx, r.ny, r.coef)
+        s.second = (f[3], f[4], f[5])
+        out.append(s)
+    return out
+
+
+def tune_runs(runs: Sequence[RunState], device=None, *, fast_div: bool = True,
+              tile_rows: Optional[Sequence[int]] = None, depths: Sequence[int] = DEPTHS,
+              launches: int = 6, use_cache: bool = True) -> TuneResult:
+    """Fastest ``(tile_rows, depth)`` for advancing ``runs`` together."""
+    if not runs:
+        raise ValueError("no runs to tune")
+    device = torch.device(device) if device is not None else runs[0].ex.device
+    key = _key(runs, device, fast_div)
+    if use_cache and key in _CACHE:
+        return _CACHE[key]
+    tiles = list(tile_rows) if tile_rows is not None else _native.tile_rows()
+    scratch = _scratch(runs, device)
+    stream = _native.stream_handle(device)
+    launches += launches % 2
+    best, rates = None, {}
+    for ti in tiles:
+        plan = TBPlan(scratch, ti, device, fast_div=fast_div)
+        ok_depth = 0
+        for d in range(1, max(depths) + 1):
+            if not all(tb_limits_ok(r, ti, d) for r in runs):
+                break
+            ok_depth = d
+        for d in depths:
+            if d > ok_depth:
+                continue
+            plan.tiles(d)
+            for k in range(2):  # warm-up: code object, tile table, caches
+                plan.launch(d, k % 2, 0, stream)
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for k in range(launches):
+                plan.launch(d, k % 2, 0, stream)
+            ev1.record()
+            ev1.synchronize()
+            sec = ev0.elapsed_time(ev1) / 1e3
+            rate = plan.n_cells * d * launches / sec / 1e6
+            rates[f"{ti},{d}"] = round(rate, 1)
+            if best is None or rate > best[0]:
+                best = (rate, ti, d)
+    if best is None:
+        raise ValueError("no admissible tile shape: too many sources or detectors per tile")
+    res = TuneResult(best[1], best[2], rates)
+    _CACHE[key] = res
+    return res
+
+
+def save(path) -> None:
+    """Write the process cache to a JSON file."""
+    Path(path).write_text(json.dumps({k: v.as_dict() for k, v in _CACHE.items()}, indent=1))
+
+
+def load(path) -> int:
+    """Merge tuning results from a JSON file into the process cache; returns
+    the number of entries read."""
+    data = json.loads(Path(path).read_text())
+    for k, v in data.items():
+        _CACHE[k] = TuneResult(int(v["tile_rows"]), int(v["depth"]), dict(v.get("rates", {})))
+    return len(data)

@@ -72,8 +72,8 @@ class FDTD2D:
         stability_threshold: float = 1e10,
         n_check: int = 100,
         detectors: Optional[DetectorSet] = None,
-        depth: Optional[int] = None,
-        tile: Optional[int] = None,
+        depth=None,
+        tile=None,
         fast_div: bool = True,
     ) -> None:
         if grid.Nx < 4:
@@ -101,8 +101,14 @@ class FDTD2D:
         self._threshold = stability_threshold
         self._n_check = n_check
         self._detectors = detectors
+        for name, v, ok in (("depth", depth, range(1, _native.TB_MAX_DEPTH + 1)), ("tile", tile, None)):
+            if v is not None and v != "auto" and not isinstance(v, int):
+                raise ValueError(f"{name} must be an int, None or 'auto', got {v!r}")
+            if ok is not None and isinstance(v, int) and v not in ok:
+                raise ValueError(f"{name} must be in 1..{_native.TB_MAX_DEPTH}, got {v}")
         self._depth = depth
         self._tile = tile
+        self.launch_shape = None
         self._fast_div = fast_div
         self.steps_completed: int = 0
         self.elapsed_time: float = 0.0
@@ -286,13 +292,33 @@ class FDTD2D:
         self._device_check()
         runs = self._build_runs()
         inert = self._src_table.inert if self._src_table is not None else []
-        tile = self._tile or DEFAULT_TILE_ROWS
-        depth = self._depth or DEFAULT_DEPTH
+        tile, depth = self._launch_shape(runs, inert)
         key = (self._cache_key, tile, depth)
         if getattr(self, "_tb_ok_key", None) != key:
             self._tb_ok = not inert and all(tb_limits_ok(r, tile, d) for r in runs for d in range(1, depth + 1))
             self._tb_ok_key = key
         return (runs, tile, depth) if self._tb_ok else None
+
+    def _launch_shape(self, runs, inert) -> tuple[int, int]:
+        """(tile rows, depth): as given, the defaults, or, for ``"auto"``, the
+        autotuner's choice for these grids (timed once per shape set)."""
+        tile = self._tile or DEFAULT_TILE_ROWS
+        depth = self._depth or DEFAULT_DEPTH
+        if "auto" in (self._tile, self._depth) and not inert:
+            if getattr(self, "_tuned_key", None) != self._cache_key:
+                from .. import autotune
+
+                res = autotune.tune_runs(
+                    runs, fast_div=self._fast_div,
+                    tile_rows=None if self._tile == "auto" else [tile],
+                    depths=autotune.DEPTHS if self._depth == "auto" else [depth])
+                self._tuned, self._tuned_key = res, self._cache_key
+            tile, depth = self._tuned.tile_rows, self._tuned.depth
+        elif "auto" in (self._tile, self._depth):
+            tile = DEFAULT_TILE_ROWS if self._tile == "auto" else tile
+            depth = DEFAULT_DEPTH if self._depth == "auto" else depth
+        self.launch_shape = (tile, depth)
+        return tile, depth
 
     def _run_async(self, n_steps: int, setup) -> None:
         """``n_steps`` steps with the stability checks judged on the device.

@@ -328,9 +328,16 @@ class TBPlan:
 
     ``TBPlan.event_log``: when set to a list, every launch appends
     ``(depth, n_cells, start_event, end_event)`` with HIP events recorded on the
-    launch stream around it (used by the benchmark to time the kernel live)."""
+    launch stream around it (used by the benchmark to time the kernel live);
+    with ``event_stride`` k > 1 only every k-th launch is bracketed, which
+    keeps the event records' own cost out of the timed stream. ``event_pool``
+    (a list of timing events, consumed from the end) supplies the events, so
+    a caller can create and first-record them outside its timed region."""
 
     event_log: Optional[list] = None
+    event_stride: int = 1
+    event_pool: Optional[list] = None
+    _event_count: int = 0
 
     def __init__(self, runs: Sequence[RunState], tile_i: int, device: torch.device, fast_div: bool = True) -> None:
         self.tile_i = int(tile_i)
@@ -387,8 +394,16 @@ class TBPlan:
         lib = _native.load()
         log = TBPlan.event_log
         if log is not None:
-            ev0 = torch.cuda.Event(enable_timing=True)
-            ev1 = torch.cuda.Event(enable_timing=True)
+            TBPlan._event_count += 1
+            if (TBPlan._event_count - 1) % max(1, TBPlan.event_stride):
+                log = None
+        if log is not None:
+            pool = TBPlan.event_pool
+            if pool and len(pool) >= 2:
+                ev0, ev1 = pool.pop(), pool.pop()
+            else:
+                ev0 = torch.cuda.Event(enable_timing=True)
+                ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
         _native.check(lib.wf_fdtd2d_tb_advance(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_fdtd2d_tb_advance")
         if log is not None:

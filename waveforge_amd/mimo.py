@@ -61,7 +61,7 @@ class BatchedRuns:
     """Device state and launch plan of a set of independent runs."""
 
     def __init__(self, specs: Sequence[RunSpec], *, dx: float, dy: float, dt: float, device=None,
-                 tile_rows: int = 32, depth: int = 6, n_check: int = 500, stability_threshold: float = 1e10,
+                 tile_rows=32, depth=6, n_check: int = 500, stability_threshold: float = 1e10,
                  fast_div: bool = True) -> None:
         if not specs:
             raise ValueError("no runs")
@@ -90,6 +90,15 @@ class BatchedRuns:
                 rs.trace_n0 = 0
             rs.step = 0
             self.runs.append(rs)
+        self.tuned = None
+        if "auto" in (tile_rows, depth):
+            from . import autotune
+
+            self.tuned = autotune.tune_runs(self.runs, self.device, fast_div=fast_div,
+                                            tile_rows=None if tile_rows == "auto" else [tile_rows],
+                                            depths=autotune.DEPTHS if depth == "auto" else [depth])
+            self.tile_rows, self.depth = tile_rows, depth = self.tuned.tile_rows, self.tuned.depth
+        for k, rs in enumerate(self.runs):
             for d in range(1, depth + 1):
                 if not tb_limits_ok(rs, tile_rows, d):
                     raise ValueError(f"run {k}: too many sources or receivers per tile for the batched kernel")
