@@ -263,6 +263,40 @@ typedef struct wf_das2d_args {
 
 int wf_das2d(const wf_das2d_args *args, hipStream_t stream);
 
+/* Envelope delay-and-sum image with layered-medium travel times: replaces
+ * AntennaRing.compute_das_image (src/datasets/brain/antenna.py:262-333), the
+ * DAS the dataset generator runs on background-subtracted signals
+ * (src/datasets/generator.py:356-370). Stages (bitmask `stages`):
+ *   1  env[t, n] = |analytic signal of traces[t, :]|, the analytic signal of
+ *      SciPy's hilbert (antenna.py:294): real part x, imaginary part the
+ *      circular convolution of x with hilbert_k (n_t fp64 values, host
+ *      supplied), evaluated in fp64, stored fp32;
+ *   2  tt[a, iy, ix] = one-way travel time in samples from antenna a to
+ *      pixel (px[ix], py[iy]) (fp64, the reference's expression order, stored
+ *      fp32): through concentric layers of radius layer_r (mm, outermost first)
+ *      and permittivity layer_eps around (cx, cy) (antenna.py:218-260), or in
+ *      free space when n_layers is 0 (antenna.py:314-321); speed 3e8 m/s;
+ *   4  image[iy, ix] = (sum over tx-major, rx-minor pairs of
+ *      env[tx, rx][rint(tt[tx] + tt[rx])], indices outside [0, n_t) adding
+ *      nothing, accumulated in fp64)^2, stored fp32 (antenna.py:324-333).
+ * Coordinates in mm, dt in seconds. Stages may run separately (e.g. stage 2
+ * once per geometry, stages 1|4 per signal set). */
+typedef struct wf_das_env_args {
+    const float *traces;          /* (n_ant, n_ant, n_t) fp32 */
+    const double *hilbert_k;      /* n_t */
+    float *env;                   /* (n_ant, n_ant, n_t) fp32 */
+    float *tt;                    /* (n_ant, img_h, img_w) fp32 */
+    float *image;                 /* (img_h, img_w) fp32 */
+    const double *ant_x, *ant_y;  /* n_ant, mm */
+    const double *px, *py;        /* img_w, img_h pixel coordinates, mm */
+    const double *layer_r, *layer_eps;
+    double cx, cy;                /* layer centre, mm */
+    double dt;
+    int32_t n_ant, n_t, img_w, img_h, n_layers, stages;
+} wf_das_env_args;
+
+int wf_das_envelope(const wf_das_env_args *args, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,8 @@ Python) and records small input/output vectors for what the oracle's
 mathematical checks cannot pin at the bit level: the exact eager-op
 trajectory of ``FDTD2D.step`` (fields and detector traces), the ``Waveform.build``
 tables, and the DAS images of ``examples/brain_mimo_imaging.py`` and
-``examples/09_brain_clot_dataset_sample.py``. Everything runs on the CPU in
+``examples/09_brain_clot_dataset_sample.py``, and the envelope / layered-medium
+DAS of ``AntennaRing.compute_das_image``. Everything runs on the CPU in
 fp32, as the reference's tests do. Output: ``tests/golden/reference_fixtures.npz``.
 """
 
@@ -119,6 +120,22 @@ def main() -> None:
     out["das09_traces"], out["das09_antennas"] = S9, np.array(ants9, dtype=np.int64)
     out["das09_dt"] = np.array(g9.dt)
     out["das09_image"] = ex9.das_backprojection(S9, ants9, g9, 500)
+
+    # 5. envelope / layered-medium DAS of AntennaRing.compute_das_image (src/datasets/brain/antenna.py)
+    ant = load_module("ref_antenna", REF / "src" / "datasets" / "brain" / "antenna.py")
+    ge = YeeGrid(30, 30, 1e-3, 1e-3, device="cpu")
+    head = {"cx_cells": 15, "cy_cells": 14, "layers": [(12, 40.0), (11, 13.1), (9, 44.0), (8, 68.0), (6, 52.7),
+                                                       (4, 38.1)]}
+    for tag, n_t, geom, size in (("env_free", 300, None, 24), ("env_head", 301, head, 20)):
+        ring = ant.AntennaRing(6, 13, 0, ge, component="Hz")
+        ring.build_sources(S.GaussianPulse(1.0, sigma=5 * ge.dt), 0, n_t)
+        Se = rng.standard_normal((6, 6, n_t)).astype(np.float32)
+        out[f"{tag}_traces"] = Se
+        out[f"{tag}_positions"] = np.array([(i, j) for i, j, _ in ring.positions], dtype=np.int64)
+        out[f"{tag}_dt"] = np.array(ge.dt)
+        out[f"{tag}_image"] = ring.compute_das_image(Se, image_size=size, head_geometry=geom)
+    out["env_head_geometry"] = np.array([head["cx_cells"], head["cy_cells"]] +
+                                        [v for layer in head["layers"] for v in layer], dtype=np.float64)
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT} ({OUT.stat().st_size} bytes, {len(out)} arrays)")
 

@@ -82,6 +82,9 @@ int main(void) {
   O(wf_run2d_f64, ex) O(wf_run2d_f64, pstrip) O(wf_run2d_f64, traces) O(wf_run2d_f64, trace_stride)
   O(wf_run2d_f64, n_grp) O(wf_run2d_f64, step) O(wf_run2d_f64, dh) O(wf_run2d_f64, cmy)
   O(wf_f64_args, runs) O(wf_f64_args, n_runs) O(wf_f64_args, step_offset) O(wf_f64_args, pad)
+  printf("wf_das_env_args size %zu\n", sizeof(wf_das_env_args));
+  O(wf_das_env_args, traces) O(wf_das_env_args, layer_eps) O(wf_das_env_args, cx) O(wf_das_env_args, dt)
+  O(wf_das_env_args, n_ant) O(wf_das_env_args, stages)
   return 0;
 }
 """
@@ -100,7 +103,7 @@ def test_struct_layout_matches_header(tmp_path):
     types = {"wf_inj2d": _native.WfInj2d, "wf_run2d": _native.WfRun2d, "wf_tb_args": _native.WfTbArgs,
              "wf_das2d_args": _native.WfDas2dArgs, "wf_adj_args": _native.WfAdjArgs,
              "wf_inj2d_f64": _native.WfInj2dF64, "wf_run2d_f64": _native.WfRun2dF64,
-             "wf_f64_args": _native.WfF64Args}
+             "wf_f64_args": _native.WfF64Args, "wf_das_env_args": _native.WfDasEnvArgs}
     checked = 0
     for line in out:
         if not line.strip():

@@ -31,6 +31,7 @@ EXPORTED_SYMBOLS = (
     "wf_das2d",
     "wf_adjoint_step",
     "wf_fdtd2d_step_f64",
+    "wf_das_envelope",
 )
 
 _vp = ctypes.c_void_p
@@ -136,6 +137,15 @@ class WfF64Args(ctypes.Structure):
                                                        "pad")]
 
 
+class WfDasEnvArgs(ctypes.Structure):
+    """``wf_das_env_args``."""
+
+    _fields_ = [(n, _vp) for n in ("traces", "hilbert_k", "env", "tt", "image", "ant_x", "ant_y", "px", "py",
+                                   "layer_r", "layer_eps")] + \
+        [(n, _f64) for n in ("cx", "cy", "dt")] + \
+        [(n, _i32) for n in ("n_ant", "n_t", "img_w", "img_h", "n_layers", "stages")]
+
+
 _lock = threading.Lock()
 _lib = None
 
@@ -177,6 +187,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_adjoint_step.restype = ctypes.c_int
         lib.wf_fdtd2d_step_f64.argtypes = [ctypes.POINTER(WfF64Args), _vp]
         lib.wf_fdtd2d_step_f64.restype = ctypes.c_int
+        lib.wf_das_envelope.argtypes = [ctypes.POINTER(WfDasEnvArgs), _vp]
+        lib.wf_das_envelope.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]
         lib.wf_das2d.restype = ctypes.c_int
         ver = lib.wf_abi_version()
