@@ -36,10 +36,14 @@ extern "C" {
 #define WF_ERR_HIP (-2)
 #define WF_ERR_UNSUPPORTED (-3)
 
-/* Field component codes of an injection (src/core/sources.py:49). */
+/* Field component codes of an injection (src/core/sources.py:49). The 2D
+ * TM path uses the first three; the 3D path all six. */
 #define WF_COMP_EX 0
 #define WF_COMP_EY 1
 #define WF_COMP_HZ 2
+#define WF_COMP_EZ 3
+#define WF_COMP_HX 4
+#define WF_COMP_HY 5
 
 /* One soft-source contribution: field[cell] += wave[n] * amp.
  * Contributions that hit the same (comp, cell) form a group and are applied in
@@ -184,6 +188,44 @@ typedef struct wf_f64_args {
  * covering all runs (the per-step semantics of wf_fdtd2d_step). Replaces
  * FDTD2D.step() on a float64 grid (src/core/fdtd2d.py:187-266). */
 int wf_fdtd2d_step_f64(const wf_f64_args *args, hipStream_t stream);
+
+/* One 3D simulation (FDTD3D). Fields are (nx, ny, nz) fp32, row-major with
+ * k fastest: cell = (i*ny + j)*nz + k, the layout of a contiguous (Nx, Ny, Nz)
+ * torch tensor (src/core/fields.py:156-168). nz = 1 is the reference's
+ * 2D-equivalent mode (no z faces). Sources use wf_inj2d with any WF_COMP_*
+ * code and 3D cell indices; detectors record component det_comp[d] at
+ * det_cell[d] after every step. */
+typedef struct wf_run3d {
+    float *ex, *ey, *ez, *hx, *hy, *hz;
+    const float *ca, *cb;         /* per-cell coefficients; NULL => free space */
+    float *pstrip;                /* wf_fdtd3d_strip_floats(nx, ny, nz) floats: Mur snapshot planes */
+    const wf_inj2d *inj;
+    const int32_t *grp_start;
+    const int32_t *det_cell;
+    const int32_t *det_comp;
+    float *traces;                /* traces[d*trace_stride + (n - trace_n0)] */
+    int64_t trace_stride;
+    int32_t n_grp, n_inj;
+    int32_t n_det, trace_n0, trace_len;
+    int32_t nx, ny, nz;
+    float dh, de, dx, dy, dz;     /* fp32(dt/MU0), fp32(dt/EPS0), fp32 cell sizes */
+    float cmx, cmy, cmz;          /* fp32 Mur coefficients (boundaries.py:468-470) */
+    int32_t step;
+    int32_t flags;                /* reserved, 0 */
+} wf_run3d;
+
+/* Floats of the Mur snapshot workspace of one 3D run: the planes at index
+ * 0, 1, n-2, n-1 across each tangential axis of each H component,
+ * 8*(ny*nz + nx*nz) + (nz > 1 ? 8*nx*ny : 0). */
+int64_t wf_fdtd3d_strip_floats(int32_t nx, int32_t ny, int32_t nz);
+
+/* Advance `n_steps` reference-exact 3D steps in place, starting at
+ * run->step: MurABC3D.snapshot (boundaries.py:495-504), the sources, the
+ * Faraday slice updates (fdtd3d.py:309-330), MurABC3D.apply
+ * (boundaries.py:506-575), the Ampere slice updates (fdtd3d.py:336-359) and
+ * the detectors; replaces one call of FDTD3D.step() (fdtd3d.py:262-373) per
+ * step. `runs` is a HOST array of n_runs descriptors. */
+int wf_fdtd3d_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream);
 
 /* Stability judgement on the device, so the stepping stream never waits for
  * the host: for each run r, if any of the three maxima maxabs[3r..3r+2]

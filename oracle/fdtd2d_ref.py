@@ -93,13 +93,14 @@ class OracleSource:
     amp: Optional[torch.Tensor] = None     # (n_cells,)
     kind: str = "cells"
     position: int = 0
+    k_idx: Optional[torch.Tensor] = None   # 3D grids: z index of each cell (default 0)
 
     def inject(self, fields: dict, n: int) -> None:
         target = fields[self.component]
         if self.kind == "cells":
             w_now = self.wave[n:n + 1]
             values = w_now * self.amp
-            k_idx = torch.zeros_like(self.i_idx)
+            k_idx = torch.zeros_like(self.i_idx) if self.k_idx is None else self.k_idx
             target[..., self.i_idx, self.j_idx, k_idx] += values
         else:
             w_now = self.wave[n]

@@ -7,7 +7,8 @@ mathematical checks cannot pin at the bit level: the exact eager-op
 trajectory of ``FDTD2D.step`` (fields and detector traces), the ``Waveform.build``
 tables, and the DAS images of ``examples/brain_mimo_imaging.py`` and
 ``examples/09_brain_clot_dataset_sample.py``, and the envelope / layered-medium
-DAS of ``AntennaRing.compute_das_image``. Everything runs on the CPU in
+DAS of ``AntennaRing.compute_das_image``, and ``FDTD3D`` trajectories (lossy
+``MaterialMap3D`` media, free space, and the Nz = 1 mode). Everything runs on the CPU in
 fp32, as the reference's tests do. Output: ``tests/golden/reference_fixtures.npz``.
 """
 
@@ -134,6 +135,66 @@ def main() -> None:
         out[f"{tag}_positions"] = np.array([(i, j) for i, j, _ in ring.positions], dtype=np.int64)
         out[f"{tag}_dt"] = np.array(ge.dt)
         out[f"{tag}_image"] = ring.compute_das_image(Se, image_size=size, head_geometry=geom)
+    # 6. 3D trajectories (src/core/fdtd3d.py, MurABC3D, MaterialMap3D)
+    from core.boundaries import MurABC3D
+    from core.fdtd3d import FDTD3D
+    from core.materials import MaterialMap3D
+
+    g3 = YeeGrid(12, 10, 1e-3, 1e-3, Nz=8, dz=1e-3, device="cpu")
+    f3 = FieldSet(g3)
+    m3 = MaterialMap3D(g3, default=Material("coupling", 20.0, 0.1))
+    m3.add_sphere((6, 5, 4), 3.2, Material("brain", 45.0, 0.8))
+    m3.add_box((1, 1, 5), (4, 3, 7), Material("bone", 12.0, 0.2))
+    m3.add_ellipsoid((8, 6, 3), (2.5, 1.5, 2.0), Material("clot", 60.0, 1.2))
+    Ca3, Cb3 = m3.build3d()
+    src3 = [
+        S.PointSource(S.RickerWavelet(1.0, 30e9), 6, 5, "Ez", k=4, grid=g3, N_steps=60),
+        S.PointSource(S.GaussianPulse(0.7, sigma=6 * g3.dt), 0, 4, "Hx", k=3, grid=g3, N_steps=60),
+        S.PointSource(S.SinusoidalSource(0.3, 40e9), 5, 9, "Hy", k=2, grid=g3, N_steps=60),
+        S.PointSource(S.ModulatedGaussian(0.5, 30e9, sigma=2e-11), 3, 3, "Ex", k=7, grid=g3, N_steps=60),
+        S.PlaneSource(S.GaussianPulse(0.05, sigma=5 * g3.dt), "xy", 2, "Ey", grid=g3, N_steps=60),
+        S.PointSource(S.UWBPulse(1.0, 10e9, 40e9), 6, 5, "Ez", k=4, grid=g3, N_steps=60),
+        S.PointSource(S.GaussianPulse(0.4, sigma=4 * g3.dt), 7, 0, "Hz", k=5, grid=g3, N_steps=60),
+    ]
+    sim3 = FDTD3D(g3, f3, MurABC3D(g3, f3.Hx, f3.Hy, f3.Hz), S.SourceCollection(src3), Ca=Ca3, Cb=Cb3,
+                  n_check=10)
+    for _ in range(45):
+        sim3.step()
+    out["fd3_Ca"], out["fd3_Cb"], out["fd3_dt"] = Ca3.numpy(), Cb3.numpy(), np.array(g3.dt)
+    for k, s_ in enumerate(src3):
+        out[f"fd3_wave{k}"] = s_._waveform_tensor.numpy()
+    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        out[f"fd3_{c}"] = getattr(f3, c).numpy()
+    out["fd3_last_field_max"] = np.array(sim3.last_field_max)
+    # free space, 30 steps
+    g4 = YeeGrid(10, 9, 1e-3, 1e-3, Nz=7, dz=1e-3, device="cpu")
+    f4 = FieldSet(g4)
+    src4 = [S.PointSource(S.GaussianPulse(1.0, sigma=5 * g4.dt), 5, 4, "Ez", k=3, grid=g4, N_steps=40),
+            S.PointSource(S.RickerWavelet(0.5, 40e9), 2, 6, "Hz", k=1, grid=g4, N_steps=40)]
+    sim4 = FDTD3D(g4, f4, MurABC3D(g4, f4.Hx, f4.Hy, f4.Hz), S.SourceCollection(src4), n_check=10)
+    for _ in range(30):
+        sim4.step()
+    out["fs3_dt"] = np.array(g4.dt)
+    for k, s_ in enumerate(src4):
+        out[f"fs3_wave{k}"] = s_._waveform_tensor.numpy()
+    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        out[f"fs3_{c}"] = getattr(f4, c).numpy()
+    out["fs3_last_field_max"] = np.array(sim4.last_field_max)
+    # Nz = 1 mode
+    g5 = YeeGrid(16, 14, 1e-3, 1e-3, Nz=1, device="cpu")
+    f5 = FieldSet(g5)
+    src5 = [S.PointSource(S.GaussianPulse(1.0, sigma=5 * g5.dt), 8, 7, "Hz", grid=g5, N_steps=40),
+            S.PointSource(S.GaussianPulse(0.5, sigma=4 * g5.dt), 5, 9, "Ez", grid=g5, N_steps=40),
+            S.PointSource(S.GaussianPulse(0.5, sigma=4 * g5.dt), 4, 3, "Hx", grid=g5, N_steps=40)]
+    sim5 = FDTD3D(g5, f5, MurABC3D(g5, f5.Hx, f5.Hy, f5.Hz), S.SourceCollection(src5))
+    for _ in range(30):
+        sim5.step()
+    out["nz1_dt"] = np.array(g5.dt)
+    for k, s_ in enumerate(src5):
+        out[f"nz1_wave{k}"] = s_._waveform_tensor.numpy()
+    for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+        out[f"nz1_{c}"] = getattr(f5, c).numpy()
+
     out["env_head_geometry"] = np.array([head["cx_cells"], head["cy_cells"]] +
                                         [v for layer in head["layers"] for v in layer], dtype=np.float64)
     np.savez_compressed(OUT, **out)

@@ -16,7 +16,7 @@ HEADER = ROOT / "include" / "waveforge_hip.h"
 
 def declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(wf_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char \*)\s*(wf_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_header_declarations_match_binding():
@@ -85,6 +85,10 @@ int main(void) {
   printf("wf_das_env_args size %zu\n", sizeof(wf_das_env_args));
   O(wf_das_env_args, traces) O(wf_das_env_args, layer_eps) O(wf_das_env_args, cx) O(wf_das_env_args, dt)
   O(wf_das_env_args, n_ant) O(wf_das_env_args, stages)
+  printf("wf_run3d size %zu\n", sizeof(wf_run3d));
+  O(wf_run3d, ex) O(wf_run3d, hz) O(wf_run3d, pstrip) O(wf_run3d, det_comp) O(wf_run3d, traces)
+  O(wf_run3d, trace_stride) O(wf_run3d, n_grp) O(wf_run3d, nz) O(wf_run3d, dh) O(wf_run3d, cmz)
+  O(wf_run3d, step) O(wf_run3d, flags)
   return 0;
 }
 """
@@ -103,7 +107,8 @@ def test_struct_layout_matches_header(tmp_path):
     types = {"wf_inj2d": _native.WfInj2d, "wf_run2d": _native.WfRun2d, "wf_tb_args": _native.WfTbArgs,
              "wf_das2d_args": _native.WfDas2dArgs, "wf_adj_args": _native.WfAdjArgs,
              "wf_inj2d_f64": _native.WfInj2dF64, "wf_run2d_f64": _native.WfRun2dF64,
-             "wf_f64_args": _native.WfF64Args, "wf_das_env_args": _native.WfDasEnvArgs}
+             "wf_f64_args": _native.WfF64Args, "wf_das_env_args": _native.WfDasEnvArgs,
+             "wf_run3d": _native.WfRun3d}
     checked = 0
     for line in out:
         if not line.strip():

@@ -15,6 +15,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libwaveforge_hip.so"
 ABI_VERSION = 1
 
 WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
+WF_COMP3 = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
 TB_MAX_DEPTH = 8
 
 # Every symbol the header declares; tests check the built library exports them.
@@ -32,6 +33,8 @@ EXPORTED_SYMBOLS = (
     "wf_adjoint_step",
     "wf_fdtd2d_step_f64",
     "wf_das_envelope",
+    "wf_fdtd3d_step",
+    "wf_fdtd3d_strip_floats",
 )
 
 _vp = ctypes.c_void_p
@@ -146,6 +149,17 @@ class WfDasEnvArgs(ctypes.Structure):
         [(n, _i32) for n in ("n_ant", "n_t", "img_w", "img_h", "n_layers", "stages")]
 
 
+class WfRun3d(ctypes.Structure):
+    """``wf_run3d``."""
+
+    _fields_ = [(n, _vp) for n in ("ex", "ey", "ez", "hx", "hy", "hz", "ca", "cb", "pstrip", "inj", "grp_start",
+                                   "det_cell", "det_comp", "traces")] + \
+        [("trace_stride", _i64)] + \
+        [(n, _i32) for n in ("n_grp", "n_inj", "n_det", "trace_n0", "trace_len", "nx", "ny", "nz")] + \
+        [(n, _f32) for n in ("dh", "de", "dx", "dy", "dz", "cmx", "cmy", "cmz")] + \
+        [("step", _i32), ("flags", _i32)]
+
+
 _lock = threading.Lock()
 _lib = None
 
@@ -187,6 +201,10 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_adjoint_step.restype = ctypes.c_int
         lib.wf_fdtd2d_step_f64.argtypes = [ctypes.POINTER(WfF64Args), _vp]
         lib.wf_fdtd2d_step_f64.restype = ctypes.c_int
+        lib.wf_fdtd3d_step.argtypes = [ctypes.POINTER(WfRun3d), _i32, _i32, _vp]
+        lib.wf_fdtd3d_step.restype = ctypes.c_int
+        lib.wf_fdtd3d_strip_floats.argtypes = [_i32, _i32, _i32]
+        lib.wf_fdtd3d_strip_floats.restype = ctypes.c_int64
         lib.wf_das_envelope.argtypes = [ctypes.POINTER(WfDasEnvArgs), _vp]
         lib.wf_das_envelope.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]

@@ -1,4 +1,4 @@
-"""Hz point detectors recorded inside the stepping kernels.
+"""Point detectors recorded inside the stepping kernels.
 
 The reference records receivers in its callers' step loops:
 ``sig_buf[:, step] = fields.Hz[rx_i, rx_j, 0]`` after each ``sim.step()``
@@ -7,6 +7,9 @@ The reference records receivers in its callers' step loops:
 cells and a device trace buffer that the HIP kernels fill as they step:
 ``traces[..., d, n - start_step]`` is Hz at ``cells[d]`` after step ``n``
 (0-based), i.e. the value the reference loop stores in column ``n``.
+On 3D grids cells are ``(i, j, k)`` and ``component`` may be any of the six
+(the dataset generator records Ez, ``src/datasets/brain/antenna.py:173-198``);
+the 2D TM stepper records Hz only.
 """
 
 from __future__ import annotations
@@ -14,14 +17,29 @@ from __future__ import annotations
 import torch
 
 
-class DetectorSet:
-    """Hz detectors at integer cells ``(i, j)`` recording ``n_steps`` samples."""
+COMPONENT_CODES = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
 
-    def __init__(self, cells, n_steps: int, *, grid, batch_size: int = 1, start_step: int = 0) -> None:
-        cells = [(int(i), int(j)) for i, j in cells]
-        for i, j in cells:
-            if not (0 <= i < grid.Nx and 0 <= j < grid.Ny):
-                raise ValueError(f"detector cell ({i}, {j}) outside grid ({grid.Nx}, {grid.Ny})")
+
+class DetectorSet:
+    """Detectors of one field component at integer cells ``(i, j)`` or
+    ``(i, j, k)``, recording ``n_steps`` samples."""
+
+    def __init__(self, cells, n_steps: int, *, grid, batch_size: int = 1, start_step: int = 0,
+                 component: str = "Hz") -> None:
+        if component not in COMPONENT_CODES:
+            raise ValueError(f"component must be one of {tuple(COMPONENT_CODES)}, got {component!r}")
+        cells = [tuple(int(v) for v in c) for c in cells]
+        Nz = getattr(grid, "Nz", 1)
+        for c in cells:
+            if len(c) == 2:
+                c3 = (c[0], c[1], 0)
+            elif len(c) == 3:
+                c3 = c
+            else:
+                raise ValueError(f"detector cell {c} must be (i, j) or (i, j, k)")
+            i, j, k = c3
+            if not (0 <= i < grid.Nx and 0 <= j < grid.Ny and 0 <= k < Nz):
+                raise ValueError(f"detector cell {c} outside grid ({grid.Nx}, {grid.Ny}, {Nz})")
         if n_steps < 1:
             raise ValueError(f"n_steps must be >= 1, got {n_steps!r}")
         self.cells = cells
@@ -30,8 +48,12 @@ class DetectorSet:
         self.batch_size = int(batch_size)
         shape = (len(cells), self.n_steps) if batch_size == 1 else (batch_size, len(cells), self.n_steps)
         self.traces = torch.zeros(shape, dtype=torch.float32, device=grid.device)
-        flat = [i * grid.Ny + j for i, j in cells]
+        c3 = [c if len(c) == 3 else (c[0], c[1], 0) for c in cells]
+        flat = [(i * grid.Ny + j) * Nz + k for i, j, k in c3]
         self.cell_index = torch.tensor(flat, dtype=torch.int32, device=grid.device)
+        self.component = component
+        self.component_index = torch.full((len(cells),), COMPONENT_CODES[component], dtype=torch.int32,
+                                          device=grid.device)
 
     def __len__(self) -> int:
         return len(self.cells)

@@ -87,6 +87,8 @@ class FDTD2D:
             )
         if (Ca is None) != (Cb is None):
             raise ValueError("Ca and Cb must be provided together or both omitted.")
+        if detectors is not None and getattr(detectors, "component", "Hz") != "Hz":
+            raise ValueError("the 2D TM stepper records Hz detectors only")
         self._grid = grid
         self._fields = fields
         self._boundary = boundary

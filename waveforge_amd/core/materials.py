@@ -150,6 +150,76 @@ class MaterialMap:
         )
 
 
+class MaterialMap3D(MaterialMap):
+    """``MaterialMap`` with 3D primitives and ``build3d`` (``src/core/materials.py:281-477``):
+    spheres, cylinders, boxes, ellipsoids and custom masks over fp32 index
+    grids ``(Nx, Ny, Nz)``, painted in order over the default material."""
+
+    def __init__(self, grid, default: Material | None = None) -> None:
+        super().__init__(grid, default)
+        self._regions3d: list = []
+        self._materials3d: list[Material] = []
+
+    def add_sphere(self, center, radius: float, material: Material) -> "MaterialMap3D":
+        ci, cj, ck = float(center[0]), float(center[1]), float(center[2])
+        r2 = radius * radius
+        self._regions3d.append(lambda I, J, K: (I - ci) ** 2 + (J - cj) ** 2 + (K - ck) ** 2 <= r2)
+        self._materials3d.append(material)
+        return self
+
+    def add_cylinder(self, center, axis: str, radius: float, height: float, material: Material) -> "MaterialMap3D":
+        ci, cj, ck = float(center[0]), float(center[1]), float(center[2])
+        r2 = radius * radius
+        h2 = height / 2.0
+        if axis == "x":
+            def mask(I, J, K):
+                return ((J - cj) ** 2 + (K - ck) ** 2 <= r2) & ((I - ci).abs() <= h2)
+        elif axis == "y":
+            def mask(I, J, K):
+                return ((I - ci) ** 2 + (K - ck) ** 2 <= r2) & ((J - cj).abs() <= h2)
+        else:
+            def mask(I, J, K):
+                return ((I - ci) ** 2 + (J - cj) ** 2 <= r2) & ((K - ck).abs() <= h2)
+        self._regions3d.append(mask)
+        self._materials3d.append(material)
+        return self
+
+    def add_box(self, corner_min, corner_max, material: Material) -> "MaterialMap3D":
+        i0, j0, k0 = (float(v) for v in corner_min)
+        i1, j1, k1 = (float(v) for v in corner_max)
+        self._regions3d.append(lambda I, J, K: (I >= i0) & (I <= i1) & (J >= j0) & (J <= j1) & (K >= k0) & (K <= k1))
+        self._materials3d.append(material)
+        return self
+
+    def add_ellipsoid(self, center, semi_axes, material: Material) -> "MaterialMap3D":
+        ci, cj, ck = float(center[0]), float(center[1]), float(center[2])
+        a2, b2, c2 = (float(v) ** 2 for v in semi_axes)
+        self._regions3d.append(lambda I, J, K: (I - ci) ** 2 / a2 + (J - cj) ** 2 / b2 + (K - ck) ** 2 / c2 <= 1.0)
+        self._materials3d.append(material)
+        return self
+
+    def add_custom3d(self, mask_fn, material: Material) -> "MaterialMap3D":
+        self._regions3d.append(mask_fn)
+        self._materials3d.append(material)
+        return self
+
+    def build3d(self) -> tuple[torch.Tensor, torch.Tensor]:
+        Nx, Ny, Nz = self._grid.Nx, self._grid.Ny, self._grid.Nz
+        dt = self._grid.dt
+        ca = torch.full((Nx, Ny, Nz), self._default.ca(dt), dtype=torch.float32)
+        cb = torch.full((Nx, Ny, Nz), self._default.cb(dt), dtype=torch.float32)
+        I = torch.arange(Nx, dtype=torch.float32).view(-1, 1, 1).expand(Nx, Ny, Nz)
+        J = torch.arange(Ny, dtype=torch.float32).view(1, -1, 1).expand(Nx, Ny, Nz)
+        K = torch.arange(Nz, dtype=torch.float32).view(1, 1, -1).expand(Nx, Ny, Nz)
+        for mask_fn, mat in zip(self._regions3d, self._materials3d):
+            m = mask_fn(I, J, K)
+            ca[m] = mat.ca(dt)
+            cb[m] = mat.cb(dt)
+        _validate_ca(float(ca.min().item()))
+        dev, dtype = self._grid.device, self._grid.dtype
+        return ca.to(device=dev, dtype=dtype), cb.to(device=dev, dtype=dtype)
+
+
 def coefficients_from_maps(eps_r, sigma, dt: float, device=None, validate: bool = True):
     """Per-cell ``(Ca, Cb)`` from eps_r / sigma maps ``(Nx, Ny)``: the
     ``Material.ca/cb`` fp64 formula elementwise, stored fp32 as ``(Nx, Ny, 1)``."""

@@ -352,6 +352,25 @@ class PlaneSource:
             j = torch.arange(Ny)
         return i, j, torch.ones(i.numel(), dtype=torch.float32)
 
+    def cells3d(self, Nx: int, Ny: int, Nz: int):
+        """Cells ``(i, j, k)`` and amplitudes (1.0) this plane touches on an
+        ``(Nx, Ny, Nz)`` grid (``xy``: all (i, j) at k = position; ``xz``: all
+        (i, k) at j; ``yz``: all (j, k) at i)."""
+        p = self._position
+        if self._plane == "xy":
+            I, J = torch.meshgrid(torch.arange(Nx), torch.arange(Ny), indexing="ij")
+            i, j = I.reshape(-1), J.reshape(-1)
+            k = torch.full_like(i, p)
+        elif self._plane == "xz":
+            I, K = torch.meshgrid(torch.arange(Nx), torch.arange(Nz), indexing="ij")
+            i, k = I.reshape(-1), K.reshape(-1)
+            j = torch.full_like(i, p)
+        else:
+            J, K = torch.meshgrid(torch.arange(Ny), torch.arange(Nz), indexing="ij")
+            j, k = J.reshape(-1), K.reshape(-1)
+            i = torch.full_like(j, p)
+        return i, j, k, torch.ones(i.numel(), dtype=torch.float32)
+
     n_cells = property(lambda self: self._n_cells)
     waveform_memory_bytes = property(lambda self: self._N_steps * self._waveform_tensor.element_size())
     is_precomputed = property(lambda self: True)

@@ -102,9 +102,13 @@ class SourceTable:
     (inert in 2D TM) are kept aside in ``inert`` for the host to apply.
     """
 
-    def __init__(self, sources: Sequence, Nx: int, Ny: int, device: torch.device) -> None:
+    def __init__(self, sources: Sequence, Nx: int, Ny: int, device: torch.device, Nz: int = 1,
+                 comps_map: Optional[dict] = None) -> None:
+        """``comps_map`` maps the components the kernels update to their codes
+        (2D TM: Ex, Ey, Hz; 3D: all six); cells are ``(i*Ny + j)*Nz + k``."""
         from .core.sources import PlaneSource
 
+        cmap = _native.WF_COMP if comps_map is None else comps_map
         comps, cells, amps, waves, lens = [], [], [], [], []
         self.inert = []
         self.keepalive = []
@@ -112,7 +116,7 @@ class SourceTable:
         for order, src in enumerate(sources):
             comp = src.component
             wave = src._waveform_tensor
-            if comp not in _native.WF_COMP:
+            if comp not in cmap:
                 self.inert.append(src)
                 self.min_wave_len = len(wave) if self.min_wave_len is None else min(self.min_wave_len, len(wave))
                 continue
@@ -120,12 +124,15 @@ class SourceTable:
             wave = wave.contiguous()
             self.keepalive.append(wave)
             if isinstance(src, PlaneSource):
-                i, j, a = src.cells(Nx, Ny)
+                i, j, k, a = src.cells3d(Nx, Ny, Nz)
             else:
                 i, j, a = src._i_idx.cpu(), src._j_idx.cpu(), src._amp.detach().float().cpu()
+                kk = getattr(src, "_k_idx", None)
+                k = torch.zeros_like(i) if kk is None else kk.cpu()
             n = i.numel()
-            comps.append(np.full(n, _native.WF_COMP[comp], dtype=np.int32))
-            cells.append((i.numpy().astype(np.int64) * Ny + j.numpy().astype(np.int64)).astype(np.int32))
+            comps.append(np.full(n, cmap[comp], dtype=np.int32))
+            flat = (i.numpy().astype(np.int64) * Ny + j.numpy().astype(np.int64)) * Nz + k.numpy().astype(np.int64)
+            cells.append(flat.astype(np.int32))
             amps.append(a.numpy().astype(np.float32))
             waves.append(np.full(n, wave.data_ptr(), dtype=np.uint64))
             lens.append(np.full(n, wave.numel(), dtype=np.int32))
@@ -133,7 +140,7 @@ class SourceTable:
         if comps:
             comp = np.concatenate(comps)
             cell = np.concatenate(cells)
-            key = comp.astype(np.int64) * (int(Nx) * int(Ny)) + cell
+            key = comp.astype(np.int64) * (int(Nx) * int(Ny) * int(Nz)) + cell
             perm = np.argsort(key, kind="stable")
             rec = np.zeros(perm.size, dtype=INJ_DTYPE)
             rec["wave"] = np.concatenate(waves)[perm]
