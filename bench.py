@@ -58,6 +58,8 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="skip the serving section")
     p.add_argument("--serving-requests", type=int, default=48)
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
+    p.add_argument("--no-3d", action="store_true", help="skip the 3D section")
+    p.add_argument("--size3d", type=int, default=256)
     p.add_argument("--fd-cells", type=int, default=32, help="seeded cells of the central-difference check")
     p.add_argument("--copy-probe", action="store_true",
                    help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
@@ -239,6 +241,49 @@ def serving_suite(dev, n_requests: int) -> dict:
     }
 
 
+def fdtd3d_suite(dev, n: int, steps: int = 200, warmup: int = 20) -> dict:
+    """3D Yee stepper (FDTD3D.run) on an n^3 lossy medium: a coupling box with a
+    layered sphere (MaterialMap3D), an Ez point source at the centre, Mur on
+    all six faces, stability check every 100 steps. Algorithmic traffic 56 B
+    per cell-update (read six fields, Ca, Cb; write six fields)."""
+    import torch
+
+    from waveforge_amd import core
+
+    g = core.YeeGrid(n, n, 1e-3, 1e-3, Nz=n, dz=1e-3, device=dev)
+    m = core.MaterialMap3D(g, default=core.Material("coupling", 20.0, 0.1))
+    c = n // 2
+    m.add_sphere((c, c, c), 0.40 * n, core.Material("skull", 12.0, 0.2))
+    m.add_sphere((c, c, c), 0.36 * n, core.Material("brain", 45.0, 0.8))
+    m.add_sphere((c + n // 10, c, c - n // 12), 0.05 * n, core.Material("clot", 60.0, 1.2))
+    Ca, Cb = m.build3d()
+    f = core.FieldSet(g)
+    src = core.PointSource(core.ModulatedGaussian(fc=1e9, sigma=1.0 / (2.0 * 3.141592653589793 * 0.5e9)), c, c,
+                           "Ez", k=c, grid=g, N_steps=steps + warmup)
+    out = {"workload": f"{n}^3 lossy layered sphere (MaterialMap3D), Ez source, Mur on six faces, {steps} steps"}
+    finals = []
+    for tag, fused in (("per_stage", False), ("fused", True)):
+        f = core.FieldSet(g)
+        sim = core.FDTD3D(g, f, core.MurABC3D(g, f.Hx, f.Hy, f.Hz), core.SourceCollection([src]), Ca=Ca, Cb=Cb,
+                          fused=fused)
+        sim.run(warmup)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        sim.run(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        rate = n ** 3 * steps / (ms / 1e3)
+        out[tag] = {"ms_per_step": round(ms / steps, 4), "mcell_updates_per_s": round(rate / 1e6, 1),
+                    "algorithmic_gb_per_s": round(rate * 56 / 1e9, 1), "roofline_frac": round(rate * 56 / 8e12, 4)}
+        finals.append(f.Ez.clone())
+        out["fields_finite"] = bool(all(torch.isfinite(getattr(f, k)).all() for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")))
+    out["paths_equal"] = bool(torch.equal(finals[0], finals[1]))
+    out["default_path"] = "per_stage"
+    return out
+
+
 def adjoint_suite(dev, n_fd: int) -> dict:
     """Trace-misfit gradient on the 256^2 head phantom, 16 TX x 16 RX, 3000
     steps, checkpoints every 100 steps: d_obs from the clot phantom, the model
@@ -418,7 +463,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
         base = cpu_baseline(wl, args.cpu_steps)
-    imaging = serving_res = adjoint = None
+    imaging = serving_res = adjoint = fd3 = None
     if not args.no_imaging:
         print("[bench] batched imaging section", file=sys.stderr, flush=True)
         imaging = imaging_suite(dev)
@@ -428,6 +473,9 @@ def main() -> int:
     if not args.no_adjoint:
         print("[bench] adjoint section", file=sys.stderr, flush=True)
         adjoint = adjoint_suite(dev, args.fd_cells)
+    if not args.no_3d:
+        print("[bench] 3D section", file=sys.stderr, flush=True)
+        fd3 = fdtd3d_suite(dev, args.size3d)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -457,6 +505,7 @@ def main() -> int:
         "imaging": imaging,
         "serving": serving_res,
         "adjoint": adjoint,
+        "fdtd3d": fd3,
     }
     print(json.dumps(out))
     return 0

@@ -197,6 +197,7 @@ int wf_fdtd2d_step_f64(const wf_f64_args *args, hipStream_t stream);
  * det_cell[d] after every step. */
 typedef struct wf_run3d {
     float *ex, *ey, *ez, *hx, *hy, *hz;
+    float *ex2, *ey2, *ez2, *hx2, *hy2, *hz2;  /* second state (fused path); unused by wf_fdtd3d_step */
     const float *ca, *cb;         /* per-cell coefficients; NULL => free space */
     float *pstrip;                /* wf_fdtd3d_strip_floats(nx, ny, nz) floats: Mur snapshot planes */
     const wf_inj2d *inj;
@@ -226,6 +227,16 @@ int64_t wf_fdtd3d_strip_floats(int32_t nx, int32_t ny, int32_t nz);
  * the detectors; replaces one call of FDTD3D.step() (fdtd3d.py:262-373) per
  * step. `runs` is a HOST array of n_runs descriptors. */
 int wf_fdtd3d_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream);
+
+/* The same steps with one fused pass per step: step s reads state A (ex..hz)
+ * and writes state B (ex2..hz2) when s is even, the reverse when odd, so
+ * `n_steps` must be even and the result lands in state A. Per step: Mur
+ * snapshot planes and sources on the read state, one launch that streams
+ * each (j, k) tile along i computing Faraday everywhere and Ampere on the
+ * cells whose stencil touches no Mur face, then the Mur faces and the
+ * remaining E cells, then the detectors. Results equal wf_fdtd3d_step's bit
+ * for bit. */
+int wf_fdtd3d_fused_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream);
 
 /* Stability judgement on the device, so the stepping stream never waits for
  * the host: for each run r, if any of the three maxima maxabs[3r..3r+2]

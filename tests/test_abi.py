@@ -86,7 +86,7 @@ int main(void) {
   O(wf_das_env_args, traces) O(wf_das_env_args, layer_eps) O(wf_das_env_args, cx) O(wf_das_env_args, dt)
   O(wf_das_env_args, n_ant) O(wf_das_env_args, stages)
   printf("wf_run3d size %zu\n", sizeof(wf_run3d));
-  O(wf_run3d, ex) O(wf_run3d, hz) O(wf_run3d, pstrip) O(wf_run3d, det_comp) O(wf_run3d, traces)
+  O(wf_run3d, ex) O(wf_run3d, hz) O(wf_run3d, ex2) O(wf_run3d, hz2) O(wf_run3d, ca) O(wf_run3d, pstrip) O(wf_run3d, det_comp) O(wf_run3d, traces)
   O(wf_run3d, trace_stride) O(wf_run3d, n_grp) O(wf_run3d, nz) O(wf_run3d, dh) O(wf_run3d, cmz)
   O(wf_run3d, step) O(wf_run3d, flags)
   return 0;

@@ -280,3 +280,23 @@ def test_divergence_raises_at_check_step(device):
     with pytest.raises(core.fdtd3d.SimulationDivergedError, match=r"diverged at step 100: "):
         sim.run(300)
     assert sim.steps_completed == 100
+
+
+@pytest.mark.parametrize("shape,material", [((70, 45, 130), True), ((33, 64, 63), False), ((66, 30, 1), True)])
+def test_fused_equals_per_stage(device, shape, material):
+    """Several i-chunks (32 planes), j-tiles (14 rows) and k-tiles (62 lanes)."""
+    from waveforge_amd import core
+
+    nx, ny, nz = shape
+    runs = []
+    for fused in (True, False):
+        g, f, srcs, det_sets, kw, _ = _case(core, device, nx, ny, nz, 77, material, steps=12)
+        sim = core.FDTD3D(g, f, core.MurABC3D(g, f.Hx, f.Hy, f.Hz), core.SourceCollection(srcs),
+                          detectors=det_sets[0], fused=fused, **kw)
+        sim.run(12)
+        torch.cuda.synchronize()
+        runs.append(({c: getattr(f, c).clone() for c in COMPS}, det_sets[0].traces.clone()))
+    print(f"[fdtd3d] fused vs per-stage {shape}", flush=True)
+    for c in COMPS:
+        assert torch.equal(runs[0][0][c], runs[1][0][c]), c
+    assert torch.equal(runs[0][1], runs[1][1])

@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "wf_das_envelope",
     "wf_fdtd3d_step",
     "wf_fdtd3d_strip_floats",
+    "wf_fdtd3d_fused_step",
 )
 
 _vp = ctypes.c_void_p
@@ -152,8 +153,8 @@ class WfDasEnvArgs(ctypes.Structure):
 class WfRun3d(ctypes.Structure):
     """``wf_run3d``."""
 
-    _fields_ = [(n, _vp) for n in ("ex", "ey", "ez", "hx", "hy", "hz", "ca", "cb", "pstrip", "inj", "grp_start",
-                                   "det_cell", "det_comp", "traces")] + \
+    _fields_ = [(n, _vp) for n in ("ex", "ey", "ez", "hx", "hy", "hz", "ex2", "ey2", "ez2", "hx2", "hy2", "hz2",
+                                   "ca", "cb", "pstrip", "inj", "grp_start", "det_cell", "det_comp", "traces")] + \
         [("trace_stride", _i64)] + \
         [(n, _i32) for n in ("n_grp", "n_inj", "n_det", "trace_n0", "trace_len", "nx", "ny", "nz")] + \
         [(n, _f32) for n in ("dh", "de", "dx", "dy", "dz", "cmx", "cmy", "cmz")] + \
@@ -203,6 +204,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_fdtd2d_step_f64.restype = ctypes.c_int
         lib.wf_fdtd3d_step.argtypes = [ctypes.POINTER(WfRun3d), _i32, _i32, _vp]
         lib.wf_fdtd3d_step.restype = ctypes.c_int
+        lib.wf_fdtd3d_fused_step.argtypes = [ctypes.POINTER(WfRun3d), _i32, _i32, _vp]
+        lib.wf_fdtd3d_fused_step.restype = ctypes.c_int
         lib.wf_fdtd3d_strip_floats.argtypes = [_i32, _i32, _i32]
         lib.wf_fdtd3d_strip_floats.restype = ctypes.c_int64
         lib.wf_das_envelope.argtypes = [ctypes.POINTER(WfDasEnvArgs), _vp]

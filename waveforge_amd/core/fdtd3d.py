@@ -6,7 +6,14 @@ properties as ``src/core/fdtd3d.py`` (``FDTD3D`` :118-580,
 short sequence of HIP launches (``wf_fdtd3d_step``: Mur snapshot planes,
 sources, Faraday, Mur faces in the reference's order, Ampere, detectors);
 ``run(n)`` advances in chunks that end on the ``n_check`` steps, where the
-six-component stability check runs exactly as the reference's.
+six-component stability check runs exactly as the reference's. With
+``fused=True`` the steps of a chunk take the fused path
+(``wf_fdtd3d_fused_step``: one streaming pass per step over a second state
+buffer, same values bit for bit), with one in-place step first when the
+chunk is odd, so results always land in the ``FieldSet`` tensors; it moves
+as many HBM bytes as the per-stage pair on this hardware (its tiles' halo
+rows and unaligned row ends are fetched again by neighbouring tiles), so the
+per-stage path is the default.
 
 The ``FieldSet`` tensors are the live state (read and written in place);
 ``sim._sources`` is read at every call. ``compile_step`` exists for API
@@ -48,7 +55,7 @@ class FDTD3D:
     def __init__(self, grid: YeeGrid, fields: FieldSet, boundary: object, sources: Optional[object] = None, *,
                  Ca: Optional[torch.Tensor] = None, Cb: Optional[torch.Tensor] = None,
                  stability_threshold: float = 1e10, n_check: int = 100,
-                 detectors: Optional[DetectorSet] = None) -> None:
+                 detectors: Optional[DetectorSet] = None, fused: bool = False) -> None:
         if grid.Nx < 4:
             raise ValueError(f"grid.Nx must be >= 4 for a valid 3D Yee stencil, got {grid.Nx}")
         if grid.Ny < 4:
@@ -76,6 +83,8 @@ class FDTD3D:
         self._threshold = stability_threshold
         self._n_check = n_check
         self._detectors = detectors
+        self._fused = fused
+        self._second = None
         self.steps_completed: int = 0
         self.elapsed_time: float = 0.0
         self.mcells_per_second: float = 0.0
@@ -129,11 +138,18 @@ class FDTD3D:
             self._coef_keep = (ca, cb)
         lib = _native.load()
         n_strip = int(lib.wf_fdtd3d_strip_floats(g.Nx, g.Ny, g.Nz))
+        second = None
+        if self._fused:
+            if self._second is None or self._second[0].shape != f.Hz.shape:
+                self._second = tuple(torch.empty_like(f.Hz) for _ in range(6))
+            second = [self._members(t) for t in self._second]
         b_ = self._boundary
         self._strips, structs = [], []
         for b in range(B):
             r = _native.WfRun3d()
             r.ex, r.ey, r.ez, r.hx, r.hy, r.hz = (m[b].data_ptr() for m in members)
+            if second is not None:
+                r.ex2, r.ey2, r.ez2, r.hx2, r.hy2, r.hz2 = (m[b].data_ptr() for m in second)
             if ca_m[b] is not None:
                 r.ca, r.cb = ca_m[b].data_ptr(), cb_m[b].data_ptr()
             strip = torch.empty(n_strip, dtype=torch.float32, device=dev)
@@ -172,12 +188,23 @@ class FDTD3D:
                 f"source waveform table has {limit} samples; cannot inject at step "
                 f"{max(self.steps_completed, limit)} (build sources with a larger N_steps)"
             )
-        for r in structs:
-            r.step = self.steps_completed
-        arr = (_native.WfRun3d * len(structs))(*structs)
         lib = _native.load()
-        _native.check(lib.wf_fdtd3d_step(arr, len(structs), int(n_steps), ctypes.c_void_p(self._stream())),
-                      "wf_fdtd3d_step")
+        stream = ctypes.c_void_p(self._stream())
+        first = self.steps_completed
+        if self._fused and n_steps >= 2 and n_steps % 2:
+            for r in structs:
+                r.step = first
+            arr = (_native.WfRun3d * len(structs))(*structs)
+            _native.check(lib.wf_fdtd3d_step(arr, len(structs), 1, stream), "wf_fdtd3d_step")
+            first += 1
+            n_steps -= 1
+        for r in structs:
+            r.step = first
+        arr = (_native.WfRun3d * len(structs))(*structs)
+        if self._fused and n_steps >= 2:
+            _native.check(lib.wf_fdtd3d_fused_step(arr, len(structs), int(n_steps), stream), "wf_fdtd3d_fused_step")
+        else:
+            _native.check(lib.wf_fdtd3d_step(arr, len(structs), int(n_steps), stream), "wf_fdtd3d_step")
 
     def step(self) -> None:
         """Advance all six components by one time step in place (``fdtd3d.py:262-373``)."""

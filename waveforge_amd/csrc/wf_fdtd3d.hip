@@ -243,6 +243,232 @@ __global__ void __launch_bounds__(256) k3_rec(wf_run3d r, int n) {
         r.traces[d * r.trace_stride + t] = comp_ptr(r, r.det_comp[d])[r.det_cell[d]];
 }
 
+// ---------------------------------------------------------------------------
+// Fused path: one streaming pass per step over ping-pong buffers.
+//
+// A workgroup of 16 x 64 threads owns a (j, k) tile: rows j0-1 .. j0+14 and
+// lanes k0-1 .. k0+62, i.e. a core of 14 x 62 cells with a one-cell halo, and
+// walks i over a chunk of planes (starting one plane early so the
+// previous plane's new H is in registers). For each plane it stages the old E
+// of the plane in LDS, computes Faraday (new H, "F") on every cell but the
+// high-side halo from the LDS neighbours j+1, k+1 and the next plane's E in
+// registers, stages F in LDS, and writes the core's F and, on the cells whose
+// Ampere stencil touches no Mur face cell ([2, n-2] on every axis), the new
+// E from the LDS neighbours j-1, k-1 and the previous plane's F. The Mur faces
+// (k3_mur1/k3_mur2 on the written state) and the E cells outside that box
+// (k3_eshell) follow. Reads come only from the old state, so tiles and chunks
+// never race.
+// ---------------------------------------------------------------------------
+constexpr int kFLanes = 64;
+constexpr int kFRows = 16;
+constexpr int kFCoreK = kFLanes - 2;
+constexpr int kFCoreJ = kFRows - 2;
+constexpr int kResidentBlocks = 512;  // 256 CUs x 2 workgroups of 1024 threads
+
+struct Bufs {
+    const float *ex, *ey, *ez, *hx, *hy, *hz;  // state read (old)
+    float *nex, *ney, *nez, *nhx, *nhy, *nhz;  // state written (new)
+};
+
+template <bool kMat>
+__global__ void __launch_bounds__(kFLanes * kFRows) __attribute__((amdgpu_waves_per_eu(8, 8)))
+k3_fused(wf_run3d r, Bufs b, int chunk, int ntk, int ntj, int n_tiles) {
+    const int nx = r.nx, ny = r.ny, nz = r.nz;
+    const int l = threadIdx.x, row = threadIdx.y;
+    // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD walks a
+    // contiguous band of tiles (k fastest, then j, then i-chunk), so the halo
+    // rows and misaligned row ends that neighbouring tiles share meet in one L2
+    const int per = (n_tiles + 7) / 8;
+    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+    if (t >= n_tiles) return;
+    const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
+    const int k = tk * kFCoreK - 1 + l;
+    const int j = tj * kFCoreJ - 1 + row;
+    const int i0 = ti * chunk;
+    const int i1 = min(nx, i0 + chunk);
+    const bool in_g = j >= 0 && j < ny && k >= 0 && k < nz;
+    const bool f_ok = in_g && row <= kFRows - 2 && l <= kFLanes - 2;
+    const bool core = in_g && row >= 1 && row <= kFRows - 2 && l >= 1 && l <= kFLanes - 2;
+    // With nz >= 4 the z faces of Hx, Hy on rows j in [1, ny-2] of planes
+    // i in [1, nx-2] are finished here (their Mur reads no x- or y-face value),
+    // so Ampere runs on every k of the interior (i, j) columns; with nz = 1
+    // (no z faces) it runs on the [2, n-2] box and the shell kernel does the rest.
+    const bool zext = nz >= 4;
+    const bool box_jk = core && j >= 2 && j <= ny - 2 && (zext || (k >= 2 && k <= nz - 2));
+    const bool zface = zext && f_ok && (k == 0 || k == nz - 1) && j >= 1 && j <= ny - 2;
+    const Strips S(nx, ny, nz);
+    const int64_t xy = (int64_t)nx * ny;
+    const int64_t si = (int64_t)ny * nz, sj = nz;
+    const int64_t cjk = in_g ? (int64_t)j * sj + k : 0;
+    __shared__ float sE[3][kFRows][kFLanes];
+    __shared__ float sF[3][kFRows][kFLanes];
+    int i = i0 > 0 ? i0 - 1 : 0;
+    // registers: E of planes i and i+1 and H of plane i; the E and H loads for
+    // the following plane are issued before the barriers
+    float ex = 0.f, ey = 0.f, ez = 0.f, exn = 0.f, eyn = 0.f, ezn = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
+    if (in_g) {
+        const int64_t c = (int64_t)i * si + cjk;
+        ex = b.ex[c];
+        ey = b.ey[c];
+        ez = b.ez[c];
+        if (i + 1 < nx) {
+            exn = b.ex[c + si];
+            eyn = b.ey[c + si];
+            ezn = b.ez[c + si];
+        }
+        hx = b.hx[c];
+        hy = b.hy[c];
+        hz = b.hz[c];
+    }
+    float fhy_prev = 0.f, fhz_prev = 0.f;
+    for (; i < i1; ++i) {
+        const int64_t c = (int64_t)i * si + cjk;
+        float exnn = 0.f, eynn = 0.f, eznn = 0.f, hxn = 0.f, hyn = 0.f, hzn = 0.f, ca = 0.f, cb = 0.f;
+        if (kMat && box_jk && i >= i0) {  // used after two barriers: loaded for this plane
+            ca = r.ca[c];
+            cb = r.cb[c];
+        }
+        if (in_g && i + 1 < i1) {
+            if (i + 2 < nx) {
+                exnn = b.ex[c + 2 * si];
+                eynn = b.ey[c + 2 * si];
+                eznn = b.ez[c + 2 * si];
+            }
+            hxn = b.hx[c + si];
+            hyn = b.hy[c + si];
+            hzn = b.hz[c + si];
+        }
+        sE[0][row][l] = ex;
+        sE[1][row][l] = ey;
+        sE[2][row][l] = ez;
+        __syncthreads();
+        float fhx = hx, fhy = hy, fhz = hz;
+        if (f_ok) {
+            if (j <= ny - 2 && k <= nz - 2) {
+                const float a = (sE[1][row][l + 1] - ey) / r.dz;
+                const float bb = (sE[2][row + 1][l] - ez) / r.dy;
+                fhx = hx + r.dh * (a - bb);
+            }
+            if (i <= nx - 2 && k <= nz - 2) {
+                const float a = (ezn - ez) / r.dx;
+                const float bb = (sE[0][row][l + 1] - ex) / r.dz;
+                fhy = hy + r.dh * (a - bb);
+            }
+            if (i <= nx - 2 && j <= ny - 2) {
+                const float a = (sE[0][row + 1][l] - ex) / r.dy;
+                const float bb = (eyn - ey) / r.dx;
+                fhz = hz + r.dh * (a - bb);
+            }
+        }
+        sF[0][row][l] = fhx;
+        sF[1][row][l] = fhy;
+        sF[2][row][l] = fhz;
+        __syncthreads();
+        if (zface && i >= 1 && i <= nx - 2) {
+            // Mur on the z faces (boundaries.py:557-575): f[face] = P[in] + cz (f[in] - P[face]),
+            // the inner value from the neighbouring lane, P from the snapshot planes
+            const int64_t ij = (int64_t)i * ny + j;
+            const float *PE = r.pstrip + S.e, *PF = r.pstrip + S.f;
+            const int lin = k == 0 ? l + 1 : l - 1;
+            const int s_in = k == 0 ? 1 : 2, s_face = k == 0 ? 0 : 3;
+            fhx = PE[s_in * xy + ij] + r.cmz * (sF[0][row][lin] - PE[s_face * xy + ij]);
+            fhy = PF[s_in * xy + ij] + r.cmz * (sF[1][row][lin] - PF[s_face * xy + ij]);
+            sF[0][row][l] = fhx;
+            sF[1][row][l] = fhy;
+        }
+        __syncthreads();
+        if (core && i >= i0) {
+            b.nhx[c] = fhx;
+            b.nhy[c] = fhy;
+            b.nhz[c] = fhz;
+            if (box_jk && i >= 2 && i <= nx - 2) {
+                float vx = ex, vy = ey;
+                if (k >= 1) {
+                    const float cx = ((fhz - sF[2][row - 1][l]) / r.dy) - ((fhy - sF[1][row][l - 1]) / r.dz);
+                    const float cy = ((fhx - sF[0][row][l - 1]) / r.dz) - ((fhz - fhz_prev) / r.dx);
+                    vx = kMat ? ca * ex + cb * cx : ex + r.de * cx;
+                    vy = kMat ? ca * ey + cb * cy : ey + r.de * cy;
+                }
+                const float cz = ((fhy - fhy_prev) / r.dx) - ((fhx - sF[0][row - 1][l]) / r.dy);
+                b.nex[c] = vx;
+                b.ney[c] = vy;
+                b.nez[c] = kMat ? ca * ez + cb * cz : ez + r.de * cz;
+            }
+        }
+        fhy_prev = fhy;
+        fhz_prev = fhz;
+        ex = exn;
+        ey = eyn;
+        ez = ezn;
+        exn = exnn;
+        eyn = eynn;
+        ezn = eznn;
+        hx = hxn;
+        hy = hyn;
+        hz = hzn;
+    }
+}
+
+// Shell of an axis of length n: the indices outside [2, n-2] ({0, 1, n-1} for
+// n >= 4, {0} for n = 1), and the count of indices inside.
+__device__ __forceinline__ int shell_count(int n) { return n == 1 ? 1 : 3; }
+__device__ __forceinline__ int shell_index(int t, int n) { return t < 2 ? t : n - 1; }
+
+// E on every cell the fused kernel leaves: the planes i in {0, 1, nx-1}, the
+// rows j in {0, 1, ny-1} of the other planes, and (nz = 1 only) the rest;
+// updated where the reference updates it (from the final new H), copied where
+// it never does.
+template <bool kMat>
+__global__ void __launch_bounds__(256) k3_eshell(wf_run3d r, Bufs b) {
+    const int nx = r.nx, ny = r.ny, nz = r.nz;
+    const int sx = shell_count(nx), sy = shell_count(ny), sz = shell_count(nz);
+    const int mx = nx - sx, my = ny - sy;
+    const int64_t n1 = (int64_t)sx * ny * nz, n2 = (int64_t)mx * sy * nz;
+    const int64_t n3 = nz == 1 ? (int64_t)mx * my * sz : 0;  // with nz >= 4 the fused kernel does the k-slab
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t < n1) {
+        const int64_t yzc = (int64_t)ny * nz;
+        i = shell_index((int)(t / yzc), nx);
+        const int64_t u = t % yzc;
+        j = (int)(u / nz);
+        k = (int)(u % nz);
+    } else if ((t -= n1) < n2) {
+        const int64_t per_i = (int64_t)sy * nz;
+        i = 2 + (int)(t / per_i);
+        const int64_t u = t % per_i;
+        j = shell_index((int)(u / nz), ny);
+        k = (int)(u % nz);
+    } else if ((t -= n2) < n3) {
+        const int64_t per_i = (int64_t)my * sz;
+        i = 2 + (int)(t / per_i);
+        const int64_t u = t % per_i;
+        j = 2 + (int)(u / sz);
+        k = shell_index((int)(u % sz), nz);
+    } else {
+        return;
+    }
+    const int64_t sj = nz, si = (int64_t)ny * nz;
+    const int64_t c = (int64_t)i * si + (int64_t)j * sj + k;
+    const float *hx = b.nhx, *hy = b.nhy, *hz = b.nhz;
+    float vx = b.ex[c], vy = b.ey[c], vz = b.ez[c];
+    if (j >= 1 && k >= 1) {
+        const float curl = ((hz[c] - hz[c - sj]) / r.dy) - ((hy[c] - hy[c - 1]) / r.dz);
+        vx = kMat ? r.ca[c] * vx + r.cb[c] * curl : vx + r.de * curl;
+    }
+    if (i >= 1 && k >= 1) {
+        const float curl = ((hx[c] - hx[c - 1]) / r.dz) - ((hz[c] - hz[c - si]) / r.dx);
+        vy = kMat ? r.ca[c] * vy + r.cb[c] * curl : vy + r.de * curl;
+    }
+    if (i >= 1 && j >= 1) {
+        const float curl = ((hy[c] - hy[c - si]) / r.dx) - ((hx[c] - hx[c - sj]) / r.dy);
+        vz = kMat ? r.ca[c] * vz + r.cb[c] * curl : vz + r.de * curl;
+    }
+    b.nex[c] = vx;
+    b.ney[c] = vy;
+    b.nez[c] = vz;
+}
+
 int validate3(const wf_run3d &r) {
     if (!r.ex || !r.ey || !r.ez || !r.hx || !r.hy || !r.hz || !r.pstrip) {
         wf::set_error("run3d: null field or strip pointer");
@@ -265,6 +491,15 @@ int validate3(const wf_run3d &r) {
 }
 
 }  // namespace
+
+wf_run3d view_of(const wf_run3d &r, bool second) {
+    wf_run3d v = r;
+    if (second) {
+        v.ex = r.ex2; v.ey = r.ey2; v.ez = r.ez2;
+        v.hx = r.hx2; v.hy = r.hy2; v.hz = r.hz2;
+    }
+    return v;
+}
 
 extern "C" int64_t wf_fdtd3d_strip_floats(int32_t nx, int32_t ny, int32_t nz) { return Strips(nx, ny, nz).total; }
 
@@ -298,6 +533,63 @@ extern "C" int wf_fdtd3d_step(const wf_run3d *runs, int32_t n_runs, int32_t n_st
             if (r.n_det > 0) hipLaunchKernelGGL(k3_rec, dim3(1), dim3(256), 0, stream, r, n);
         }
         const int rc = wf::check_launch("wf_fdtd3d_step");
+        if (rc) return rc;
+    }
+    return WF_OK;
+}
+
+extern "C" int wf_fdtd3d_fused_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream) {
+    if (!runs || n_runs < 1 || n_steps < 0 || (n_steps & 1)) {
+        wf::set_error("wf_fdtd3d_fused_step: bad arguments (n_steps must be even)");
+        return WF_ERR_ARG;
+    }
+    for (int b = 0; b < n_runs; ++b) {
+        const int rc = validate3(runs[b]);
+        if (rc) return rc;
+        const wf_run3d &r = runs[b];
+        if (!r.ex2 || !r.ey2 || !r.ez2 || !r.hx2 || !r.hy2 || !r.hz2) {
+            wf::set_error("wf_fdtd3d_fused_step: second state buffers missing");
+            return WF_ERR_ARG;
+        }
+    }
+    for (int s = 0; s < n_steps; ++s) {
+        const bool odd = s & 1;
+        for (int bi = 0; bi < n_runs; ++bi) {
+            const wf_run3d rd = view_of(runs[bi], odd);   // read state
+            const wf_run3d wr = view_of(runs[bi], !odd);  // written state
+            const int n = runs[bi].step + s;
+            const Strips S(rd.nx, rd.ny, rd.nz);
+            const unsigned snap_blocks = (unsigned)std::min<int64_t>((S.total + 255) / 256, 4096);
+            hipLaunchKernelGGL(k3_snap, dim3(snap_blocks), dim3(256), 0, stream, rd);
+            if (rd.n_grp > 0) hipLaunchKernelGGL(k3_src, dim3(1), dim3(256), 0, stream, rd, n);
+            Bufs bf{rd.ex, rd.ey, rd.ez, rd.hx, rd.hy, rd.hz, wr.ex, wr.ey, wr.ez, wr.hx, wr.hy, wr.hz};
+            // i-chunks sized so the grid is about one round of resident workgroups
+            const int tiles = ((rd.nz + kFCoreK - 1) / kFCoreK) * ((rd.ny + kFCoreJ - 1) / kFCoreJ);
+            const int n_chunks = std::max(1, std::min((rd.nx + 7) / 8, kResidentBlocks / std::max(1, tiles)));
+            const int chunk = (rd.nx + n_chunks - 1) / n_chunks;
+            const int ntk = (rd.nz + kFCoreK - 1) / kFCoreK, ntj = (rd.ny + kFCoreJ - 1) / kFCoreJ;
+            const int n_tiles = ntk * ntj * ((rd.nx + chunk - 1) / chunk);
+            const dim3 gf(8u * (unsigned)((n_tiles + 7) / 8));
+            if (rd.ca)
+                hipLaunchKernelGGL(k3_fused<true>, gf, dim3(kFLanes, kFRows), 0, stream, rd, bf, chunk, ntk, ntj,
+                                   n_tiles);
+            else
+                hipLaunchKernelGGL(k3_fused<false>, gf, dim3(kFLanes, kFRows), 0, stream, rd, bf, chunk, ntk, ntj,
+                                   n_tiles);
+            const int64_t m1 = 4 * (int64_t)wr.ny * wr.nz + 2 * (int64_t)wr.nx * wr.nz;
+            hipLaunchKernelGGL(k3_mur1, dim3((unsigned)((m1 + 255) / 256)), dim3(256), 0, stream, wr);
+            const int64_t m2 = 2 * (int64_t)wr.nx * wr.nz + (wr.nz > 1 ? 4 * (int64_t)wr.nx * wr.ny : 0);
+            hipLaunchKernelGGL(k3_mur2, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, stream, wr);
+            const int sx = wr.nx == 1 ? 1 : 3, sy = wr.ny == 1 ? 1 : 3, sz = wr.nz == 1 ? 1 : 3;
+            const int64_t nsh = (int64_t)sx * wr.ny * wr.nz + (int64_t)(wr.nx - sx) * sy * wr.nz +
+                                (wr.nz == 1 ? (int64_t)(wr.nx - sx) * (wr.ny - sy) * sz : 0);
+            if (rd.ca)
+                hipLaunchKernelGGL(k3_eshell<true>, dim3((unsigned)((nsh + 255) / 256)), dim3(256), 0, stream, rd, bf);
+            else
+                hipLaunchKernelGGL(k3_eshell<false>, dim3((unsigned)((nsh + 255) / 256)), dim3(256), 0, stream, rd, bf);
+            if (wr.n_det > 0) hipLaunchKernelGGL(k3_rec, dim3(1), dim3(256), 0, stream, wr, n);
+        }
+        const int rc = wf::check_launch("wf_fdtd3d_fused_step");
         if (rc) return rc;
     }
     return WF_OK;
