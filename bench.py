@@ -60,6 +60,7 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
     p.add_argument("--no-3d", action="store_true", help="skip the 3D section")
     p.add_argument("--size3d", type=int, default=256)
+    p.add_argument("--steps3d", type=int, default=200)
     p.add_argument("--fd-cells", type=int, default=32, help="seeded cells of the central-difference check")
     p.add_argument("--copy-probe", action="store_true",
                    help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
@@ -475,7 +476,7 @@ def main() -> int:
         adjoint = adjoint_suite(dev, args.fd_cells)
     if not args.no_3d:
         print("[bench] 3D section", file=sys.stderr, flush=True)
-        fd3 = fdtd3d_suite(dev, args.size3d)
+        fd3 = fdtd3d_suite(dev, args.size3d, steps=args.steps3d)
     out = {
         "metric": METRIC,
         "value": round(value, 1),
