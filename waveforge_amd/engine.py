@@ -103,7 +103,7 @@ class SourceTable:
     """
 
     def __init__(self, sources: Sequence, Nx: int, Ny: int, device: torch.device, Nz: int = 1,
-                 comps_map: Optional[dict] = None) -> None:
+                 comps_map: Optional[dict] = None, upload: bool = True) -> None:
         """``comps_map`` maps the components the kernels update to their codes
         (2D TM: Ex, Ey, Hz; 3D: all six); cells are ``(i*Ny + j)*Nz + k``."""
         from .core.sources import PlaneSource
@@ -154,13 +154,36 @@ class SourceTable:
             self.n_inj = int(perm.size)
             self.n_grp = int(starts.size)
             self.group_cells = rec["cell"][starts].copy()
-            self.inj = torch.from_numpy(rec.view(np.uint8).copy()).to(device)
-            self.grp_start = torch.from_numpy(grp).to(device)
+            self.host_inj, self.host_grp = rec, grp
+            if upload:
+                self.inj = torch.from_numpy(rec.view(np.uint8).copy()).to(device)
+                self.grp_start = torch.from_numpy(grp).to(device)
+            else:
+                self.inj = self.grp_start = None
         else:
             self.n_inj = self.n_grp = 0
             self.group_cells = np.zeros(0, dtype=np.int32)
+            self.host_inj = self.host_grp = None
             self.inj = None
             self.grp_start = None
+
+    @staticmethod
+    def upload_many(tables: Sequence["SourceTable"], device: torch.device) -> None:
+        """Upload the host records of tables built with ``upload=False`` in one
+        copy each for contributions and group offsets; every table then points
+        into the shared buffers."""
+        live = [t for t in tables if t.n_grp]
+        if not live:
+            return
+        inj = torch.from_numpy(np.concatenate([t.host_inj.view(np.uint8) for t in live])).to(device)
+        grp = torch.from_numpy(np.concatenate([t.host_grp for t in live])).to(device)
+        o_i = o_g = 0
+        for t in live:
+            n_b, n_g = t.host_inj.nbytes, t.host_grp.size
+            t.inj, t.grp_start = inj[o_i:o_i + n_b], grp[o_g:o_g + n_g]
+            t.keepalive += [inj, grp]
+            o_i += n_b
+            o_g += n_g
 
     def ptrs(self) -> tuple[int, int]:
         return (self.inj.data_ptr() if self.inj is not None else 0,
@@ -247,6 +270,11 @@ def tb_limits_ok(run: RunState, tile_i: int, depth: int) -> bool:
     """True if every tile region of ``run`` holds at most the per-tile source
     groups and core detectors the temporally blocked kernel can stage."""
     TI, TJ = tile_i, _native.tile_cols(depth)
+    n_grp = run.sources.n_grp if run.sources is not None else 0
+    n_det = run.det_cell.numel() if run.det_cell is not None else 0
+    if n_grp <= TB_MAX_GROUPS_PER_TILE and n_det <= TB_MAX_DETS_PER_TILE and \
+            not (run.sources is not None and run.sources.inert):
+        return True  # no tile can hold more than the whole run has
     nti = (run.nx + TI - 1) // TI
     ntj = (run.ny + TJ - 1) // TJ
     if run.sources is not None and run.sources.n_grp:

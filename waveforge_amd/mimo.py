@@ -75,21 +75,47 @@ class BatchedRuns:
         self.n_check, self.threshold = n_check, stability_threshold
         coef = StepCoefficients.from_grid(dt, dx, dy)
         self.runs: list[RunState] = []
+        dev = self.device
+        # one zeroed block for every run's two states, one for every trace buffer,
+        # one upload for all detector cells and one for all source tables
         for k, sp in enumerate(self.specs):
-            dev = self.device
-            ex, ey, hz, ex2, ey2, hz2 = (torch.zeros((sp.nx, sp.ny), dtype=torch.float32, device=dev)
-                                         for _ in range(6))
+            if sp.nx < 4 or sp.ny < 4:
+                raise ValueError(f"run {k}: grid must be at least 4x4, got {sp.nx}x{sp.ny}")
+            for (c, i, j, _w, _a) in sp.sources:
+                if not (0 <= i < sp.nx and 0 <= j < sp.ny):
+                    raise ValueError(f"run {k}: source cell ({i}, {j}) outside the {sp.nx}x{sp.ny} grid")
+            for (i, j) in sp.receivers:
+                if not (0 <= i < sp.nx and 0 <= j < sp.ny):
+                    raise ValueError(f"run {k}: receiver ({i}, {j}) outside the {sp.nx}x{sp.ny} grid")
+        cells = [sp.nx * sp.ny for sp in self.specs]
+        self._state = torch.zeros(6 * sum(cells), dtype=torch.float32, device=dev)
+        n_rec = [len(sp.receivers) * max(1, sp.n_record) if sp.receivers else 0 for sp in self.specs]
+        self._trace_buf = torch.zeros(max(1, sum(n_rec)), dtype=torch.float32, device=dev)
+        det_host = [np.array([i * sp.ny + j for i, j in sp.receivers], dtype=np.int32) if sp.receivers
+                    else np.zeros(0, dtype=np.int32) for sp in self.specs]
+        det_all = torch.from_numpy(np.concatenate(det_host) if det_host else np.zeros(0, np.int32)).to(dev)
+        tables = []
+        o_s = o_t = o_d = 0
+        for k, sp in enumerate(self.specs):
+            n = cells[k]
+            f = self._state[o_s:o_s + 6 * n].view(6, sp.nx, sp.ny)
+            o_s += 6 * n
             srcs = [CellSource(c, w, i, j, a) for (c, i, j, w, a) in sp.sources]
-            table = SourceTable(srcs, sp.nx, sp.ny, dev)
-            rs = RunState(ex, ey, hz, sp.ca, sp.cb, sp.nx, sp.ny, coef, sources=table)
-            rs.second = (ex2, ey2, hz2)
+            table = SourceTable(srcs, sp.nx, sp.ny, dev, upload=False)
+            tables.append(table)
+            rs = RunState(f[0], f[1], f[2], sp.ca, sp.cb, sp.nx, sp.ny, coef, sources=table)
+            rs.second = (f[3], f[4], f[5])
             if sp.receivers:
-                cells = [i * sp.ny + j for i, j in sp.receivers]
-                rs.det_cell = torch.tensor(cells, dtype=torch.int32, device=dev)
-                rs.traces = torch.zeros((len(cells), max(1, sp.n_record)), dtype=torch.float32, device=dev)
+                nd = len(sp.receivers)
+                rs.det_cell = det_all[o_d:o_d + nd]
+                rs.extra["det_cells_np"] = det_host[k].astype(np.int64)
+                rs.traces = self._trace_buf[o_t:o_t + n_rec[k]].view(nd, max(1, sp.n_record))
                 rs.trace_n0 = 0
+                o_d += nd
+                o_t += n_rec[k]
             rs.step = 0
             self.runs.append(rs)
+        SourceTable.upload_many(tables, dev)
         self.tuned = None
         if "auto" in (tile_rows, depth):
             from . import autotune
