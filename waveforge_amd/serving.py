@@ -65,6 +65,7 @@ class ImagingRequest:
     failed: bool = False
     _event: object = None
     _runs: list = field(default_factory=list)
+    _state: object = None
     _bytes: int = 0
 
     @property
@@ -133,13 +134,17 @@ class ImagingServer:
         ca, cb = ca.reshape(nx, ny), cb.reshape(nx, ny)
         wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(
             req.n_steps + self.quantum, self.dt, dev, torch.float32)
+        if any(not (0 <= i < nx and 0 <= j < ny) for i, j in req.antennas):
+            raise ValueError(f"request {req.rid}: antenna cell outside the {nx}x{ny} grid")
         cells = torch.tensor([i * ny + j for i, j in req.antennas], dtype=torch.int32, device=dev)
         n_ant = len(req.antennas)
         req.traces = torch.zeros((n_ant, n_ant, req.n_steps), dtype=torch.float32, device=dev)
-        runs = []
+        runs, tables = [], []
+        state = torch.zeros((n_ant, 6, nx, ny), dtype=torch.float32, device=dev)  # both states of every TX run
         for t, (i, j) in enumerate(req.antennas):
-            f = [torch.zeros((nx, ny), dtype=torch.float32, device=dev) for _ in range(6)]
-            table = SourceTable([CellSource("Hz", wave, i, j, 1.0)], nx, ny, dev)
+            f = state[t]
+            table = SourceTable([CellSource("Hz", wave, i, j, 1.0)], nx, ny, dev, upload=False)
+            tables.append(table)
             rs = RunState(f[0], f[1], f[2], ca, cb, nx, ny, self.coef, sources=table, det_cell=cells,
                           traces=req.traces[t], trace_n0=0, step=0)
             rs.second = (f[3], f[4], f[5])
@@ -149,7 +154,9 @@ class ImagingServer:
                 if not tb_limits_ok(rs, self.tile_rows, d):
                     raise ValueError(f"request {req.rid}: receivers too dense for the batched kernel")
             runs.append(rs)
+        SourceTable.upload_many(tables, dev)
         req._runs = runs
+        req._state = state
         req._bytes = self.footprint(req)
         req.admitted = now
         self.used += req._bytes
@@ -235,6 +242,7 @@ class ImagingServer:
                 for r in req._runs:
                     r.ex = r.ey = r.hz = r.second = None
                 req._runs = []
+                req._state = None
                 self.done.append(req)
             else:
                 still.append(req)
