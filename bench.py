@@ -64,7 +64,7 @@ def parse():
     p.add_argument("--fd-cells", type=int, default=32, help="seeded cells of the central-difference check")
     p.add_argument("--copy-probe", action="store_true",
                    help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
-                        "traffic used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
+                        "traffic used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings; prints its rate")
     return p.parse_args()
 
 
@@ -74,10 +74,17 @@ def copy_probe() -> int:
     n = 512 * 1024 * 1024 // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda:0")
     dst = torch.empty_like(src)
-    for _ in range(3):
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record()
+    for k in range(3):
         dst.copy_(src)
+        ev[k + 1].record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "dispatches": 3}))
+    ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(3)]
+    # the copy's read + write rate: the streaming bandwidth a kernel can expect on this stack
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "dispatches": 3,
+                      "copy_ms": [round(m, 4) for m in ms],
+                      "copy_gb_per_s": round(2 * n * 4 / (min(ms) / 1e3) / 1e9, 1)}))
     return 0
 
 
@@ -285,7 +292,7 @@ def fdtd3d_suite(dev, n: int, steps: int = 200, warmup: int = 20) -> dict:
         finals.append(f.Ez.clone())
         out["fields_finite"] = bool(all(torch.isfinite(getattr(f, k)).all() for k in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")))
     out["paths_equal"] = bool(torch.equal(finals[0], finals[1]))
-    out["default_path"] = "per_stage"
+    out["default_path"] = "fused"
     return out
 
 

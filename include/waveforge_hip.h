@@ -212,8 +212,17 @@ typedef struct wf_run3d {
     float dh, de, dx, dy, dz;     /* fp32(dt/MU0), fp32(dt/EPS0), fp32 cell sizes */
     float cmx, cmy, cmz;          /* fp32 Mur coefficients (boundaries.py:468-470) */
     int32_t step;
-    int32_t flags;                /* reserved, 0 */
+    int32_t flags;                /* WF_RUN3D_* bits, else 0 */
 } wf_run3d;
+
+/* wf_run3d.flags bit, honoured by wf_fdtd3d_fused_step only: the Mur snapshot
+ * (boundaries.py:495-504, taken before the sources are added) equals the read
+ * state on every Mur plane, because no H-component source cell lies on a
+ * plane at index 0, 1, n-2 or n-1 across either tangential axis of its
+ * component. The step then takes no snapshot copy: the Mur faces read the
+ * read state, and the z faces of Hx, Hy are applied inside the streaming
+ * pass. Setting it when a source does lie on such a plane gives wrong faces. */
+#define WF_RUN3D_SNAP_FROM_STATE 1
 
 /* Floats of the Mur snapshot workspace of one 3D run: the planes at index
  * 0, 1, n-2, n-1 across each tangential axis of each H component,
@@ -230,12 +239,14 @@ int wf_fdtd3d_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStr
 
 /* The same steps with one fused pass per step: step s reads state A (ex..hz)
  * and writes state B (ex2..hz2) when s is even, the reverse when odd, so
- * `n_steps` must be even and the result lands in state A. Per step: Mur
- * snapshot planes and sources on the read state, one launch that streams
- * each (j, k) tile along i computing Faraday everywhere and Ampere on the
- * cells whose stencil touches no Mur face, then the Mur faces and the
- * remaining E cells, then the detectors. Results equal wf_fdtd3d_step's bit
- * for bit. */
+ * `n_steps` must be even and the result lands in state A. Per step: the Mur
+ * snapshot planes (skipped under WF_RUN3D_SNAP_FROM_STATE) and the sources on
+ * the read state; one launch in which each wavefront streams a 4-row x 64-lane
+ * (j, k) tile along i, computing Faraday everywhere, the z faces of Hx, Hy
+ * away from the x and y faces (under the flag), and Ampere on the cells whose
+ * stencil touches no face still to be written; then the Mur faces left, the
+ * remaining E cells and the detectors. Results equal wf_fdtd3d_step's bit for
+ * bit. Requires 4*ny*nz < 2^31 (WF_ERR_UNSUPPORTED otherwise). */
 int wf_fdtd3d_fused_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream);
 
 /* Stability judgement on the device, so the stepping stream never waits for

@@ -31,6 +31,14 @@ def test_library_loads_and_exports_every_symbol():
     assert isinstance(lib.wf_last_error(), bytes)
 
 
+def test_header_constants_match_binding():
+    text = HEADER.read_text()
+    defs = dict(re.findall(r"^#define (WF_\w+) \(?(-?\d+)\)?", text, flags=re.M))
+    assert int(defs["WF_RUN3D_SNAP_FROM_STATE"]) == _native.WF_RUN3D_SNAP_FROM_STATE
+    for name, code in _native.WF_COMP3.items():
+        assert int(defs[f"WF_COMP_{name.upper()}"]) == code
+
+
 def test_tile_shapes_listed():
     rows = _native.tile_rows()
     assert 32 in rows and all(r > 0 for r in rows)

@@ -170,3 +170,34 @@ def test_division_range_for_millimetre_cells():
     lo, span = division_range(float(np.float32(1e-3)))
     assert lo == (127 - 99) << 23
     assert lo + span == (127 + 115) << 23
+
+
+def test_snapshot_free_fused_step_condition():
+    """The fused 3D step skips the Mur snapshot copy only when no H source cell
+    lies on a Mur plane of its component (index 0, 1, n-2, n-1 across a
+    tangential axis); E sources never matter."""
+    from types import SimpleNamespace
+
+    from waveforge_amd import _native
+    from waveforge_amd.core.fdtd3d import snapshot_is_state
+    from waveforge_amd.engine import INJ_DTYPE
+
+    nx, ny, nz = 20, 16, 12
+
+    def table(entries, nz=nz):
+        rec = np.zeros(len(entries), dtype=INJ_DTYPE)
+        for n, (comp, i, j, k) in enumerate(entries):
+            rec[n]["comp"] = _native.WF_COMP3[comp]
+            rec[n]["cell"] = (i * ny + j) * nz + k
+        return SimpleNamespace(n_grp=len(entries), host_inj=rec)
+
+    assert snapshot_is_state(SimpleNamespace(n_grp=0, host_inj=None), nx, ny, nz)
+    assert snapshot_is_state(table([("Ez", 0, 0, 0), ("Ex", 1, 15, 11), ("Ey", 19, 1, 10)]), nx, ny, nz)
+    assert snapshot_is_state(table([("Hx", 0, 5, 5), ("Hy", 7, 0, 5), ("Hz", 7, 7, 0)]), nx, ny, nz)  # normal axes
+    assert snapshot_is_state(table([("Hx", 10, 2, 9), ("Hy", 17, 13, 2), ("Hz", 2, 13, 11)]), nx, ny, nz)
+    for bad in [("Hx", 5, 1, 5), ("Hx", 5, 5, 10), ("Hy", 18, 5, 5), ("Hy", 5, 5, 0), ("Hz", 0, 5, 5),
+                ("Hz", 5, 14, 5)]:
+        assert not snapshot_is_state(table([("Ez", 5, 5, 5), bad]), nx, ny, nz), bad
+    # Nz = 1: no z faces, so k = 0 is no Mur plane
+    assert snapshot_is_state(table([("Hx", 5, 5, 0), ("Hy", 5, 5, 0)], nz=1), nx, ny, 1)
+    assert not snapshot_is_state(table([("Hx", 5, 0, 0)], nz=1), nx, ny, 1)

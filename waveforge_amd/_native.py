@@ -16,6 +16,7 @@ ABI_VERSION = 1
 
 WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 WF_COMP3 = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
+WF_RUN3D_SNAP_FROM_STATE = 1  # wf_run3d.flags: the Mur snapshot equals the read state
 TB_MAX_DEPTH = 8
 
 # Every symbol the header declares; tests check the built library exports them.

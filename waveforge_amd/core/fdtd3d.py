@@ -6,14 +6,14 @@ properties as ``src/core/fdtd3d.py`` (``FDTD3D`` :118-580,
 short sequence of HIP launches (``wf_fdtd3d_step``: Mur snapshot planes,
 sources, Faraday, Mur faces in the reference's order, Ampere, detectors);
 ``run(n)`` advances in chunks that end on the ``n_check`` steps, where the
-six-component stability check runs exactly as the reference's. With
-``fused=True`` the steps of a chunk take the fused path
-(``wf_fdtd3d_fused_step``: one streaming pass per step over a second state
-buffer, same values bit for bit), with one in-place step first when the
-chunk is odd, so results always land in the ``FieldSet`` tensors; it moves
-as many HBM bytes as the per-stage pair on this hardware (its tiles' halo
-rows and unaligned row ends are fetched again by neighbouring tiles), so the
-per-stage path is the default.
+six-component stability check runs exactly as the reference's. The steps of
+a chunk take the fused path by default (``wf_fdtd3d_fused_step``: one
+streaming pass per step that reads the ``FieldSet`` tensors' state and writes
+a second state buffer, then the reverse, same values bit for bit), with one
+in-place step first when the chunk is odd, so results always land in the
+``FieldSet`` tensors; it moves about 68 B per cell-update against the
+per-stage pair's 90 and runs about 1.4 times as fast at 256^3. It holds a
+second set of six field buffers; ``fused=False`` keeps every step in place.
 
 The ``FieldSet`` tensors are the live state (read and written in place);
 ``sim._sources`` is read at every call. ``compile_step`` exists for API
@@ -29,6 +29,7 @@ import time
 import warnings
 from typing import Optional
 
+import numpy as np
 import torch
 
 from .. import _native
@@ -40,6 +41,30 @@ from .grid import YeeGrid
 EPS0: float = 8.8541878128e-12
 MU0: float = 1.2566370614e-6
 _C0: float = 1.0 / math.sqrt(EPS0 * MU0)
+
+
+def snapshot_is_state(table: SourceTable, Nx: int, Ny: int, Nz: int) -> bool:
+    """True when no H-component source cell lies on a Mur plane of its
+    component (index 0, 1, n-2 or n-1 across either tangential axis: Hx y and
+    z, Hy x and z, Hz x and y; ``boundaries.py:495-504``). The snapshot, taken
+    before the sources are added, then equals the state the fused pass reads,
+    and the fused step needs no snapshot copy."""
+    if not table.n_grp:
+        return True
+    comp = table.host_inj["comp"]
+    cell = table.host_inj["cell"].astype(np.int64)
+    k = cell % Nz
+    j = (cell // Nz) % Ny
+    i = cell // (Ny * Nz)
+
+    def on(v, n):
+        return (v <= 1) | (v >= n - 2)
+
+    zf = on(k, Nz) if Nz > 1 else np.zeros(cell.shape, dtype=bool)
+    c = _native.WF_COMP3
+    bad = ((comp == c["Hx"]) & (on(j, Ny) | zf)) | ((comp == c["Hy"]) & (on(i, Nx) | zf)) | \
+          ((comp == c["Hz"]) & (on(i, Nx) | on(j, Ny)))
+    return not bool(bad.any())
 
 
 class SimulationDivergedError(RuntimeError):
@@ -55,7 +80,7 @@ class FDTD3D:
     def __init__(self, grid: YeeGrid, fields: FieldSet, boundary: object, sources: Optional[object] = None, *,
                  Ca: Optional[torch.Tensor] = None, Cb: Optional[torch.Tensor] = None,
                  stability_threshold: float = 1e10, n_check: int = 100,
-                 detectors: Optional[DetectorSet] = None, fused: bool = False) -> None:
+                 detectors: Optional[DetectorSet] = None, fused: bool = True) -> None:
         if grid.Nx < 4:
             raise ValueError(f"grid.Nx must be >= 4 for a valid 3D Yee stencil, got {grid.Nx}")
         if grid.Ny < 4:
@@ -144,6 +169,7 @@ class FDTD3D:
                 self._second = tuple(torch.empty_like(f.Hz) for _ in range(6))
             second = [self._members(t) for t in self._second]
         b_ = self._boundary
+        flags = _native.WF_RUN3D_SNAP_FROM_STATE if snapshot_is_state(self._src_table, g.Nx, g.Ny, g.Nz) else 0
         self._strips, structs = [], []
         for b in range(B):
             r = _native.WfRun3d()
@@ -170,6 +196,7 @@ class FDTD3D:
             r.cmx = f32(getattr(b_, "C_mur_x", (_C0_REF * g.dt - g.dx) / (_C0_REF * g.dt + g.dx)))
             r.cmy = f32(getattr(b_, "C_mur_y", (_C0_REF * g.dt - g.dy) / (_C0_REF * g.dt + g.dy)))
             r.cmz = f32(getattr(b_, "C_mur_z", (_C0_REF * g.dt - g.dz) / (_C0_REF * g.dt + g.dz)))
+            r.flags = flags  # read by the fused path only
             structs.append(r)
         self._structs = structs
         self._key = key

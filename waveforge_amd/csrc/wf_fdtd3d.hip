@@ -134,7 +134,23 @@ __device__ __forceinline__ void mur_face(float *f, int64_t face, int64_t in, flo
     f[face] = p_in + C * (f[in] - p_face);
 }
 
-__global__ void __launch_bounds__(256) k3_mur1(wf_run3d r) {
+// The H of the read state: where the snapshot is taken from when the run
+// carries WF_RUN3D_SNAP_FROM_STATE (the fused path reads one state and writes
+// another, so the read state still holds the step's starting H).
+struct Prev {
+    const float *hx, *hy, *hz;
+};
+
+// Snapshot value of component f at plane index s (0, 1, n-2, n-1 for s = 0..3)
+// across an axis: from the strip planes at P[s*plane + idx], or from the read
+// state at `cell`.
+template <bool kState>
+__device__ __forceinline__ float snap(const float *P, int s, int64_t plane, int64_t idx, const float *f, int64_t cell) {
+    return kState ? f[cell] : P[s * plane + idx];
+}
+
+template <bool kState>
+__global__ void __launch_bounds__(256) k3_mur1(wf_run3d r, Prev pv) {
     const int nx = r.nx, ny = r.ny, nz = r.nz;
     const Strips S(nx, ny, nz);
     const int64_t yz = (int64_t)ny * nz, xz = (int64_t)nx * nz;
@@ -146,10 +162,12 @@ __global__ void __launch_bounds__(256) k3_mur1(wf_run3d r) {
         const int side = (int)(u / yz);
         const int64_t jk = u - side * yz;
         const float *P = r.pstrip + (is_hz ? S.b : S.a);
+        const float *F = is_hz ? pv.hz : pv.hy;
         float *f = is_hz ? r.hz : r.hy;
         const int fi = side ? nx - 1 : 0, ii = side ? nx - 2 : 1;
-        const float p_in = P[(side ? 2 : 1) * yz + jk], p_face = P[(side ? 3 : 0) * yz + jk];
-        mur_face(f, fi * yz + jk, ii * yz + jk, p_in, p_face, r.cmx);
+        const int64_t face = fi * yz + jk, in = ii * yz + jk;
+        mur_face(f, face, in, snap<kState>(P, side ? 2 : 1, yz, jk, F, in), snap<kState>(P, side ? 3 : 0, yz, jk, F, face),
+                 r.cmx);
     } else if (t < 2 * n_x + n_y) {  // Hx on y faces
         const int64_t u = t - 2 * n_x;
         const int side = (int)(u / xz);
@@ -157,16 +175,27 @@ __global__ void __launch_bounds__(256) k3_mur1(wf_run3d r) {
         const int64_t i = ik / nz, k = ik - i * nz;
         const float *P = r.pstrip + S.c;
         const int fj = side ? ny - 1 : 0, ij = side ? ny - 2 : 1;
-        const float p_in = P[(side ? 2 : 1) * xz + ik], p_face = P[(side ? 3 : 0) * xz + ik];
-        mur_face(r.hx, (i * ny + fj) * nz + k, (i * ny + ij) * nz + k, p_in, p_face, r.cmy);
+        const int64_t face = (i * ny + fj) * nz + k, in = (i * ny + ij) * nz + k;
+        mur_face(r.hx, face, in, snap<kState>(P, side ? 2 : 1, xz, ik, pv.hx, in),
+                 snap<kState>(P, side ? 3 : 0, xz, ik, pv.hx, face), r.cmy);
     }
 }
 
-__global__ void __launch_bounds__(256) k3_mur2(wf_run3d r) {
+// z-face columns (i, j) per face: all nx*ny, or (kEdges) only those on an x or
+// y face, which are the ones the fused pass leaves when it applies the rest.
+template <bool kEdges>
+__host__ __device__ __forceinline__ int64_t zface_columns(int nx, int ny, int nz) {
+    if (nz == 1) return 0;
+    return kEdges ? 2 * (int64_t)ny + 2 * (int64_t)(nx - 2) : (int64_t)nx * ny;
+}
+
+template <bool kState>
+__global__ void __launch_bounds__(256) k3_mur2(wf_run3d r, Prev pv) {
     const int nx = r.nx, ny = r.ny, nz = r.nz;
     const Strips S(nx, ny, nz);
     const int64_t xz = (int64_t)nx * nz, xy = (int64_t)nx * ny;
-    const int64_t n_y = 2 * xz, n_z = nz > 1 ? 2 * xy : 0;
+    const int64_t zc = zface_columns<kState>(nx, ny, nz);
+    const int64_t n_y = 2 * xz, n_z = 2 * zc;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n_y) {  // Hz on y faces
         const int side = (int)(t / xz);
@@ -174,18 +203,30 @@ __global__ void __launch_bounds__(256) k3_mur2(wf_run3d r) {
         const int64_t i = ik / nz, k = ik - i * nz;
         const float *P = r.pstrip + S.d;
         const int fj = side ? ny - 1 : 0, ij = side ? ny - 2 : 1;
-        const float p_in = P[(side ? 2 : 1) * xz + ik], p_face = P[(side ? 3 : 0) * xz + ik];
-        mur_face(r.hz, (i * ny + fj) * nz + k, (i * ny + ij) * nz + k, p_in, p_face, r.cmy);
+        const int64_t face = (i * ny + fj) * nz + k, in = (i * ny + ij) * nz + k;
+        mur_face(r.hz, face, in, snap<kState>(P, side ? 2 : 1, xz, ik, pv.hz, in),
+                 snap<kState>(P, side ? 3 : 0, xz, ik, pv.hz, face), r.cmy);
     } else if (t < n_y + 2 * n_z) {  // Hx then Hy on z faces
         const bool is_hy = t >= n_y + n_z;
         const int64_t u = t - n_y - (is_hy ? n_z : 0);
-        const int side = (int)(u / xy);
-        const int64_t ij = u - side * xy;
+        const int side = (int)(u / zc);
+        int64_t ij = u - side * zc;
+        if (kState) {  // edge columns: i in {0, nx-1} (every j), then j in {0, ny-1} (i in [1, nx-2])
+            const int64_t e = ij;
+            if (e < 2 * (int64_t)ny) {
+                ij = (e < ny ? 0 : (int64_t)(nx - 1) * ny) + e % ny;
+            } else {
+                const int64_t f = e - 2 * (int64_t)ny;
+                ij = (1 + f % (nx - 2)) * ny + (f < nx - 2 ? 0 : ny - 1);
+            }
+        }
         const float *P = r.pstrip + (is_hy ? S.f : S.e);
+        const float *F = is_hy ? pv.hy : pv.hx;
         float *f = is_hy ? r.hy : r.hx;
         const int fk = side ? nz - 1 : 0, ik = side ? nz - 2 : 1;
-        const float p_in = P[(side ? 2 : 1) * xy + ij], p_face = P[(side ? 3 : 0) * xy + ij];
-        mur_face(f, ij * nz + fk, ij * nz + ik, p_in, p_face, r.cmz);
+        const int64_t face = ij * nz + fk, in = ij * nz + ik;
+        mur_face(f, face, in, snap<kState>(P, side ? 2 : 1, xy, ij, F, in), snap<kState>(P, side ? 3 : 0, xy, ij, F, face),
+                 r.cmz);
     }
 }
 
@@ -246,166 +287,253 @@ __global__ void __launch_bounds__(256) k3_rec(wf_run3d r, int n) {
 // ---------------------------------------------------------------------------
 // Fused path: one streaming pass per step over ping-pong buffers.
 //
-// A workgroup of 16 x 64 threads owns a (j, k) tile: rows j0-1 .. j0+14 and
-// lanes k0-1 .. k0+62, i.e. a core of 14 x 62 cells with a one-cell halo, and
-// walks i over a chunk of planes (starting one plane early so the
-// previous plane's new H is in registers). For each plane it stages the old E
-// of the plane in LDS, computes Faraday (new H, "F") on every cell but the
-// high-side halo from the LDS neighbours j+1, k+1 and the next plane's E in
-// registers, stages F in LDS, and writes the core's F and, on the cells whose
-// Ampere stencil touches no Mur face cell ([2, n-2] on every axis), the new
-// E from the LDS neighbours j-1, k-1 and the previous plane's F. The Mur faces
-// (k3_mur1/k3_mur2 on the written state) and the E cells outside that box
+// A wavefront owns a (j, k) tile of R rows (j0 .. j0+R-1) by 64 lanes
+// (k0 .. k0+63, k0 a multiple of 64, so each row access is whole aligned
+// lines) and walks i over a chunk of planes, starting one plane early so the
+// previous plane's new H is in registers. Row j0-1 (whose new Hx, Hz Ampere
+// reads at j-1) and row j0+R (whose old Ex, Ez Faraday reads at j+1) ride
+// along. The k-neighbours are the adjacent lanes (DPP whole-wave shifts); the
+// two that cross the tile's k edges -- old Ex, Ey at k0+64 for Faraday in
+// lane 63 and the new Hx, Hy at k0-1 for Ampere in lane 0 -- are loaded and
+// computed once per row in spare lanes (lane q serves row q) and handed to
+// the edge lane with readlane. There is no LDS and no barrier: every
+// wavefront is independent; a workgroup holds consecutive tiles only so that
+// they run side by side on one CU.
+//
+// Per plane: Faraday on every tile cell; with WF_RUN3D_SNAP_FROM_STATE the Mur
+// z faces of Hx, Hy on rows j in [1, ny-2] of planes i in [1, nx-2] (they read
+// no x- or y-face value; the snapshot is the old H in registers); then the new
+// H of the core rows and, on the cells whose Ampere stencil touches no Mur face
+// cell (i, j in [2, n-2]; k in [2, nz-2], or every k when the z faces were
+// applied here), the new E. The x and y faces (k3_mur1/k3_mur2 on the written
+// state; z faces again on the edge columns) and the E cells outside that box
 // (k3_eshell) follow. Reads come only from the old state, so tiles and chunks
 // never race.
 // ---------------------------------------------------------------------------
-constexpr int kFLanes = 64;
-constexpr int kFRows = 16;
-constexpr int kFCoreK = kFLanes - 2;
-constexpr int kFCoreJ = kFRows - 2;
-constexpr int kResidentBlocks = 512;  // 256 CUs x 2 workgroups of 1024 threads
+constexpr int kSRows = 4;   // rows per wavefront tile
+constexpr int kSWaves = 4;  // wavefronts per SIMD the register budget is sized for
+constexpr int kSGroup = 4;  // wavefronts per workgroup (consecutive tiles)
 
 struct Bufs {
     const float *ex, *ey, *ez, *hx, *hy, *hz;  // state read (old)
     float *nex, *ney, *nez, *nhx, *nhy, *nhz;  // state written (new)
 };
 
-template <bool kMat>
-__global__ void __launch_bounds__(kFLanes * kFRows) __attribute__((amdgpu_waves_per_eu(8, 8)))
-k3_fused(wf_run3d r, Bufs b, int chunk, int ntk, int ntj, int n_tiles) {
+__device__ __forceinline__ float shl_fill(float v, float fill) {  // lane+1's v; lane 63 takes fill
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shr_fill(float v, float fill) {  // lane-1's v; lane 0 takes fill
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(fill), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_of(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// One plane of one component as a raw buffer: a wave-uniform descriptor plus a
+// 32-bit lane byte offset per access; an offset at or past the plane's end
+// (kOut) loads 0 and stores nothing.
+constexpr uint32_t kOut = 0xffffffffu;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_of(const float *base, int64_t o, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base + o), 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t p, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p, off, 0, 0));
+}
+__device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t p, uint32_t off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, off, 0, 0);
+}
+
+// Tile-edge values in the spare lanes: lane q holds row j0-1+q's old E and H
+// at k0-1 with Ez at j+1 and i+1 and Ey, Ex at k0 (its new Hx, Hy), and its
+// old Ex, Ey at k0+64.
+struct Edge {
+    float x, y, z, hx, hy, zj, zi, y0, x0;
+    float ex, ey;
+};
+
+template <bool kMat, int R>
+__global__ void __launch_bounds__(64 * kSGroup) __attribute__((amdgpu_waves_per_eu(kSWaves)))
+k3_stream(wf_run3d r, Bufs b, int chunk, int ntk, int ntj, int n_tiles) {
     const int nx = r.nx, ny = r.ny, nz = r.nz;
-    const int l = threadIdx.x, row = threadIdx.y;
-    // XCD-aware tile order: workgroup b runs on XCD b % 8; each XCD walks a
-    // contiguous band of tiles (k fastest, then j, then i-chunk), so the halo
-    // rows and misaligned row ends that neighbouring tiles share meet in one L2
-    const int per = (n_tiles + 7) / 8;
-    const int t = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
-    if (t >= n_tiles) return;
+    const int l = (int)(threadIdx.x & 63u);
+    // XCD-aware order: workgroup w runs on XCD w % 8; each XCD takes a
+    // contiguous band of tiles (k fastest, then j, then i-chunk), so the rows
+    // neighbouring tiles share meet in one L2
+    const int per = (int)(gridDim.x >> 3);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: keeps tile math scalar
+    const int t = ((int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3)) * kSGroup + wave;
+    if (t >= n_tiles) return;  // whole wavefront
     const int tk = t % ntk, tj = (t / ntk) % ntj, ti = t / (ntk * ntj);
-    const int k = tk * kFCoreK - 1 + l;
-    const int j = tj * kFCoreJ - 1 + row;
-    const int i0 = ti * chunk;
-    const int i1 = min(nx, i0 + chunk);
-    const bool in_g = j >= 0 && j < ny && k >= 0 && k < nz;
-    const bool f_ok = in_g && row <= kFRows - 2 && l <= kFLanes - 2;
-    const bool core = in_g && row >= 1 && row <= kFRows - 2 && l >= 1 && l <= kFLanes - 2;
-    // With nz >= 4 the z faces of Hx, Hy on rows j in [1, ny-2] of planes
-    // i in [1, nx-2] are finished here (their Mur reads no x- or y-face value),
-    // so Ampere runs on every k of the interior (i, j) columns; with nz = 1
-    // (no z faces) it runs on the [2, n-2] box and the shell kernel does the rest.
-    const bool zext = nz >= 4;
-    const bool box_jk = core && j >= 2 && j <= ny - 2 && (zext || (k >= 2 && k <= nz - 2));
-    const bool zface = zext && f_ok && (k == 0 || k == nz - 1) && j >= 1 && j <= ny - 2;
-    const Strips S(nx, ny, nz);
-    const int64_t xy = (int64_t)nx * ny;
-    const int64_t si = (int64_t)ny * nz, sj = nz;
-    const int64_t cjk = in_g ? (int64_t)j * sj + k : 0;
-    __shared__ float sE[3][kFRows][kFLanes];
-    __shared__ float sF[3][kFRows][kFLanes];
-    int i = i0 > 0 ? i0 - 1 : 0;
-    // registers: E of planes i and i+1 and H of plane i; the E and H loads for
-    // the following plane are issued before the barriers
-    float ex = 0.f, ey = 0.f, ez = 0.f, exn = 0.f, eyn = 0.f, ezn = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
-    if (in_g) {
-        const int64_t c = (int64_t)i * si + cjk;
-        ex = b.ex[c];
-        ey = b.ey[c];
-        ez = b.ez[c];
-        if (i + 1 < nx) {
-            exn = b.ex[c + si];
-            eyn = b.ey[c + si];
-            ezn = b.ez[c + si];
-        }
-        hx = b.hx[c];
-        hy = b.hy[c];
-        hz = b.hz[c];
+    const int k0 = tk * 64, j0 = tj * R, k = k0 + l;
+    const int i0 = ti * chunk, i1 = min(nx, i0 + chunk);
+    const int64_t si = (int64_t)ny * nz;
+    const bool kin = k < nz;
+    const bool kf = k <= nz - 2;
+    const bool zinl = nz >= 4 && (r.flags & WF_RUN3D_SNAP_FROM_STATE) != 0;
+    const bool kbox = zinl ? kin : (k >= 2 && k <= nz - 2);
+    const bool ztile = zinl && (k0 == 0 || k0 + 64 >= nz);
+    const bool kface = kin && (k == 0 || k == nz - 1);
+    // rows q = 0 .. R+1 are j = j0-1+q; byte offsets inside a plane (the host
+    // checks 4*ny*nz < 2^31), kOut outside the grid
+    const int pb = 4 * ny * nz;
+    uint32_t cq[R + 2];
+    bool vq[R + 2];
+#pragma unroll
+    for (int q = 0; q < R + 2; ++q) {
+        const int j = j0 - 1 + q;
+        vq[q] = kin && j >= 0 && j < ny;
+        cq[q] = vq[q] ? 4u * (uint32_t)(j * nz + k) : kOut;
     }
-    float fhy_prev = 0.f, fhz_prev = 0.f;
+    const int jh = j0 - 1 + l;
+    const bool hrow = l <= R && jh >= 0 && jh < ny;
+    const bool lo_ok = hrow && k0 >= 1;
+    const bool hi_ok = hrow && k0 + 64 < nz;
+    const uint32_t c_lo = lo_ok ? 4u * (uint32_t)(jh * nz + k0 - 1) : kOut;
+    const uint32_t c_lo1 = lo_ok ? c_lo + 4u : kOut;
+    const uint32_t c_loj = lo_ok && jh + 1 < ny ? c_lo + 4u * (uint32_t)nz : kOut;
+    const uint32_t c_hi = hi_ok ? 4u * (uint32_t)(jh * nz + k0 + 64) : kOut;
+
+    int i = i0 > 0 ? i0 - 1 : 0;
+    // old E of the current plane (carried from the previous iteration) and the
+    // new Hy, Hz of the previous plane
+    float ex[R + 1], ey[R + 1], ez[R + 1], exT, ezT;
+    float hyp[R + 1], hzp[R + 1];
+    {
+        const int64_t o = (int64_t)i * si;
+        const auto pex = plane_of(b.ex, o, pb), pey = plane_of(b.ey, o, pb), pez = plane_of(b.ez, o, pb);
+#pragma unroll
+        for (int q = 0; q <= R; ++q) {
+            ex[q] = ld(pex, cq[q]);
+            ey[q] = ld(pey, cq[q]);
+            ez[q] = ld(pez, cq[q]);
+            hyp[q] = hzp[q] = 0.f;
+        }
+        exT = ld(pex, cq[R + 1]);
+        ezT = ld(pez, cq[R + 1]);
+    }
     for (; i < i1; ++i) {
-        const int64_t c = (int64_t)i * si + cjk;
-        float exnn = 0.f, eynn = 0.f, eznn = 0.f, hxn = 0.f, hyn = 0.f, hzn = 0.f, ca = 0.f, cb = 0.f;
-        if (kMat && box_jk && i >= i0) {  // used after two barriers: loaded for this plane
-            ca = r.ca[c];
-            cb = r.cb[c];
-        }
-        if (in_g && i + 1 < i1) {
-            if (i + 2 < nx) {
-                exnn = b.ex[c + 2 * si];
-                eynn = b.ey[c + 2 * si];
-                eznn = b.ez[c + 2 * si];
+        const int64_t o = (int64_t)i * si;
+        const bool ifar = i <= nx - 2;
+        const int64_t o1 = ifar ? o + si : o;  // plane i+1 (unused on the last plane)
+        const bool ib = i >= 2 && i <= nx - 2;
+        // this plane's H, coefficients and tile-edge values, the next plane's E
+        float ex1[R + 1], ey1[R + 1], ez1[R + 1], hx[R + 1], hy[R + 1], hz[R + 1], ca[R + 1], cb[R + 1];
+        const auto pex = plane_of(b.ex, o, pb), pey = plane_of(b.ey, o, pb), pez = plane_of(b.ez, o, pb);
+        const auto nex = plane_of(b.ex, o1, pb), ney = plane_of(b.ey, o1, pb), nez = plane_of(b.ez, o1, pb);
+        const auto phx = plane_of(b.hx, o, pb), phy = plane_of(b.hy, o, pb), phz = plane_of(b.hz, o, pb);
+        // loads issue in the order the arithmetic consumes them (the edge values
+        // first), so each wait covers only what the next step needs
+        Edge hc;
+        hc.x = ld(pex, c_lo);
+        hc.y = ld(pey, c_lo);
+        hc.z = ld(pez, c_lo);
+        hc.hx = ld(phx, c_lo);
+        hc.hy = ld(phy, c_lo);
+        hc.y0 = ld(pey, c_lo1);
+        hc.x0 = ld(pex, c_lo1);
+        hc.zj = ld(pez, c_loj);
+        hc.zi = ld(nez, c_lo);
+        hc.ex = ld(pex, c_hi);
+        hc.ey = ld(pey, c_hi);
+#pragma unroll
+        for (int q = 0; q <= R; ++q) {
+            ex1[q] = ld(nex, cq[q]);
+            ey1[q] = ld(ney, cq[q]);
+            ez1[q] = ld(nez, cq[q]);
+            hx[q] = ld(phx, cq[q]);
+            hy[q] = q > 0 ? ld(phy, cq[q]) : 0.f;  // row j0-1 needs no Hy
+            hz[q] = ld(phz, cq[q]);
+            ca[q] = cb[q] = 0.f;
+            if (kMat && q > 0) {
+                const int j = j0 - 1 + q;
+                const uint32_t cc = ib && kbox && j >= 2 && j <= ny - 2 ? cq[q] : kOut;
+                ca[q] = ld(plane_of(r.ca, o, pb), cc);
+                cb[q] = ld(plane_of(r.cb, o, pb), cc);
             }
-            hxn = b.hx[c + si];
-            hyn = b.hy[c + si];
-            hzn = b.hz[c + si];
         }
-        sE[0][row][l] = ex;
-        sE[1][row][l] = ey;
-        sE[2][row][l] = ez;
-        __syncthreads();
-        float fhx = hx, fhy = hy, fhz = hz;
-        if (f_ok) {
-            if (j <= ny - 2 && k <= nz - 2) {
-                const float a = (sE[1][row][l + 1] - ey) / r.dz;
-                const float bb = (sE[2][row + 1][l] - ez) / r.dy;
-                fhx = hx + r.dh * (a - bb);
-            }
-            if (i <= nx - 2 && k <= nz - 2) {
-                const float a = (ezn - ez) / r.dx;
-                const float bb = (sE[0][row][l + 1] - ex) / r.dz;
-                fhy = hy + r.dh * (a - bb);
-            }
-            if (i <= nx - 2 && j <= ny - 2) {
-                const float a = (sE[0][row + 1][l] - ex) / r.dy;
-                const float bb = (eyn - ey) / r.dx;
-                fhz = hz + r.dh * (a - bb);
+        const float exT1 = ld(nex, cq[R + 1]), ezT1 = ld(nez, cq[R + 1]);
+        // new Hx, Hy at (j0-1+l, k0-1) in the spare lanes (Faraday, fdtd3d.py:309-322)
+        const float lhx = jh <= ny - 2 ? hc.hx + r.dh * (((hc.y0 - hc.y) / r.dz) - ((hc.zj - hc.z) / r.dy)) : hc.hx;
+        const float lhy = ifar ? hc.hy + r.dh * (((hc.zi - hc.z) / r.dx) - ((hc.x0 - hc.x) / r.dz)) : hc.hy;
+        // Faraday (fdtd3d.py:309-330): Hx on [:, :-1, :-1], Hy on [:-1, :, :-1], Hz on [:-1, :-1, :]
+        float fhx[R + 1], fhy[R + 1], fhz[R + 1];
+#pragma unroll
+        for (int q = 0; q <= R; ++q) {
+            const int j = j0 - 1 + q;
+            const bool jf = j <= ny - 2;
+            const float eyk = shl_fill(ey[q], lane_of(hc.ey, q));
+            const float ezj = q < R ? ez[q + 1] : ezT;
+            const float exj = q < R ? ex[q + 1] : exT;
+            fhx[q] = jf && kf ? hx[q] + r.dh * (((eyk - ey[q]) / r.dz) - ((ezj - ez[q]) / r.dy)) : hx[q];
+            fhz[q] = ifar && jf ? hz[q] + r.dh * (((exj - ex[q]) / r.dy) - ((ey1[q] - ey[q]) / r.dx)) : hz[q];
+            if (q > 0) {
+                const float exk = shl_fill(ex[q], lane_of(hc.ex, q));
+                fhy[q] = ifar && kf ? hy[q] + r.dh * (((ez1[q] - ez[q]) / r.dx) - ((exk - ex[q]) / r.dz)) : hy[q];
+            } else {
+                fhy[q] = 0.f;
             }
         }
-        sF[0][row][l] = fhx;
-        sF[1][row][l] = fhy;
-        sF[2][row][l] = fhz;
-        __syncthreads();
-        if (zface && i >= 1 && i <= nx - 2) {
-            // Mur on the z faces (boundaries.py:557-575): f[face] = P[in] + cz (f[in] - P[face]),
-            // the inner value from the neighbouring lane, P from the snapshot planes
-            const int64_t ij = (int64_t)i * ny + j;
-            const float *PE = r.pstrip + S.e, *PF = r.pstrip + S.f;
-            const int lin = k == 0 ? l + 1 : l - 1;
-            const int s_in = k == 0 ? 1 : 2, s_face = k == 0 ? 0 : 3;
-            fhx = PE[s_in * xy + ij] + r.cmz * (sF[0][row][lin] - PE[s_face * xy + ij]);
-            fhy = PF[s_in * xy + ij] + r.cmz * (sF[1][row][lin] - PF[s_face * xy + ij]);
-            sF[0][row][l] = fhx;
-            sF[1][row][l] = fhy;
-        }
-        __syncthreads();
-        if (core && i >= i0) {
-            b.nhx[c] = fhx;
-            b.nhy[c] = fhy;
-            b.nhz[c] = fhz;
-            if (box_jk && i >= 2 && i <= nx - 2) {
-                float vx = ex, vy = ey;
-                if (k >= 1) {
-                    const float cx = ((fhz - sF[2][row - 1][l]) / r.dy) - ((fhy - sF[1][row][l - 1]) / r.dz);
-                    const float cy = ((fhx - sF[0][row][l - 1]) / r.dz) - ((fhz - fhz_prev) / r.dx);
-                    vx = kMat ? ca * ex + cb * cx : ex + r.de * cx;
-                    vy = kMat ? ca * ey + cb * cy : ey + r.de * cy;
+        // Mur z faces of Hx, Hy (boundaries.py:557-575): f[face] = P[in] + cz (f[in] - P[face])
+        if (ztile) {
+            const bool iz = i >= 1 && i <= nx - 2;
+#pragma unroll
+            for (int q = 0; q <= R; ++q) {
+                const int j = j0 - 1 + q;
+                const bool zf = iz && kface && j >= 1 && j <= ny - 2;
+                // both shifts run on the whole wavefront before the select: a
+                // DPP move under a divergent branch reads disabled lanes as absent
+                {
+                    const float fn = shl_fill(fhx[q], 0.f), fp = shr_fill(fhx[q], lane_of(lhx, q));
+                    const float pn = shl_fill(hx[q], 0.f), pp = shr_fill(hx[q], lane_of(hc.hx, q));
+                    const float fin = k == 0 ? fn : fp, pin = k == 0 ? pn : pp;
+                    fhx[q] = zf ? pin + r.cmz * (fin - hx[q]) : fhx[q];
                 }
-                const float cz = ((fhy - fhy_prev) / r.dx) - ((fhx - sF[0][row - 1][l]) / r.dy);
-                b.nex[c] = vx;
-                b.ney[c] = vy;
-                b.nez[c] = kMat ? ca * ez + cb * cz : ez + r.de * cz;
+                if (q > 0) {
+                    const float fn = shl_fill(fhy[q], 0.f), fp = shr_fill(fhy[q], lane_of(lhy, q));
+                    const float pn = shl_fill(hy[q], 0.f), pp = shr_fill(hy[q], lane_of(hc.hy, q));
+                    const float fin = k == 0 ? fn : fp, pin = k == 0 ? pn : pp;
+                    fhy[q] = zf ? pin + r.cmz * (fin - hy[q]) : fhy[q];
+                }
             }
         }
-        fhy_prev = fhy;
-        fhz_prev = fhz;
-        ex = exn;
-        ey = eyn;
-        ez = ezn;
-        exn = exnn;
-        eyn = eynn;
-        ezn = eznn;
-        hx = hxn;
-        hy = hyn;
-        hz = hzn;
+        // new H of the core rows, and Ampere (fdtd3d.py:336-359) on the box
+        const bool keep = i >= i0;
+#pragma unroll
+        for (int q = 1; q <= R; ++q) {
+            const int j = j0 - 1 + q;
+            const float hxm = shr_fill(fhx[q], lane_of(lhx, q));  // Hx(k-1)
+            const float hym = shr_fill(fhy[q], lane_of(lhy, q));  // Hy(k-1)
+            const bool box = ib && kbox && vq[q] && j >= 2 && j <= ny - 2;
+            const float cxv = ((fhz[q] - fhz[q - 1]) / r.dy) - ((fhy[q] - hym) / r.dz);
+            const float cyv = ((fhx[q] - hxm) / r.dz) - ((fhz[q] - hzp[q]) / r.dx);
+            const float czv = ((fhy[q] - hyp[q]) / r.dx) - ((fhx[q] - fhx[q - 1]) / r.dy);
+            float vx = ex[q], vy = ey[q];
+            if (k >= 1) {
+                vx = kMat ? ca[q] * ex[q] + cb[q] * cxv : ex[q] + r.de * cxv;
+                vy = kMat ? ca[q] * ey[q] + cb[q] * cyv : ey[q] + r.de * cyv;
+            }
+            const float vz = kMat ? ca[q] * ez[q] + cb[q] * czv : ez[q] + r.de * czv;
+            if (keep && vq[q]) {
+                st(plane_of(b.nhx, o, pb), cq[q], fhx[q]);
+                st(plane_of(b.nhy, o, pb), cq[q], fhy[q]);
+                st(plane_of(b.nhz, o, pb), cq[q], fhz[q]);
+                if (box) {
+                    st(plane_of(b.nex, o, pb), cq[q], vx);
+                    st(plane_of(b.ney, o, pb), cq[q], vy);
+                    st(plane_of(b.nez, o, pb), cq[q], vz);
+                }
+            }
+            hyp[q] = fhy[q];
+            hzp[q] = fhz[q];
+        }
+#pragma unroll
+        for (int q = 0; q <= R; ++q) {
+            ex[q] = ex1[q];
+            ey[q] = ey1[q];
+            ez[q] = ez1[q];
+        }
+        exT = exT1;
+        ezT = ezT1;
     }
 }
 
@@ -415,7 +543,8 @@ __device__ __forceinline__ int shell_count(int n) { return n == 1 ? 1 : 3; }
 __device__ __forceinline__ int shell_index(int t, int n) { return t < 2 ? t : n - 1; }
 
 // E on every cell the fused kernel leaves: the planes i in {0, 1, nx-1}, the
-// rows j in {0, 1, ny-1} of the other planes, and (nz = 1 only) the rest;
+// rows j in {0, 1, ny-1} of the other planes, and the k-slab of the rest
+// unless the fused pass applied the z faces;
 // updated where the reference updates it (from the final new H), copied where
 // it never does.
 template <bool kMat>
@@ -424,7 +553,10 @@ __global__ void __launch_bounds__(256) k3_eshell(wf_run3d r, Bufs b) {
     const int sx = shell_count(nx), sy = shell_count(ny), sz = shell_count(nz);
     const int mx = nx - sx, my = ny - sy;
     const int64_t n1 = (int64_t)sx * ny * nz, n2 = (int64_t)mx * sy * nz;
-    const int64_t n3 = nz == 1 ? (int64_t)mx * my * sz : 0;  // with nz >= 4 the fused kernel does the k-slab
+    // the k-slab {0, 1, nz-1} of the remaining columns, unless the fused pass
+    // applied the z faces itself and so updated every k there
+    const bool zinl = nz >= 4 && (r.flags & WF_RUN3D_SNAP_FROM_STATE) != 0;
+    const int64_t n3 = zinl ? 0 : (int64_t)mx * my * sz;
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int i, j, k;
     if (t < n1) {
@@ -523,9 +655,9 @@ extern "C" int wf_fdtd3d_step(const wf_run3d *runs, int32_t n_runs, int32_t n_st
             const dim3 g3((r.nz + kBX - 1) / kBX, (r.ny + kBY - 1) / kBY, r.nx);
             hipLaunchKernelGGL(k3_h, g3, dim3(kBX, kBY), 0, stream, r);
             const int64_t m1 = 4 * (int64_t)r.ny * r.nz + 2 * (int64_t)r.nx * r.nz;
-            hipLaunchKernelGGL(k3_mur1, dim3((unsigned)((m1 + 255) / 256)), dim3(256), 0, stream, r);
-            const int64_t m2 = 2 * (int64_t)r.nx * r.nz + (r.nz > 1 ? 4 * (int64_t)r.nx * r.ny : 0);
-            hipLaunchKernelGGL(k3_mur2, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, stream, r);
+            hipLaunchKernelGGL(k3_mur1<false>, dim3((unsigned)((m1 + 255) / 256)), dim3(256), 0, stream, r, Prev{});
+            const int64_t m2 = 2 * (int64_t)r.nx * r.nz + 4 * zface_columns<false>(r.nx, r.ny, r.nz);
+            hipLaunchKernelGGL(k3_mur2<false>, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, stream, r, Prev{});
             if (r.ca)
                 hipLaunchKernelGGL(k3_e<true>, g3, dim3(kBX, kBY), 0, stream, r);
             else
@@ -536,6 +668,20 @@ extern "C" int wf_fdtd3d_step(const wf_run3d *runs, int32_t n_runs, int32_t n_st
         if (rc) return rc;
     }
     return WF_OK;
+}
+
+// Resident wavefronts of the device the stream runs on, for sizing i-chunks.
+static int resident_waves() {
+    static int cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256 * 4 * kSWaves;
+    if (!cached[dev]) {
+        int n_cu = 0;
+        if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu < 1)
+            n_cu = 256;
+        cached[dev] = n_cu * 4 * kSWaves;
+    }
+    return cached[dev];
 }
 
 extern "C" int wf_fdtd3d_fused_step(const wf_run3d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream) {
@@ -551,38 +697,54 @@ extern "C" int wf_fdtd3d_fused_step(const wf_run3d *runs, int32_t n_runs, int32_
             wf::set_error("wf_fdtd3d_fused_step: second state buffers missing");
             return WF_ERR_ARG;
         }
+        if (4 * (int64_t)r.ny * r.nz > INT32_MAX) {
+            wf::set_error("wf_fdtd3d_fused_step: an (ny, nz) plane of %d x %d cells exceeds 2^31 bytes", r.ny, r.nz);
+            return WF_ERR_UNSUPPORTED;
+        }
     }
+    const int resident = resident_waves();
     for (int s = 0; s < n_steps; ++s) {
         const bool odd = s & 1;
         for (int bi = 0; bi < n_runs; ++bi) {
             const wf_run3d rd = view_of(runs[bi], odd);   // read state
             const wf_run3d wr = view_of(runs[bi], !odd);  // written state
             const int n = runs[bi].step + s;
-            const Strips S(rd.nx, rd.ny, rd.nz);
-            const unsigned snap_blocks = (unsigned)std::min<int64_t>((S.total + 255) / 256, 4096);
-            hipLaunchKernelGGL(k3_snap, dim3(snap_blocks), dim3(256), 0, stream, rd);
+            const bool from_state = (rd.flags & WF_RUN3D_SNAP_FROM_STATE) != 0;
+            if (!from_state) {
+                const Strips S(rd.nx, rd.ny, rd.nz);
+                const unsigned snap_blocks = (unsigned)std::min<int64_t>((S.total + 255) / 256, 4096);
+                hipLaunchKernelGGL(k3_snap, dim3(snap_blocks), dim3(256), 0, stream, rd);
+            }
             if (rd.n_grp > 0) hipLaunchKernelGGL(k3_src, dim3(1), dim3(256), 0, stream, rd, n);
             Bufs bf{rd.ex, rd.ey, rd.ez, rd.hx, rd.hy, rd.hz, wr.ex, wr.ey, wr.ez, wr.hx, wr.hy, wr.hz};
-            // i-chunks sized so the grid is about one round of resident workgroups
-            const int tiles = ((rd.nz + kFCoreK - 1) / kFCoreK) * ((rd.ny + kFCoreJ - 1) / kFCoreJ);
-            const int n_chunks = std::max(1, std::min((rd.nx + 7) / 8, kResidentBlocks / std::max(1, tiles)));
+            // i-chunks sized so the tiles are about one round of resident wavefronts
+            const int ntk = (rd.nz + 63) / 64, ntj = (rd.ny + kSRows - 1) / kSRows;
+            const int tiles = ntk * ntj;
+            int n_chunks = std::max(1, std::min(std::max(1, rd.nx / 8), resident / std::max(1, tiles)));
             const int chunk = (rd.nx + n_chunks - 1) / n_chunks;
-            const int ntk = (rd.nz + kFCoreK - 1) / kFCoreK, ntj = (rd.ny + kFCoreJ - 1) / kFCoreJ;
-            const int n_tiles = ntk * ntj * ((rd.nx + chunk - 1) / chunk);
-            const dim3 gf(8u * (unsigned)((n_tiles + 7) / 8));
+            n_chunks = (rd.nx + chunk - 1) / chunk;
+            const int n_tiles = tiles * n_chunks;
+            const dim3 gf(8u * (unsigned)(((n_tiles + kSGroup - 1) / kSGroup + 7) / 8)), bs(64 * kSGroup);
             if (rd.ca)
-                hipLaunchKernelGGL(k3_fused<true>, gf, dim3(kFLanes, kFRows), 0, stream, rd, bf, chunk, ntk, ntj,
-                                   n_tiles);
+                hipLaunchKernelGGL((k3_stream<true, kSRows>), gf, bs, 0, stream, rd, bf, chunk, ntk, ntj, n_tiles);
             else
-                hipLaunchKernelGGL(k3_fused<false>, gf, dim3(kFLanes, kFRows), 0, stream, rd, bf, chunk, ntk, ntj,
-                                   n_tiles);
+                hipLaunchKernelGGL((k3_stream<false, kSRows>), gf, bs, 0, stream, rd, bf, chunk, ntk, ntj, n_tiles);
+            const Prev pv{rd.hx, rd.hy, rd.hz};
             const int64_t m1 = 4 * (int64_t)wr.ny * wr.nz + 2 * (int64_t)wr.nx * wr.nz;
-            hipLaunchKernelGGL(k3_mur1, dim3((unsigned)((m1 + 255) / 256)), dim3(256), 0, stream, wr);
-            const int64_t m2 = 2 * (int64_t)wr.nx * wr.nz + (wr.nz > 1 ? 4 * (int64_t)wr.nx * wr.ny : 0);
-            hipLaunchKernelGGL(k3_mur2, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, stream, wr);
+            const unsigned g1 = (unsigned)((m1 + 255) / 256);
+            if (from_state) {
+                hipLaunchKernelGGL(k3_mur1<true>, dim3(g1), dim3(256), 0, stream, wr, pv);
+                const int64_t m2 = 2 * (int64_t)wr.nx * wr.nz + 4 * zface_columns<true>(wr.nx, wr.ny, wr.nz);
+                hipLaunchKernelGGL(k3_mur2<true>, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, stream, wr, pv);
+            } else {
+                hipLaunchKernelGGL(k3_mur1<false>, dim3(g1), dim3(256), 0, stream, wr, pv);
+                const int64_t m2 = 2 * (int64_t)wr.nx * wr.nz + 4 * zface_columns<false>(wr.nx, wr.ny, wr.nz);
+                hipLaunchKernelGGL(k3_mur2<false>, dim3((unsigned)((m2 + 255) / 256)), dim3(256), 0, stream, wr, pv);
+            }
+            const bool zinl = from_state && wr.nz >= 4;
             const int sx = wr.nx == 1 ? 1 : 3, sy = wr.ny == 1 ? 1 : 3, sz = wr.nz == 1 ? 1 : 3;
             const int64_t nsh = (int64_t)sx * wr.ny * wr.nz + (int64_t)(wr.nx - sx) * sy * wr.nz +
-                                (wr.nz == 1 ? (int64_t)(wr.nx - sx) * (wr.ny - sy) * sz : 0);
+                                (zinl ? 0 : (int64_t)(wr.nx - sx) * (wr.ny - sy) * sz);
             if (rd.ca)
                 hipLaunchKernelGGL(k3_eshell<true>, dim3((unsigned)((nsh + 255) / 256)), dim3(256), 0, stream, rd, bf);
             else
