@@ -21,9 +21,11 @@
 //   1. soft sources inside the region, in collection order, added to the
 //      owning lane's registers (src/core/sources.py:1060-1071); LDS keeps the
 //      pre-injection Hz, which is the Mur snapshot (src/core/boundaries.py:237);
-//   2. Faraday on interior cells (src/core/fdtd2d.py:232-235); Mur edge cells
-//      (src/core/boundaries.py:259-274) are written to LDS by the interior cell
-//      whose Faraday value they need; edge lanes reload them after the barrier;
+//   2. Faraday on interior cells (src/core/fdtd2d.py:232-235); on tiles that
+//      touch the grid boundary, a short pass after it (one thread per face cell,
+//      from the Faraday values and the snapshot rows and columns next to the
+//      boundary kept in LDS) writes the Mur faces (src/core/boundaries.py:259-274),
+//      and the face lanes reload them; interior tiles skip both;
 //   3. Ampere (src/core/fdtd2d.py:244-257); Hz detectors owned by the core are
 //      read from LDS (examples/brain_mimo_imaging.py:196-198).
 //
@@ -103,6 +105,8 @@ k_tb(wf_tb_args a) {
     __shared__ int s_ngrp, s_ndet, s_ey_src;
     __shared__ unsigned int s_rowmask[4];  // per wavefront: bit k set if row w+4k holds a source
     __shared__ unsigned int s_max[3];
+    __shared__ float s_prow[2][kW];  // edge tiles: snapshot of rows i = 1 and i = nx-2
+    __shared__ float s_pcol[2][RI];  // edge tiles: snapshot of columns j = 1 and j = ny-2
 
     const int tid = threadIdx.x;
     const int lane = tid & (kW - 1);
@@ -241,8 +245,28 @@ k_tb(wf_tb_args a) {
             }
             if (ey_src) __syncthreads();
         }
-        // 2. Faraday + Mur (owner writes through LDS). Row conditions are
-        // wavefront-uniform; per-lane conditions become selects.
+        // 2. Faraday. Row conditions are wavefront-uniform; per-lane conditions
+        // become selects. Edge tiles first keep the snapshot of the rows and
+        // columns next to the boundary (Faraday overwrites them in LDS), which
+        // the Mur pass below reads.
+        if (edge) {
+            // the edge passes' values are formed inside their branches (the
+            // empty asm keeps the compiler from hoisting them out of the step
+            // loop, where they would hold scalar registers for every tile)
+            int e_i0 = gi0, e_j0 = gj0;
+            asm volatile("" : "+s"(e_i0), "+s"(e_j0));
+            const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
+            const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;  // region lanes of j = 1, ny-2
+            if (tid < kW) {
+                if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * kW + tid];
+                if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * kW + tid];
+            } else if (tid >= 2 * kW && tid < 2 * kW + RI) {
+                const int li = tid - 2 * kW;
+                if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
+                if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int li = w + 4 * k;
@@ -256,56 +280,61 @@ k_tb(wf_tb_args a) {
                 const float F = hz[k] + dh * (av - bv);
                 if (gi >= 1 && gi <= nx - 2) {
                     const bool lane_int = gj >= 1 && gj <= ny - 2;
-                    if (edge && lane_int) {
-                        const float p_own = s_hz[c];
-                        if (gi == 1 && li >= 1) {
-                            const float p0 = s_hz[c - kW];
-                            const float L = p_own + cx * (F - p0);
-                            if (gj == 1 && lane >= 1) {
-                                const float p00 = s_hz[c - kW - 1];
-                                s_hz[c - kW - 1] = p0 + cy * (L - p00);
-                            }
-                            if (gj == ny - 2 && lane + 1 < kW) {
-                                const float p01 = s_hz[c - kW + 1];
-                                s_hz[c - kW + 1] = p0 + cy * (L - p01);
-                            }
-                            s_hz[c - kW] = L;
-                        }
-                        if (gi == nx - 2 && li + 1 < RI) {
-                            const float p1 = s_hz[c + kW];
-                            const float Rv = p_own + cx * (F - p1);
-                            if (gj == 1 && lane >= 1) {
-                                const float p10 = s_hz[c + kW - 1];
-                                s_hz[c + kW - 1] = p1 + cy * (Rv - p10);
-                            }
-                            if (gj == ny - 2 && lane + 1 < kW) {
-                                const float p11 = s_hz[c + kW + 1];
-                                s_hz[c + kW + 1] = p1 + cy * (Rv - p11);
-                            }
-                            s_hz[c + kW] = Rv;
-                        }
-                        if (gj == 1 && lane >= 1) {
-                            const float pl = s_hz[c - 1];
-                            s_hz[c - 1] = p_own + cy * (F - pl);
-                        }
-                        if (gj == ny - 2 && lane + 1 < kW) {
-                            const float pr = s_hz[c + 1];
-                            s_hz[c + 1] = p_own + cy * (F - pr);
-                        }
-                    }
                     hz[k] = lane_int ? F : hz[k];
                     if (lane_int) s_hz[c] = F;
                 }
             }
         }
         __syncthreads();
-        if (edge) {  // Mur values written by their owners
+        if (edge) {
+            // Mur (src/core/boundaries.py:259-274) on the faces next to this
+            // step's Faraday rows [s-1, RI-s): f[face] = P[in] + C (f[in] - P[face]),
+            // x faces first; a corner takes the y-face formula over the x-face
+            // value beside it. One thread per face cell; every value it reads
+            // is Faraday's or the snapshot's, so the pass needs no barrier.
+            int e_i0 = gi0, e_j0 = gj0;
+            asm volatile("" : "+s"(e_i0), "+s"(e_j0));
+            const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;
+            const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;
+            if (tid < 2 * kW) {  // x faces: rows i = 0 (from i = 1) and i = nx-1 (from i = nx-2)
+                const int side = tid >> 6, l = tid & (kW - 1);
+                const int rin = side ? r2 : r1, rf = side ? rin + 1 : rin - 1;
+                const int gjl = e_j0 + l;
+                if (rin >= s - 1 && rin < RI - s && rf >= 0 && rf < RI && gjl >= 1 && gjl <= ny - 2) {
+                    const float p_in = s_prow[side][l], p_f = s_hz[rf * kW + l];
+                    const float L = p_in + cx * (s_hz[rin * kW + l] - p_f);
+                    if (gjl == 1 && l >= 1) {
+                        const float pc = s_hz[rf * kW + l - 1];
+                        s_hz[rf * kW + l - 1] = p_f + cy * (L - pc);
+                    }
+                    if (gjl == ny - 2 && l + 1 < kW) {
+                        const float pc = s_hz[rf * kW + l + 1];
+                        s_hz[rf * kW + l + 1] = p_f + cy * (L - pc);
+                    }
+                    s_hz[rf * kW + l] = L;
+                }
+            } else if (tid - 2 * kW < RI) {  // y faces: j = 0 (from j = 1) and j = ny-1 (from j = ny-2)
+                const int li = tid - 2 * kW, gil = e_i0 + li;
+                if (li >= s - 1 && li < RI - s && gil >= 1 && gil <= nx - 2) {
+                    if (c1 >= 1 && c1 < kW) {
+                        const float pf = s_hz[li * kW + c1 - 1];
+                        s_hz[li * kW + c1 - 1] = s_pcol[0][li] + cy * (s_hz[li * kW + c1] - pf);
+                    }
+                    if (c2 >= 0 && c2 + 1 < kW) {
+                        const float pf = s_hz[li * kW + c2 + 1];
+                        s_hz[li * kW + c2 + 1] = s_pcol[1][li] + cy * (s_hz[li * kW + c2] - pf);
+                    }
+                }
+            }
+            __syncthreads();
+            // face cells take their Mur values. LDS equals the registers on
+            // every other cell of the rows that stay valid (Faraday wrote its
+            // values to both; a source added outside them lands only on faces,
+            // which Mur overwrites), so whole rows are reloaded.
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 const int li = w + 4 * k;
-                const int gi = gi0 + li;
-                if (li < RI && gi >= 0 && gi < nx && col_in && (gi == 0 || gi == nx - 1 || gj == 0 || gj == ny - 1))
-                    hz[k] = s_hz[li * kW + lane];
+                if (li < RI) hz[k] = s_hz[li * kW + lane];
             }
         }
         // 3. Ampere + detectors.
