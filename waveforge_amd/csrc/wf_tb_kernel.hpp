@@ -214,10 +214,14 @@ k_tb(wf_tb_args a) {
     const Divider divx{R.dx, a.rdx, a.div_lo_x, a.div_span_x};
     const Divider divy{R.dy, a.rdy, a.div_lo_y, a.div_span_y};
     (void)de;
+    // the step index, read once: the kernel's own stores (traces) could alias the
+    // descriptor as far as the compiler knows, so a read inside the loop would
+    // be a global load and wait on every step
+    const int n0 = __builtin_amdgcn_readfirstlane(R.step + a.step_offset);
 
 #pragma unroll 1
     for (int s = 1; s <= D; ++s) {
-        const int n = R.step + a.step_offset + s - 1;
+        const int n = n0 + s - 1;
         // 1. Soft sources into the owning lane's registers (rows that hold a
         // source are flagged per wavefront, so the scan is skipped elsewhere).
         if (ngrp > 0) {
