@@ -222,12 +222,19 @@ k_tb(wf_tb_args a) {
 #pragma unroll 1
     for (int s = 1; s <= D; ++s) {
         const int n = n0 + s - 1;
+        // row conditions below are formed from this per-step copy rather than
+        // hoisted out of the step loop as one held mask per row
+        int gib = gi0;
+        asm volatile("" : "+s"(gib));
         // 1. Soft sources into the owning lane's registers (rows that hold a
         // source are flagged per wavefront, so the scan is skipped elsewhere).
         if (ngrp > 0) {
+            // tested here, not hoisted out of the step loop as one held mask per row
+            unsigned int rm = __builtin_amdgcn_readfirstlane(rowmask);  // one per wavefront
+            asm volatile("" : "+s"(rm));
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                if (rowmask & (1u << k)) {
+                if (rm & (1u << k)) {
                     const int li = w + 4 * k;
                     for (int q = 0; q < ngrp; ++q) {
                         const int cl = s_grp_cell[q];
@@ -275,7 +282,7 @@ k_tb(wf_tb_args a) {
         for (int k = 0; k < NR; ++k) {
             const int li = w + 4 * k;
             if (li >= s - 1 && li < RI - s) {
-                const int gi = gi0 + li;
+                const int gi = gib + li;
                 const float ex_next = lane_next(ex[k]);
                 const int c = li * kW + lane;
                 const float ey_down = s_ey[c + kW];
@@ -350,7 +357,7 @@ k_tb(wf_tb_args a) {
         for (int k = 0; k < NR; ++k) {
             const int li = w + 4 * k;
             if (li >= s && li < RI - s) {
-                const int gi = gi0 + li;
+                const int gi = gib + li;
                 const float hz_left = lane_prev(hz[k]);
                 const int c = li * kW + lane;
                 const float hz_up = s_hz[c - kW];
