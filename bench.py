@@ -59,6 +59,7 @@ def parse():
     p.add_argument("--serving-requests", type=int, default=48)
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
     p.add_argument("--no-3d", action="store_true", help="skip the 3D section")
+    p.add_argument("--no-das", action="store_true", help="skip the DAS and reference-CPU-case sections")
     p.add_argument("--size3d", type=int, default=256)
     p.add_argument("--steps3d", type=int, default=200)
     p.add_argument("--fd-cells", type=int, default=32, help="seeded cells of the central-difference check")
@@ -88,33 +89,150 @@ def copy_probe() -> int:
     return 0
 
 
-def cpu_baseline(wl, n_steps: int) -> dict:
-    """Time the oracle (torch CPU restatement of the reference step) on a sample."""
+def cpu_baseline(wl, n_steps: int, slab=None, das=None) -> dict:
+    """Time the oracle (torch CPU restatement of the reference step) on a sample
+    of the headline run with every host core, and (``single_thread``) on a
+    shorter sample with one. With ``slab`` (the reference's own CPU case, 256^2 slab,
+    1000 steps) the oracle runs it in full with all cores and with one; with
+    ``das`` (synthetic traces of a 384^2 16-element array) it times the NumPy
+    restatement of the examples' backprojection onto the full grid and of the
+    dataset generator's envelope DAS onto 40 x 40."""
+    import numpy as np
     import torch
 
+    from oracle.das_env_ref import compute_das_image
+    from oracle.das_ref import das_backprojection as das_np
     from oracle.fdtd2d_ref import OracleFDTD2D, OracleSource
 
     cores = os.cpu_count() or 1
-    torch.set_num_threads(cores)
+
+    def oracle_for(w):
+        g = w.grid
+        wave = w.sim._sources._sources[0]._waveform_tensor.cpu()
+        i, j = w.source_cell
+        return OracleFDTD2D(g.Nx, g.Ny, g.dx, g.dy, g.dt, Ca=w.Ca.cpu(), Cb=w.Cb.cpu(),
+                            sources=[OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]),
+                                                  torch.tensor([1.0]))],
+                            detectors=w.detectors.cells)
+
+    def rate(w, steps, warm, threads):
+        torch.set_num_threads(threads)
+        o = oracle_for(w)
+        o.run(warm)
+        t0 = time.perf_counter()
+        o.run(steps)
+        el = time.perf_counter() - t0
+        return steps * w.grid.Nx * w.grid.Ny / el / 1e6, el
+
     g = wl.grid
-    src = wl.sim._sources._sources[0]
-    wave = src._waveform_tensor.cpu()
-    i, j = wl.source_cell
-    o = OracleFDTD2D(g.Nx, g.Ny, g.dx, g.dy, g.dt, Ca=wl.Ca.cpu(), Cb=wl.Cb.cpu(),
-                     sources=[OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0]))],
-                     detectors=wl.detectors.cells)
-    o.run(2)
-    t0 = time.perf_counter()
-    o.run(n_steps)
-    el = time.perf_counter() - t0
-    return {
-        "value": n_steps * g.Nx * g.Ny / el / 1e6,
+    value, _ = rate(wl, n_steps, 2, cores)
+    single, _ = rate(wl, 2, 1, 1)
+    out = {
+        "value": value,
         "unit": "Mcell-updates/s",
-        "cores": torch.get_num_threads(),
+        "cores": cores,
         "kind": "port",
         "sample": f"{g.Nx}x{g.Ny} blob medium, steps 3-{2 + n_steps} ({n_steps} timed steps) of the same run, "
-                  f"oracle torch-CPU restatement, {torch.get_num_threads()} threads",
+                  f"oracle torch-CPU restatement, {cores} threads",
+        "single_thread": {"value": round(single, 2), "unit": "Mcell-updates/s", "cores": 1,
+                          "sample": f"{g.Nx}x{g.Ny} blob medium, steps 2-3 (2 timed steps), 1 thread"},
     }
+    if slab is not None:
+        r_all, s_all = rate(slab, slab.n_steps, 0, cores)
+        r_one, s_one = rate(slab, slab.n_steps, 0, 1)
+        out["slab256"] = {"workload": slab.description + " (the reference's CPU case), in full",
+                     "all_cores": {"threads": cores, "seconds": round(s_all, 3), "mcell_updates_per_s": round(r_all, 1)},
+                     "one_core": {"threads": 1, "seconds": round(s_one, 3), "mcell_updates_per_s": round(r_one, 1)}}
+    torch.set_num_threads(cores)
+    if das is not None:
+        S, ants, n, dx, dt, geom = das["S"], das["ants"], das["n"], das["dx"], das["dt"], das["geom"]
+        t0 = time.perf_counter()
+        das_np(S, ants, n, n, dx, dt, S.shape[2])
+        t_bp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        compute_das_image(S, ants, n, n, dx, dt, S.shape[2], 40, geom)
+        t_env = time.perf_counter() - t0
+        gathers = n * n * S.shape[0] * S.shape[1]
+        out["das"] = {"sample": f"one {n}x{n} image of {S.shape[0]}x{S.shape[1]}x{S.shape[2]} synthetic traces, "
+                                "NumPy restatements",
+                      "backprojection_ms": round(t_bp * 1e3, 1),
+                      "backprojection_gather_adds_per_s": round(gathers / t_bp, 1),
+                      "envelope_40x40_ms": round(t_env * 1e3, 1)}
+    return out
+
+
+def das_case():
+    """Synthetic traces (seeded normal, 16 TX x 16 RX x 2000 samples) for the
+    16-element array of a 384^2 head phantom, with the phantom's layered-medium
+    geometry; DAS timing does not depend on the trace values."""
+    import numpy as np
+
+    from waveforge_amd import dataset, workloads
+
+    n = 384
+    ph = workloads.head_phantom(n, 16, seed=3)
+    S = np.random.default_rng(5).standard_normal((16, 16, 2000)).astype(np.float32)
+    c = (n - 1) / 2.0
+    geom = {"cx_cells": c, "cy_cells": c, "layers": dataset.head_layers(n)}
+    return {"S": S, "ants": ph.antennas, "n": n, "dx": 1e-3, "dt": workloads.imaging_dt(), "geom": geom}
+
+
+def das_suite(dev, case) -> dict:
+    """Both DAS flavours on the GPU for one 384^2 16-element array (16 x 16 x
+    2000 traces): the examples' per-pixel backprojection onto the full grid
+    (``core.das_backprojection``, fp64 accumulation; H*W*N_tx*N_rx gather-adds
+    per image) and the dataset generator's envelope DAS onto a 40 x 40 image
+    with layered-medium delays (``antenna.EnvelopeDAS``: Hilbert envelope of
+    every trace, then the delay-and-sum). Each is timed per call, traces
+    already on the device, median of 10 calls after a warm-up."""
+    import torch
+
+    from waveforge_amd import core
+    from waveforge_amd.antenna import EnvelopeDAS
+
+    n, dt, dx = case["n"], case["dt"], case["dx"]
+    S = torch.from_numpy(case["S"]).to(dev)
+    g = core.YeeGrid(n, n, dx, dx, courant=0.99, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(10):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[len(ts) // 2]
+
+    t_bp = timed(lambda: core.das_backprojection(S, case["ants"], g, S.shape[2], device=dev, return_tensor=True))
+    env = EnvelopeDAS(case["ants"], (n, n), dx, dt, 40, case["geom"], device=dev)
+    t_env = timed(lambda: env(S))
+    gathers = n * n * S.shape[0] * S.shape[1]
+    return {
+        "workload": f"{n}x{n} grid, 16-element array, 16 x 16 x {S.shape[2]} synthetic traces on the device",
+        "backprojection": {"image": [n, n], "ms_per_image": round(t_bp * 1e3, 3),
+                           "gather_adds_per_s": round(gathers / t_bp, 1)},
+        "envelope_das": {"image": [40, 40], "ms_per_image": round(t_env * 1e3, 3)},
+    }
+
+
+def slab_suite(dev, slab) -> dict:
+    """The reference's own CPU case on the GPU: FDTD2D.run over the 256^2 slab
+    for all 1000 steps (after one untimed run of a twin), wall time with the
+    stability checks every 100 steps included."""
+    import torch
+
+    from waveforge_amd import workloads
+
+    workloads.slab_256(dev).sim.run(slab.n_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    slab.sim.run(slab.n_steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": slab.description, "seconds": round(el, 4),
+            "mcell_updates_per_s": round(slab.n_steps * slab.grid.Nx * slab.grid.Ny / el / 1e6, 1)}
 
 
 def imaging_suite(dev) -> dict:
@@ -471,10 +589,15 @@ def main() -> int:
             roofline["traffic_source"] += "copy-kernel calibrated, per depth-%d launch)" % depth
     tr = wl.detectors.traces
     finite = bool(torch.isfinite(tr).all()) and bool(torch.isfinite(sim.fields.Hz).all())
-    base = None
+    base = das_res = slab_res = None
+    dcase = das_case()
+    if not args.no_das:
+        print("[bench] DAS and reference-CPU-case sections", file=sys.stderr, flush=True)
+        das_res = das_suite(dev, dcase)
+        slab_res = slab_suite(dev, workloads.slab_256(dev))
     if not args.no_cpu_baseline:
         print(f"[bench] CPU baseline: {args.cpu_steps} oracle steps", file=sys.stderr, flush=True)
-        base = cpu_baseline(wl, args.cpu_steps)
+        base = cpu_baseline(wl, args.cpu_steps, slab=workloads.slab_256(dev), das=dcase)
     imaging = serving_res = adjoint = fd3 = None
     if not args.no_imaging:
         print("[bench] batched imaging section", file=sys.stderr, flush=True)
@@ -514,6 +637,8 @@ def main() -> int:
                      "rates_mcell_per_s": tuned.rates},
         "cpu_baseline": base,
         "fields_finite": finite,
+        "das": das_res,
+        "slab256": slab_res,
         "imaging": imaging,
         "serving": serving_res,
         "adjoint": adjoint,
