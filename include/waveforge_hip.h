@@ -137,6 +137,12 @@ typedef struct wf_tb_args {
     int32_t pad;
     const int32_t *stop;       /* NULL or device n_runs flags: a run whose flag is set is
                                   copied through unchanged (it failed a stability check) */
+    int64_t hist_stride;       /* 0, or floats between consecutive state slots: the launch
+                                  then writes the state after each of its `depth` steps s
+                                  to the written state + (s-1)*hist_stride instead of only
+                                  the last one (materials, IEEE division, tile_i 32, no
+                                  maxabs_out or stop); the adjoint's recompute of a
+                                  segment's state history */
 } wf_tb_args;
 
 int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
@@ -284,7 +290,9 @@ int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, h
  * right/left, Faraday, snapshot) leaving the adjoint of u_n in (lex, ley, lhz),
  * and accumulates dL/dCa and dL/dCb (fp64) per run. ex_prev/ey_prev are the
  * fields before step n, hz_next the Hz after it. Sources are affine and drop
- * out. No reference code counterpart: the reference's differentiable-FDTD
+ * out. With n_steps > 1 the call runs steps step, step-1, ..., step-n_steps+1
+ * in turn, the state pointers moving back by hist_stride floats per step (a
+ * segment of a state history laid out slot after slot). No reference code counterpart: the reference's differentiable-FDTD
  * plan is design prose (docs/architecture/DIFFERENTIABLE_PHYSICS.md:33); the
  * forward it differentiates is FDTD2D.step (src/core/fdtd2d.py:187-266). */
 typedef struct wf_adj_args {
@@ -292,14 +300,15 @@ typedef struct wf_adj_args {
     const float *ex_prev, *ey_prev;  /* run r at r*state_stride */
     const float *hz_next;
     float *lex, *ley, *lhz;          /* adjoint state, run r at r*run_stride */
-    float *lex2, *ley2, *lhz2, *lp;  /* scratch, same layout; lp zero on entry and exit */
+    float *lex2, *ley2, *lhz2, *lp;  /* scratch, same layout (lp is no longer written; kept for the layout) */
     double *gca, *gcb;               /* gradient accumulators, run r at r*run_stride */
     const float *dl_dd;              /* dL/d trace[r][k][n] at r*dd_run_stride + k*dd_rx_stride + n */
     const int32_t *rx_cell;          /* n_rx receiver cells */
     int64_t run_stride, state_stride, dd_run_stride, dd_rx_stride;
     int32_t nx, ny, n_runs, n_rx, step;
     float dh, dx, dy, cmx, cmy;
-    int32_t pad;
+    int32_t n_steps;                 /* adjoint steps in this call; 0 or 1: one */
+    int64_t hist_stride;             /* floats between the state slots of consecutive steps */
 } wf_adj_args;
 
 int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream);

@@ -75,7 +75,7 @@ int main(void) {
   O(wf_tb_args, tile_j) O(wf_tb_args, depth) O(wf_tb_args, parity) O(wf_tb_args, step_offset)
   O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out) O(wf_tb_args, rdx) O(wf_tb_args, rdy)
   O(wf_tb_args, div_lo_x) O(wf_tb_args, div_span_x) O(wf_tb_args, div_lo_y) O(wf_tb_args, div_span_y)
-  O(wf_tb_args, fast_div) O(wf_tb_args, pad) O(wf_tb_args, stop)
+  O(wf_tb_args, fast_div) O(wf_tb_args, pad) O(wf_tb_args, stop) O(wf_tb_args, hist_stride)
   O(wf_das2d_args, traces) O(wf_das2d_args, ant_x) O(wf_das2d_args, ant_y) O(wf_das2d_args, image)
   O(wf_das2d_args, n_tx) O(wf_das2d_args, n_rx) O(wf_das2d_args, n_t) O(wf_das2d_args, nx_img)
   O(wf_das2d_args, ny_img) O(wf_das2d_args, gate_start) O(wf_das2d_args, clip) O(wf_das2d_args, pad)
@@ -83,7 +83,7 @@ int main(void) {
   printf("wf_adj_args size %zu\n", sizeof(wf_adj_args));
   O(wf_adj_args, ca) O(wf_adj_args, hz_next) O(wf_adj_args, lp) O(wf_adj_args, gcb) O(wf_adj_args, dl_dd)
   O(wf_adj_args, rx_cell) O(wf_adj_args, run_stride) O(wf_adj_args, dd_rx_stride) O(wf_adj_args, nx)
-  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, pad)
+  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, n_steps) O(wf_adj_args, hist_stride)
   printf("wf_inj2d_f64 size %zu\nwf_run2d_f64 size %zu\nwf_f64_args size %zu\n",
          sizeof(wf_inj2d_f64), sizeof(wf_run2d_f64), sizeof(wf_f64_args));
   O(wf_inj2d_f64, wave) O(wf_inj2d_f64, cell) O(wf_inj2d_f64, amp)
@@ -107,10 +107,11 @@ def test_struct_layout_matches_header(tmp_path):
     src.write_text(LAYOUT_C)
     exe = tmp_path / "layout"
     try:
-        subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", "-I/opt/rocm/include",
-                        str(src), "-o", str(exe)], check=True, capture_output=True)
-    except (FileNotFoundError, subprocess.CalledProcessError) as exc:
-        pytest.skip(f"cannot compile layout probe: {exc}")
+        res = subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}", "-I/opt/rocm/include",
+                              str(src), "-o", str(exe)], capture_output=True, text=True)
+    except FileNotFoundError as exc:
+        pytest.skip(f"no C compiler for the layout probe: {exc}")
+    assert res.returncode == 0, f"layout probe does not compile against the header:\n{res.stderr}"
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     types = {"wf_inj2d": _native.WfInj2d, "wf_run2d": _native.WfRun2d, "wf_tb_args": _native.WfTbArgs,
              "wf_das2d_args": _native.WfDas2dArgs, "wf_adj_args": _native.WfAdjArgs,

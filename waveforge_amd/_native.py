@@ -93,6 +93,7 @@ class WfTbArgs(ctypes.Structure):
         ("div_lo_y", ctypes.c_uint32), ("div_span_y", ctypes.c_uint32),
         ("fast_div", _i32), ("pad", _i32),
         ("stop", _vp),
+        ("hist_stride", ctypes.c_int64),
     ]
 
 
@@ -116,7 +117,7 @@ class WfAdjArgs(ctypes.Structure):
                                    "ley2", "lhz2", "lp", "gca", "gcb", "dl_dd", "rx_cell")] + \
         [(n, _i64) for n in ("run_stride", "state_stride", "dd_run_stride", "dd_rx_stride")] + \
         [(n, _i32) for n in ("nx", "ny", "n_runs", "n_rx", "step")] + \
-        [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("pad", _i32)]
+        [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("n_steps", _i32), ("hist_stride", _i64)]
 
 
 class WfInj2dF64(ctypes.Structure):

@@ -103,9 +103,10 @@ class TraceProblem:
     # ------------------------------------------------------------------
     def _history(self, ca, cb):
         """State history buffer ``H[k] = u_{s0+k}`` (k = 0..K) for one segment
-        and the run table that makes depth-1 launch k read H[k] and write
-        H[k+1] (descriptor step = k; the launch's step_offset adds s0). Built
-        once per backward: descriptors differ only in their state pointers."""
+        and the run table whose entry k makes a launch read H[k] and write
+        H[k+1], H[k+2], ... one slot per step (descriptor step = k; the launch's
+        step_offset adds s0). Built once per backward: descriptors differ only
+        in their state pointers."""
         n_tx, nx, ny, K = len(self.tx), self.nx, self.ny, self.checkpoint
         H = torch.empty((K + 1, n_tx, 3, nx, ny), dtype=torch.float32, device=self.device)
         base = []
@@ -127,30 +128,45 @@ class TraceProblem:
             ptr[:, :, getattr(WfRun2d, f_hi).offset // 8] = src + np.uint64(3 * n_tx * cell_bytes)
         tbl.view(np.int32)[:, :, WfRun2d.step.offset // 4] = np.arange(K, dtype=np.int32)[:, None]
         table = torch.from_numpy(np.ascontiguousarray(tbl).reshape(-1)).to(self.device)
-        nt = tile_count(nx, ny, self.recompute_tile_rows, 1)
-        tiles = np.stack([np.repeat(np.arange(n_tx, dtype=np.int32), nt),
-                          np.tile(np.arange(nt, dtype=np.int32), n_tx)], axis=1)
-        tiles_t = torch.from_numpy(np.ascontiguousarray(tiles)).to(self.device)
-        return H, table, tiles_t
+        return H, table
 
-    recompute_tile_rows = 32
+    recompute_tile_rows = 32  # the state-history launches exist for 32-row tiles
+    recompute_depth = 6       # steps per recompute launch (each step's state is written)
+
+    def _tile_list(self, depth: int) -> torch.Tensor:
+        """(run, tile) pairs of every run for ``depth``-step launches (cached)."""
+        cache = self.__dict__.setdefault("_tiles_by_depth", {})
+        if depth not in cache:
+            n_tx = len(self.tx)
+            nt = tile_count(self.nx, self.ny, self.recompute_tile_rows, depth)
+            tiles = np.stack([np.repeat(np.arange(n_tx, dtype=np.int32), nt),
+                              np.tile(np.arange(nt, dtype=np.int32), n_tx)], axis=1)
+            cache[depth] = torch.from_numpy(np.ascontiguousarray(tiles)).to(self.device)
+        return cache[depth]
 
     def _recompute(self, hist, start: torch.Tensor, s0: int, L: int) -> torch.Tensor:
-        """Fill ``H[0..L]`` with ``u_{s0}..u_{s0+L}`` from checkpoint ``start``."""
-        H, table, tiles_t = hist
-        n_tx = len(self.tx)
+        """Fill ``H[0..L]`` with ``u_{s0}..u_{s0+L}`` from checkpoint ``start``:
+        temporally blocked launches of up to ``recompute_depth`` steps, each
+        writing the state after every one of its steps into its slot."""
+        H, table = hist
+        n_tx, nx, ny = len(self.tx), self.nx, self.ny
         H[0].copy_(start)
         lib = _native.load()
         stream = _native.stream_handle(self.device)
         sz = ctypes.sizeof(WfRun2d)
         a = WfTbArgs()
-        a.tiles = tiles_t.data_ptr()
-        a.n_runs, a.n_tiles = n_tx, int(tiles_t.shape[0])
-        a.tile_i, a.tile_j = self.recompute_tile_rows, _native.tile_cols(1)
-        a.depth, a.parity, a.step_offset, a.has_materials = 1, 0, s0, 1
-        for k in range(L):
+        a.n_runs, a.tile_i = n_tx, self.recompute_tile_rows
+        a.parity, a.step_offset, a.has_materials = 0, s0, 1
+        a.hist_stride = n_tx * 3 * nx * ny  # floats from one slot to the next
+        k = 0
+        while k < L:
+            d = min(self.recompute_depth, L - k)
+            tiles_t = self._tile_list(d)
+            a.tiles, a.n_tiles = tiles_t.data_ptr(), int(tiles_t.shape[0])
+            a.depth, a.tile_j = d, _native.tile_cols(d)
             a.runs = table.data_ptr() + k * n_tx * sz
             _native.check(lib.wf_fdtd2d_tb_advance(ctypes.byref(a), ctypes.c_void_p(stream)), "recompute")
+            k += d
         return H
 
     def backward(self, ca: torch.Tensor, cb: torch.Tensor, ckpts, dl_dd: torch.Tensor):
@@ -184,12 +200,13 @@ class TraceProblem:
             s0 = seg * K
             L = min(K, self.n_steps - s0)
             self._recompute(hist, ckpts[seg], s0, L)
-            for k in range(L - 1, -1, -1):
-                a.step = s0 + k
-                a.ex_prev = h0 + k * step_bytes
-                a.ey_prev = h0 + k * step_bytes + nx * ny * 4
-                a.hz_next = h0 + (k + 1) * step_bytes + 2 * nx * ny * 4
-                _native.check(lib.wf_adjoint_step(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_adjoint_step")
+            # the segment's L adjoint steps in one call, from step s0+L-1 back to s0
+            k = L - 1
+            a.step, a.n_steps, a.hist_stride = s0 + k, L, step_bytes // 4
+            a.ex_prev = h0 + k * step_bytes
+            a.ey_prev = h0 + k * step_bytes + nx * ny * 4
+            a.hz_next = h0 + (k + 1) * step_bytes + 2 * nx * ny * 4
+            _native.check(lib.wf_adjoint_step(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_adjoint_step")
         del hist, H
         return gca.sum(0), gcb.sum(0)
 

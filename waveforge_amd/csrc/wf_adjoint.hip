@@ -13,10 +13,8 @@
 //   k_adj_record  lHz[rx] += dL/dd_n[rx]
 //   k_adj_e       transposed Ampere (S9, S8) + coefficient gradients;
 //                 lHz gathers the four Ampere neighbours (out of place)
-//   k_adj_mur_c   transposed Mur top then bottom (S7, S6), one thread per row i
-//   k_adj_mur_r   transposed Mur right then left (S5, S4), one thread per column j
-//   k_adj_h       transposed Faraday (S3) gathered per cell, plus the
-//                 snapshot's transpose lHz += lP (S1); lP is cleared
+//   k_adj_h       transposed Mur (S7..S4) and Faraday (S3) gathered per
+//                 cell, plus the snapshot's transpose lHz += lP (S1)
 //
 // All runs of a batch share the grid size and Ca/Cb (one phantom, many TX):
 // blockIdx.z is the run, and run r's arrays start at r * run_stride. Gradients
@@ -77,47 +75,16 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_e(wf_adj_args a) {
     a.gcb[off + c] += gcb;
 }
 
-__global__ void k_adj_mur_c(wf_adj_args a) {  // S7 then S6 transposed, thread per row i
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nx) return;
-    const int ny = a.ny;
-    const int64_t off = (int64_t)blockIdx.z * a.run_stride + (int64_t)i * ny;
-    float *h = a.lhz2 + off;
-    float *p = a.lp + off;
-    const float cy = a.cmy;
-    float t = h[ny - 1];  // top: Hz[i,ny-1] = P[i,ny-2] + cy (Hz[i,ny-2] - P[i,ny-1])
-    p[ny - 2] += t;
-    h[ny - 2] += cy * t;
-    p[ny - 1] -= cy * t;
-    h[ny - 1] = 0.f;
-    t = h[0];  // bottom: Hz[i,0] = P[i,1] + cy (Hz[i,1] - P[i,0])
-    p[1] += t;
-    h[1] += cy * t;
-    p[0] -= cy * t;
-    h[0] = 0.f;
-}
-
-__global__ void k_adj_mur_r(wf_adj_args a) {  // S5 then S4 transposed, thread per column j
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= a.ny) return;
-    const int nx = a.nx, ny = a.ny;
-    const int64_t off = (int64_t)blockIdx.z * a.run_stride + j;
-    float *h = a.lhz2 + off;
-    float *p = a.lp + off;
-    const float cx = a.cmx;
-    const int64_t r1 = (int64_t)(nx - 1) * ny, r2 = (int64_t)(nx - 2) * ny;
-    float t = h[r1];  // right: Hz[nx-1,j] = P[nx-2,j] + cx (Hz[nx-2,j] - P[nx-1,j])
-    p[r2] += t;
-    h[r2] += cx * t;
-    p[r1] -= cx * t;
-    h[r1] = 0.f;
-    t = h[0];  // left: Hz[0,j] = P[1,j] + cx (Hz[1,j] - P[0,j])
-    p[ny] += t;
-    h[ny] += cx * t;
-    p[0] -= cx * t;
-    h[0] = 0.f;
-}
-
+// Transposed Faraday (S3) with the transposed Mur (S7..S4) folded in: the Mur
+// transpose only rewrites the two outer rows and columns of lHz' (the output
+// of k_adj_e) and accumulates the snapshot's adjoint lP there, with lP zero
+// beforehand, so each cell's post-Mur lHz' and lP are short expressions in at
+// most four lHz' values, formed here in the same fp32 operation order as the
+// sequential transpose: top then bottom face of every row (S7, S6), then right
+// then left face of every column (S5, S4). Then
+//   lEx  = lEx' + (Dh/dy) (F(i, j-1) - F(i, j)),  lEy = lEy' + (Dh/dx) (F(i, j) - F(i-1, j))
+// with F the post-Mur lHz' on Faraday's range [0, nx-1) x [0, ny-1), and
+// lHz = post-Mur lHz' + lP (the snapshot's transpose, S1).
 __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a) {
     const int j = blockIdx.x * kBX + threadIdx.x;
     const int i = blockIdx.y * kBY + threadIdx.y;
@@ -126,18 +93,59 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a) {
     const int64_t off = (int64_t)blockIdx.z * a.run_stride;
     const int64_t c = (int64_t)i * ny + j;
     const float *h = a.lhz2 + off;
-    // Faraday adjoint: Hz[i,j] += Dh ((Ex[i,j+1]-Ex[i,j])/dy - (Ey[i+1,j]-Ey[i,j])/dx), i<=nx-2, j<=ny-2
-    auto F = [&](int ii, int jj) -> float {
-        return (ii >= 0 && jj >= 0 && ii <= nx - 2 && jj <= ny - 2) ? h[(int64_t)ii * ny + jj] : 0.f;
+    const float cx = a.cmx, cy = a.cmy;
+    auto H = [&](int ii, int jj) -> float { return h[(int64_t)ii * ny + jj]; };
+    // after the row faces (S7 then S6 transposed)
+    auto H1 = [&](int ii, int jj) -> float {
+        if (jj == ny - 1 || jj == 0) return 0.f;
+        if (jj == ny - 2) return H(ii, ny - 2) + cy * H(ii, ny - 1);
+        if (jj == 1) return H(ii, 1) + cy * H(ii, 0);
+        return H(ii, jj);
     };
-    const float fij = F(i, j);
-    const float lex = a.lex2[off + c] + (a.dh / a.dy) * (F(i, j - 1) - fij);
-    const float ley = a.ley2[off + c] + (a.dh / a.dx) * (fij - F(i - 1, j));
-    a.lex[off + c] = lex;
-    a.ley[off + c] = ley;
-    const float pv = a.lp[off + c];
-    a.lhz[off + c] = h[c] + pv;
-    if (pv != 0.f) a.lp[off + c] = 0.f;
+    // after the column faces (S5 then S4 transposed)
+    auto H2 = [&](int ii, int jj) -> float {
+        if (ii == nx - 1 || ii == 0) return 0.f;
+        if (ii == nx - 2) return H1(nx - 2, jj) + cx * H1(nx - 1, jj);
+        if (ii == 1) return H1(1, jj) + cx * H1(0, jj);
+        return H1(ii, jj);
+    };
+    auto F = [&](int ii, int jj) -> float {
+        return (ii >= 0 && jj >= 0 && ii <= nx - 2 && jj <= ny - 2) ? H2(ii, jj) : 0.f;
+    };
+    float fij, fl, fd, hz, pv = 0.f;
+    const bool mid_rows = i >= 3 && i <= nx - 3;  // (i, j), (i-1, j) on no column face
+    if (mid_rows && j >= 3 && j <= ny - 3) {      // no face term reaches (i, j), (i, j-1), (i-1, j)
+        fij = h[c];
+        fl = h[c - 1];
+        fd = h[c - ny];
+        hz = fij;
+    } else if (mid_rows) {  // row faces only: cheap, for the lanes at the ends of every row
+        auto F1 = [&](int ii, int jj) -> float { return (jj >= 0 && jj <= ny - 2) ? H1(ii, jj) : 0.f; };
+        fij = F1(i, j);
+        fl = F1(i, j - 1);
+        fd = F1(i - 1, j);
+        hz = H1(i, j);
+        if (j == ny - 2) pv = pv + H(i, ny - 1);
+        if (j == ny - 1) pv = pv - cy * H(i, ny - 1);
+        if (j == 1) pv = pv + H(i, 0);
+        if (j == 0) pv = pv - cy * H(i, 0);
+    } else {
+        fij = F(i, j);
+        fl = F(i, j - 1);
+        fd = F(i - 1, j);
+        hz = H2(i, j);
+        if (j == ny - 2) pv = pv + H(i, ny - 1);
+        if (j == ny - 1) pv = pv - cy * H(i, ny - 1);
+        if (j == 1) pv = pv + H(i, 0);
+        if (j == 0) pv = pv - cy * H(i, 0);
+        if (i == nx - 2) pv = pv + H1(nx - 1, j);
+        if (i == nx - 1) pv = pv - cx * H1(nx - 1, j);
+        if (i == 1) pv = pv + H1(0, j);
+        if (i == 0) pv = pv - cx * H1(0, j);
+    }
+    a.lex[off + c] = a.lex2[off + c] + (a.dh / a.dy) * (fl - fij);
+    a.ley[off + c] = a.ley2[off + c] + (a.dh / a.dx) * (fij - fd);
+    a.lhz[off + c] = hz + pv;
 }
 
 }  // namespace
@@ -154,11 +162,22 @@ extern "C" int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream) {
         wf::set_error("wf_adjoint_step: bad sizes");
         return WF_ERR_ARG;
     }
-    if (a.n_rx > 0) hipLaunchKernelGGL(k_adj_record, dim3(a.n_runs), dim3(64), 0, stream, a);
-    dim3 g((a.ny + kBX - 1) / kBX, (a.nx + kBY - 1) / kBY, a.n_runs);
-    hipLaunchKernelGGL(k_adj_e, g, dim3(kBX, kBY), 0, stream, a);
-    hipLaunchKernelGGL(k_adj_mur_c, dim3((a.nx + 63) / 64, 1, a.n_runs), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(k_adj_mur_r, dim3((a.ny + 63) / 64, 1, a.n_runs), dim3(64), 0, stream, a);
-    hipLaunchKernelGGL(k_adj_h, g, dim3(kBX, kBY), 0, stream, a);
+    const int n_steps = a.n_steps > 1 ? a.n_steps : 1;
+    if (n_steps > 1 && (a.hist_stride <= 0 || a.step - (n_steps - 1) < 0)) {
+        wf::set_error("wf_adjoint_step: %d steps need a positive hist_stride and step >= n_steps-1", n_steps);
+        return WF_ERR_ARG;
+    }
+    const dim3 g((a.ny + kBX - 1) / kBX, (a.nx + kBY - 1) / kBY, a.n_runs);
+    wf_adj_args b = a;
+    for (int t = 0; t < n_steps; ++t) {
+        const int64_t back = (int64_t)t * a.hist_stride;
+        b.step = a.step - t;
+        b.ex_prev = a.ex_prev - back;
+        b.ey_prev = a.ey_prev - back;
+        b.hz_next = a.hz_next - back;
+        if (b.n_rx > 0) hipLaunchKernelGGL(k_adj_record, dim3(b.n_runs), dim3(64), 0, stream, b);
+        hipLaunchKernelGGL(k_adj_e, g, dim3(kBX, kBY), 0, stream, b);
+        hipLaunchKernelGGL(k_adj_h, g, dim3(kBX, kBY), 0, stream, b);
+    }
     return wf::check_launch("wf_adjoint_step");
 }

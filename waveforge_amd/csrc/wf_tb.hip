@@ -60,8 +60,16 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d", a.tile_i);
         return WF_ERR_UNSUPPORTED;
     }
+    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div || a.tile_i != 32 ||
+                                                      a.maxabs_out || a.stop))) {
+        wf::set_error("wf_fdtd2d_tb_advance: a state history (hist_stride %lld) needs materials, IEEE division, "
+                      "tile_i 32 and no maxabs_out or stop flags", (long long)a.hist_stride);
+        return WF_ERR_UNSUPPORTED;
+    }
     hipError_t e;
-    if (a.has_materials)
+    if (a.hist_stride > 0)
+        e = wf_tb_launch_mat_ieee_hist(a, stream);
+    else if (a.has_materials)
         e = a.fast_div ? wf_tb_launch_mat_fast(a, stream) : wf_tb_launch_mat_ieee(a, stream);
     else
         e = a.fast_div ? wf_tb_launch_free_fast(a, stream) : wf_tb_launch_free_ieee(a, stream);

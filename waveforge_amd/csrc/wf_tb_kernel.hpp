@@ -90,7 +90,9 @@ struct TileGeom {
     static constexpr int kWaves = 512 / kRegs < 1 ? 1 : (512 / kRegs > 8 ? 8 : 512 / kRegs);
 };
 
-template <int TI, int D, bool MAT, bool FAST>
+// HIST: write the core after every step s to the written state + (s-1)*hist_stride
+// (a state history for the adjoint's recompute) instead of only after the last.
+template <int TI, int D, bool MAT, bool FAST, bool HIST = false>
 __global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(TileGeom<TI, D>::kWaves)))
 k_tb(wf_tb_args a) {
     constexpr int TJ = kW - 2 * D;
@@ -214,6 +216,7 @@ k_tb(wf_tb_args a) {
     const Divider divx{R.dx, a.rdx, a.div_lo_x, a.div_span_x};
     const Divider divy{R.dy, a.rdy, a.div_lo_y, a.div_span_y};
     (void)de;
+    const bool lane_core = lane >= D && lane < D + TJ && col_in;
     // the step index, read once: the kernel's own stores (traces) could alias the
     // descriptor as far as the compiler knows, so a read inside the loop would
     // be a global load and wait on every step
@@ -385,11 +388,25 @@ k_tb(wf_tb_args a) {
             }
         }
         __syncthreads();
+        if (HIST) {
+            const int64_t h = (int64_t)(s - 1) * a.hist_stride;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const int li = w + 4 * k;
+                const int gi = gi0 + li;
+                if (li >= D && li < D + TI && gi < nx && lane_core) {
+                    const int64_t g = h + (int64_t)gi * ny + gj;
+                    dst_ex[g] = ex[k];
+                    dst_ey[g] = ey[k];
+                    dst_hz[g] = hz[k];
+                }
+            }
+        }
     }
+    if (HIST) return;  // the last step's state is in its history slot
 
     // Write the core back (and fold the stability maxima if requested).
     unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
-    const bool lane_core = lane >= D && lane < D + TJ && col_in;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int li = w + 4 * k;
@@ -437,6 +454,29 @@ hipError_t launch_depth(const wf_tb_args &a, hipStream_t st) {
 }
 
 
+template <int TI, bool MAT, bool FAST>
+hipError_t launch_depth_hist(const wf_tb_args &a, hipStream_t st) {
+    const unsigned blocks = 8u * (unsigned)((a.n_tiles + 7) / 8);
+    switch (a.depth) {
+#define WF_TB_CASE(DD)                                                                         \
+    case DD:                                                                                   \
+        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST, true>), dim3(blocks), dim3(kNT), 0, st, a); \
+        break;
+        WF_TB_CASE(1)
+        WF_TB_CASE(2)
+        WF_TB_CASE(3)
+        WF_TB_CASE(4)
+        WF_TB_CASE(5)
+        WF_TB_CASE(6)
+        WF_TB_CASE(7)
+        WF_TB_CASE(8)
+#undef WF_TB_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace wf_tb
 
 // One object file per (materials, division) pair instantiates the kernels for
@@ -446,6 +486,7 @@ WF_TB_DECLARE(wf_tb_launch_mat_fast);
 WF_TB_DECLARE(wf_tb_launch_mat_ieee);
 WF_TB_DECLARE(wf_tb_launch_free_fast);
 WF_TB_DECLARE(wf_tb_launch_free_ieee);
+WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32-row tiles only
 
 #define WF_TB_DEFINE(NAME, MAT, FAST)                                                   \
     WF_TB_DECLARE(NAME) {                                                               \

@@ -2,3 +2,5 @@
 #include "wf_tb_kernel.hpp"
 
 WF_TB_DEFINE(wf_tb_launch_mat_ieee, true, false)
+
+WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist) { return wf_tb::launch_depth_hist<32, true, false>(a, st); }
