@@ -115,9 +115,10 @@ int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz,
  * exactly the per-step semantics above; step indices are run->step +
  * step_offset + level. If maxabs_out is non-NULL, the launch also folds
  * max|E|,|H| of its final level into maxabs_out[3*run + c] (as uint bit
- * patterns; the caller zeroes it first). fast_div selects the corrected
- * reciprocal division, which callers enable only for divisors that
- * wf_div_verify proved exact on [div_lo, div_lo + div_span).
+ * patterns; the caller zeroes it first). fast_div selects the fast quotient
+ * of wf_div_verify for every |x| < bits(div_span) (IEEE division above it and
+ * for NaN); callers enable it only for divisors that wf_div_verify proved
+ * exact on [0, div_span), and div_lo must then be 0.
  * Replaces FDTD2D.run's Python loop over step() (src/core/fdtd2d.py:328-329)
  * and run_mimo's per-TX loop (examples/brain_mimo_imaging.py:176-201). */
 #define WF_TB_MAX_DEPTH 8
@@ -275,9 +276,11 @@ int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, hipStream_t 
  * pairs (host, 2*count int32); returns count (<= cap). */
 int wf_tb_tile_shapes(int32_t *out, int32_t cap);
 
-/* Exhaustive device check that the corrected-reciprocal quotient
- * q0 = x*r, q = fma(fma(-q0, d, x), r, q0) equals IEEE x/d bit for bit for
- * every float x with lo <= bits(|x|) < lo + span, both signs, and x = +0.
+/* Exhaustive device check that the fast quotient equals IEEE x/d bit for bit
+ * for every float x with lo <= bits(|x|) < lo + span, both signs. With
+ * d = m*2^e (normal, -95 <= e <= 31) and s = 2^(31-e) the fast quotient is
+ * q0 = x*r, t = fma(q0, d*s, -(x*s)), q = fma(-t, r/s, q0): the residual
+ * taken at scale s stays exact down to the smallest subnormal x.
  * Adds the number of mismatches to *bad (device uint32, caller zeroes).
  * No reference counterpart: it licenses the fast division of
  * wf_fdtd2d_tb_advance, which replaces the reference's tensor / dx. */

@@ -4,7 +4,8 @@ Both the per-step path (``wf_fdtd2d_step``) and the temporally blocked path
 (``wf_fdtd2d_tb_advance``) must reproduce the oracle bit for bit: the kernels
 use the reference's fp32 operation order and no fused multiply-adds. Cases
 cover tiny grids (4x4 and odd sizes), grids that are not multiples of the
-tile, free space and lossy media, random initial fields, every source kind
+tile, free space and lossy media, random initial fields (also ones spread
+from subnormals to 2^80 with both zeros, for the division), every source kind
 (point, line with amplitude profile, plane) on Ex, Ey and Hz, sources on Mur
 rows/columns and stacked on one cell, detectors, and batched FieldSets.
 """
@@ -45,6 +46,8 @@ CASES = {
                           detectors=[(10, 10), (120, 60), (66, 35)]),
     "batch3": case(17, 19, steps=25, init=True, batch=3, sources=[("point", "Hz", 8, 9, "gauss")],
                    detectors=[(3, 4)]),
+    "extreme_mat": case(45, 70, steps=5, init="extreme", material=True, seed=5),
+    "extreme_free": case(38, 66, steps=5, init="extreme", seed=6),
 }
 
 
@@ -74,7 +77,7 @@ def test_per_step_path_bitwise(device, name):
 @pytest.mark.parametrize("tile", [32, 16, 48])
 @pytest.mark.parametrize("depth", [1, 2, 3, 4, 5, 8])
 @pytest.mark.parametrize("name", ["tiny5x7_mat", "mat_all_sources", "edge_sources", "odd_tiles_mat", "batch3",
-                                  "free_point_hz"])
+                                  "free_point_hz", "extreme_mat", "extreme_free"])
 def test_tb_path_bitwise(device, name, tile, depth, fast_div):
     c = CASES[name]
     print(f"[tb] case={name} tile_rows={tile} depth={depth} fast_div={fast_div}", flush=True)

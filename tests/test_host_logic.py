@@ -166,10 +166,17 @@ def test_schedule_launches_checks():
 
 
 def test_division_range_for_millimetre_cells():
-    """The verified range for dx = 1 mm covers every |x| in [2^-99, 2^115)."""
+    """The verified range for dx = 1 mm (1.024 * 2^-10) is every |x| below
+    2^86, subnormals and both zeros included; divisors whose scale factor
+    2^(31-e) would leave the normal range get no fast path."""
     lo, span = division_range(float(np.float32(1e-3)))
-    assert lo == (127 - 99) << 23
-    assert lo + span == (127 + 115) << 23
+    assert lo == 0
+    assert span == (127 + 86) << 23
+    assert division_range(2.0 ** 31) == (0, (127 + 127) << 23)
+    assert division_range(2.0 ** 32) is None
+    assert division_range(2.0 ** -95) == (0, 128 << 23)
+    assert division_range(2.0 ** -96) is None
+    assert division_range(-1e-3) is None
 
 
 def test_snapshot_free_fused_step_condition():

@@ -7,14 +7,14 @@ namespace {
 
 constexpr int kTileRows[] = {32, 16, 48};
 
-// Exhaustive check of the corrected-reciprocal division against IEEE division
-// for every x with lo <= |x bits| < lo + span (both signs). Each thread covers
-// a contiguous slice and counts mismatches.
+// Exhaustive check of the fast quotient (wf_tb::divide<true>) against IEEE
+// division for every x with lo <= |x bits| < lo + span (both signs). Each
+// thread covers a strided slice and counts mismatches.
 __global__ void __launch_bounds__(256) k_div_verify(float d, float r, unsigned int lo, unsigned int span,
                                                      unsigned int *bad) {
     const unsigned int nthreads = gridDim.x * blockDim.x;
     const unsigned int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const wf_tb::Divider v{d, r, lo, span};
+    const wf_tb::Divider v = wf_tb::make_divider(d, r, 0x7f800000u);
     unsigned int local = 0;
     for (unsigned int k = tid; k < span; k += nthreads) {
         const float x = __uint_as_float(lo + k);
@@ -23,7 +23,6 @@ __global__ void __launch_bounds__(256) k_div_verify(float d, float r, unsigned i
         if (__float_as_uint(wf_tb::divide<true>(x, v)) != __float_as_uint(q)) ++local;
         if (__float_as_uint(wf_tb::divide<true>(-x, v)) != __float_as_uint(qn)) ++local;
     }
-    if (__float_as_uint(wf_tb::divide<true>(0.0f, v)) != __float_as_uint(0.0f / d)) ++local;
     if (local) atomicAdd(bad, local);
 }
 
@@ -65,6 +64,11 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: a state history (hist_stride %lld) needs materials, IEEE division, "
                       "tile_i 32 and no maxabs_out or stop flags", (long long)a.hist_stride);
         return WF_ERR_UNSUPPORTED;
+    }
+    if (a.fast_div && (a.div_lo_x != 0u || a.div_lo_y != 0u || a.div_span_x > 0x7f800000u ||
+                       a.div_span_y > 0x7f800000u)) {
+        wf::set_error("wf_fdtd2d_tb_advance: fast_div needs div_lo = 0 and div_span <= bits(inf)");
+        return WF_ERR_ARG;
     }
     hipError_t e;
     if (a.hist_stride > 0)
@@ -125,8 +129,10 @@ extern "C" int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, h
 }
 
 extern "C" int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, hipStream_t stream) {
-    if (!bad || !(d > 0.0f) || span == 0u || lo + span < lo || lo + span > 0x7f800000u) {
-        wf::set_error("wf_div_verify: bad arguments");
+    const unsigned int be = (__builtin_bit_cast(uint32_t, d) >> 23) & 0xffu;
+    if (!bad || !(d > 0.0f) || be < 32u || be > 158u || span == 0u || lo + span < lo || lo + span > 0x7f800000u) {
+        wf::set_error("wf_div_verify: bad arguments (d must be normal with exponent in [-95, 31], "
+                      "0 < span, lo + span <= bits(inf))");
         return WF_ERR_ARG;
     }
     hipLaunchKernelGGL(k_div_verify, dim3(8192), dim3(256), 0, stream, d, r, lo, span, reinterpret_cast<unsigned int *>(bad));

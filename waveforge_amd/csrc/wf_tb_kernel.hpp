@@ -30,10 +30,12 @@
 //      read from LDS (examples/brain_mimo_imaging.py:196-198).
 //
 // Division by fp32(dx) / fp32(dy): IEEE division, or (FAST) the reciprocal
-// product corrected by one fused residual step, q1 = q0 + (x - q0*d)*r, which
-// the library proves equal to IEEE division for every |x| in [lo, lo+span)
-// by an exhaustive device check per divisor (wf_div_verify); zero uses the
-// fast form too (it is exact for +0), other |x| fall back to IEEE division.
+// product corrected by one fused residual step taken at a scale s = 2^k that
+// puts d*s in [2^31, 2^32): q0 = x*r, t = q0*(d*s) - x*s, q = q0 - t*(r/s).
+// The scaling keeps the residual exact down to the smallest subnormal x, and
+// the operand signs make q = x for x = +0 and x = -0. The library proves q
+// equal to IEEE x/d for every |x| below a bound h (wf_div_verify, exhaustive
+// per divisor); NaN and |x| >= h fall back to IEEE division.
 //
 // Tiles are mapped XCD-aware: the dispatcher deals workgroups round-robin to
 // the 8 XCDs, so block b runs on XCD b%8; we hand each XCD a contiguous band
@@ -56,22 +58,36 @@ __device__ __forceinline__ float lane_prev(float v) {  // value of lane-1 (0 in 
 }
 
 struct Divider {
-    float d, r;
-    unsigned int lo, span;
+    float d, r, d2, rk, s, h;
 };
 
-// x / d. FAST: corrected reciprocal product for every lane; if any lane of
-// the wavefront holds an x outside the verified range (or -0), the whole
-// wavefront takes the IEEE division and those lanes keep its result. The
-// ballot makes the fallback a uniform (scalar) branch.
+// d = m*2^e (normal, -95 <= e <= 31; the host only enables FAST then), r =
+// RN(1/d), and the verified range |x| < h with h = bits(span). Every factor is
+// a power-of-two rescaling, formed on the exponent bits.
+__device__ __forceinline__ Divider make_divider(float d, float r, unsigned int span) {
+    const unsigned int db = __float_as_uint(d);
+    const unsigned int k = 158u - ((db >> 23) & 0xffu);  // 31 - e
+    Divider v;
+    v.d = d;
+    v.r = r;
+    v.d2 = __uint_as_float(db + (k << 23));
+    v.rk = __uint_as_float(__float_as_uint(r) - (k << 23));
+    v.s = __uint_as_float((k + 127u) << 23);
+    v.h = __uint_as_float(span);
+    return v;
+}
+
+// x / d. FAST: the scaled corrected quotient for every lane; if any lane of
+// the wavefront holds an x outside the verified range, the whole wavefront
+// takes the IEEE division and those lanes keep its result. The ballot makes
+// the fallback a uniform (scalar) branch.
 template <bool FAST>
 __device__ __forceinline__ float divide(float x, const Divider &v) {
     if (!FAST) return x / v.d;
     const float q0 = x * v.r;
-    const float e = __builtin_fmaf(-q0, v.d, x);
-    float q = __builtin_fmaf(e, v.r, q0);
-    const unsigned int bx = __float_as_uint(x);
-    const bool slow = ((bx & 0x7fffffffu) - v.lo >= v.span) && bx != 0u;
+    const float t = __builtin_fmaf(q0, v.d2, -(x * v.s));
+    float q = __builtin_fmaf(-t, v.rk, q0);
+    const bool slow = !(__builtin_fabsf(x) < v.h);
     if (__builtin_expect(__ballot(slow) != 0ull, 0)) {
         const float qs = x / v.d;
         q = slow ? qs : q;
@@ -213,8 +229,8 @@ k_tb(wf_tb_args a) {
     const unsigned int rowmask = s_rowmask[w];
     const bool edge = (gi0 <= 0) || (gi0 + RI >= nx) || (gj0 <= 0) || (gj0 + kW >= ny);
     const float dh = R.dh, de = R.de, cx = R.cmx, cy = R.cmy;
-    const Divider divx{R.dx, a.rdx, a.div_lo_x, a.div_span_x};
-    const Divider divy{R.dy, a.rdy, a.div_lo_y, a.div_span_y};
+    const Divider divx = make_divider(R.dx, a.rdx, a.div_span_x);
+    const Divider divy = make_divider(R.dy, a.rdy, a.div_span_y);
     (void)de;
     const bool lane_core = lane >= D && lane < D + TJ && col_in;
     // the step index, read once: the kernel's own stores (traces) could alias the

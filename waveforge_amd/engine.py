@@ -307,16 +307,17 @@ def _f32_bits(x: float) -> int:
     return int(np.array(x, dtype=np.float32).view(np.uint32))
 
 
-def division_range(d: float) -> tuple[int, int]:
-    """Bit range ``[lo, lo+span)`` of ``|x|`` on which the corrected reciprocal
-    quotient is checked against IEEE ``x/d``: quotients stay normal and finite
-    and the residual ``x - q0*d`` stays normal there."""
-    ed = ((_f32_bits(d) >> 23) & 0xFF) - 127
-    lo_e = max(-99, ed - 122)
-    hi_e = min(127, ed + 125)
-    lo = (lo_e + 127) << 23
-    hi = (hi_e + 127) << 23
-    return lo, hi - lo
+def division_range(d: float) -> Optional[tuple[int, int]]:
+    """Bit range ``[0, span)`` of ``|x|`` on which the fast quotient (the
+    reciprocal product with its residual corrected at the scale that puts
+    ``d`` at 2^31) is checked against IEEE ``x/d``: every ``|x| < 2^(96+e)``
+    for ``d = m*2^e``, below where ``x*2^(31-e)`` would overflow. None for a
+    divisor outside ``-95 <= e <= 31`` (no fast path)."""
+    bits = _f32_bits(d)
+    ed = ((bits >> 23) & 0xFF) - 127
+    if not (-95 <= ed <= 31) or (bits >> 31):
+        return None
+    return 0, (96 + ed + 127) << 23
 
 
 _DIV_CACHE: dict = {}
@@ -329,11 +330,12 @@ def verified_division(d: float, device: torch.device) -> Optional[tuple[float, i
     key = (df, str(device))
     if key in _DIV_CACHE:
         return _DIV_CACHE[key]
-    if not (df > 0.0) or ((_f32_bits(df) >> 23) & 0xFF) in (0, 0xFF):
+    rng = division_range(df) if df > 0.0 else None
+    if rng is None:
         _DIV_CACHE[key] = None
         return None
     r = float(np.float32(1.0) / np.float32(df))
-    lo, span = division_range(df)
+    lo, span = rng
     bad = torch.zeros(1, dtype=torch.int32, device=device)
     lib = _native.load()
     stream = _native.stream_handle(device)
