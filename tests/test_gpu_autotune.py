@@ -11,14 +11,14 @@ pytestmark = pytest.mark.gpu
 
 
 def test_tuner_times_every_candidate_and_caches(device, tmp_path):
-    from waveforge_amd import autotune
+    from waveforge_amd import _native, autotune
 
     c = case(200, 300, steps=30, material=True, sources=[("point", "Hz", 50, 60, "gauss")], detectors=[(3, 4)])
     sim, _ = build_product(c, device, depth="auto", tile="auto")
     sim.run(30)
     res = sim._tuned
     assert sim.launch_shape == (res.tile_rows, res.depth)
-    assert len(res.rates) == 3 * 8
+    assert len(res.rates) == len(_native.tile_rows()) * 8
     assert max(res.rates.values()) == res.rates[f"{res.tile_rows},{res.depth}"]
     again = autotune.tune_runs(sim._build_runs(), fast_div=True)
     assert again is res

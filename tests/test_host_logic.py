@@ -208,3 +208,18 @@ def test_snapshot_free_fused_step_condition():
     # Nz = 1: no z faces, so k = 0 is no Mur plane
     assert snapshot_is_state(table([("Hx", 5, 5, 0), ("Hy", 5, 5, 0)], nz=1), nx, ny, 1)
     assert not snapshot_is_state(table([("Hx", 5, 0, 0)], nz=1), nx, ny, 1)
+
+
+def test_default_tile_rows_by_workload():
+    """64-row tiles only when the runs make at least 2048 of them."""
+    from types import SimpleNamespace
+
+    from waveforge_amd.core.fdtd2d import default_tile_rows
+
+    def runs(*shapes):
+        return [SimpleNamespace(nx=a, ny=b) for a, b in shapes]
+
+    assert default_tile_rows(runs((4096, 4096)), 6) == 64   # 64 x 79 tiles
+    assert default_tile_rows(runs((1024, 1024)), 6) == 32   # 16 x 20 tiles
+    assert default_tile_rows(runs(*[(256, 256)] * 512), 6) == 64
+    assert default_tile_rows(runs((256, 256)), 6) == 32

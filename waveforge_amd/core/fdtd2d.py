@@ -47,7 +47,18 @@ EPS0: float = 8.8541878128e-12
 MU0: float = 1.2566370614e-6
 
 DEFAULT_TILE_ROWS = 32
+TALL_TILE_ROWS = 64
+TALL_TILE_MIN_TILES = 2048
 DEFAULT_DEPTH = 6
+
+
+def default_tile_rows(runs, depth: int) -> int:
+    """Tile height when none is given: 64-row tiles (8-wavefront workgroups,
+    a smaller halo share) once the runs make at least 2048 of them, enough
+    workgroups to keep 256 CUs busy; 32-row tiles otherwise."""
+    tj = 64 - 2 * depth
+    n = sum(-(-r.nx // TALL_TILE_ROWS) * -(-r.ny // tj) for r in runs)
+    return TALL_TILE_ROWS if n >= TALL_TILE_MIN_TILES else DEFAULT_TILE_ROWS
 
 
 class SimulationDivergedError(RuntimeError):
@@ -304,8 +315,8 @@ class FDTD2D:
     def _launch_shape(self, runs, inert) -> tuple[int, int]:
         """(tile rows, depth): as given, the defaults, or, for ``"auto"``, the
         autotuner's choice for these grids (timed once per shape set)."""
-        tile = self._tile or DEFAULT_TILE_ROWS
-        depth = self._depth or DEFAULT_DEPTH
+        depth = self._depth if isinstance(self._depth, int) else DEFAULT_DEPTH
+        tile = self._tile if isinstance(self._tile, int) else default_tile_rows(runs, depth)
         if "auto" in (self._tile, self._depth) and not inert:
             if getattr(self, "_tuned_key", None) != self._cache_key:
                 from .. import autotune
@@ -316,9 +327,7 @@ class FDTD2D:
                     depths=autotune.DEPTHS if self._depth == "auto" else [depth])
                 self._tuned, self._tuned_key = res, self._cache_key
             tile, depth = self._tuned.tile_rows, self._tuned.depth
-        elif "auto" in (self._tile, self._depth):
-            tile = DEFAULT_TILE_ROWS if self._tile == "auto" else tile
-            depth = DEFAULT_DEPTH if self._depth == "auto" else depth
+
         self.launch_shape = (tile, depth)
         return tile, depth
 

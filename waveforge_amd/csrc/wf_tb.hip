@@ -5,7 +5,7 @@
 
 namespace {
 
-constexpr int kTileRows[] = {32, 16, 48};
+constexpr int kTileRows[] = {32, 16, 48, 64};
 
 // Exhaustive check of the fast quotient (wf_tb::divide<true>) against IEEE
 // division for every x with lo <= |x bits| < lo + span (both signs). Each
@@ -55,7 +55,7 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: parity must be 0 or 1");
         return WF_ERR_ARG;
     }
-    if (a.tile_i != 16 && a.tile_i != 32 && a.tile_i != 48) {
+    if (a.tile_i != 16 && a.tile_i != 32 && a.tile_i != 48 && a.tile_i != 64) {
         wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d", a.tile_i);
         return WF_ERR_UNSUPPORTED;
     }

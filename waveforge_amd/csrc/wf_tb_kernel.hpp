@@ -45,7 +45,6 @@
 
 namespace wf_tb {
 
-constexpr int kNT = 256;        // threads per workgroup (4 wavefronts)
 constexpr int kW = 64;          // region width = wavefront size
 constexpr int kMaxGroups = 64;  // source groups per tile region (host checks)
 constexpr int kMaxDets = 64;    // detectors per tile core (host checks)
@@ -95,24 +94,37 @@ __device__ __forceinline__ float divide(float x, const Divider &v) {
     return q;
 }
 
-// Rows per lane and the occupancy the register budget is sized for: each lane
-// keeps 5 values per row (Ex, Ey, Hz, Ca, Cb) plus ~64 registers of
-// temporaries; without the cap the scheduler hoists every row's loads and
-// runs at one or two waves per SIMD.
+// Wavefronts per workgroup, rows per lane and the occupancy the register
+// budget is sized for. Tiles of 64 rows use 8 wavefronts (the halo rows are a
+// smaller share of a taller region), shorter tiles 4. Each lane keeps 5 values
+// per row (Ex, Ey, Hz, Ca, Cb) plus ~64 registers of temporaries; without the
+// cap the scheduler hoists every row's loads and runs at one or two waves per
+// SIMD.
+template <int TI>
+constexpr int kWavesOf = TI >= 64 ? 8 : 4;
+template <int TI>
+constexpr int kThreadsOf = kWavesOf<TI> * kW;
+
 template <int TI, int D>
 struct TileGeom {
-    static constexpr int NR = (TI + 2 * D + 3) / 4;
+    static constexpr int NW = kWavesOf<TI>;
+    static constexpr int NT = NW * kW;
+    static constexpr int NR = (TI + 2 * D + NW - 1) / NW;
     static constexpr int kRegs = 5 * NR + 64;
     static constexpr int kWaves = 512 / kRegs < 1 ? 1 : (512 / kRegs > 8 ? 8 : 512 / kRegs);
+    static_assert(NR <= 32, "row flags are one 32-bit word per wavefront");
+    static_assert(NT >= 2 * kW + TI + 2 * D, "edge passes need a thread per face cell");
 };
 
 // HIST: write the core after every step s to the written state + (s-1)*hist_stride
 // (a state history for the adjoint's recompute) instead of only after the last.
 template <int TI, int D, bool MAT, bool FAST, bool HIST = false>
-__global__ void __launch_bounds__(kNT) __attribute__((amdgpu_waves_per_eu(TileGeom<TI, D>::kWaves)))
+__global__ void __launch_bounds__(kThreadsOf<TI>) __attribute__((amdgpu_waves_per_eu(TileGeom<TI, D>::kWaves)))
 k_tb(wf_tb_args a) {
     constexpr int TJ = kW - 2 * D;
     constexpr int RI = TI + 2 * D;
+    constexpr int NW = TileGeom<TI, D>::NW;
+    constexpr int kNT = TileGeom<TI, D>::NT;
     constexpr int NR = TileGeom<TI, D>::NR;
     __shared__ float s_hz[RI * kW];
     __shared__ float s_ey[RI * kW];
@@ -121,7 +133,7 @@ k_tb(wf_tb_args a) {
     __shared__ int s_det[kMaxDets];
     __shared__ int s_det_cell[kMaxDets];
     __shared__ int s_ngrp, s_ndet, s_ey_src;
-    __shared__ unsigned int s_rowmask[4];  // per wavefront: bit k set if row w+4k holds a source
+    __shared__ unsigned int s_rowmask[NW];  // per wavefront: bit k set if row w+NW*k holds a source
     __shared__ unsigned int s_max[3];
     __shared__ float s_prow[2][kW];  // edge tiles: snapshot of rows i = 1 and i = nx-2
     __shared__ float s_pcol[2][RI];  // edge tiles: snapshot of columns j = 1 and j = ny-2
@@ -154,7 +166,7 @@ k_tb(wf_tb_args a) {
         // The run failed a stability check earlier in this stream: carry its
         // state through unchanged so it ends in the live buffers as it was at
         // the failing check.
-        for (int k = w; k < TI; k += 4) {
+        for (int k = w; k < TI; k += NW) {
             const int gi = I0 + k;
             const int cj = J0 + lane;
             if (gi < nx && lane < TJ && cj < ny) {
@@ -172,13 +184,13 @@ k_tb(wf_tb_args a) {
         s_ndet = 0;
         s_ey_src = 0;
         s_max[0] = s_max[1] = s_max[2] = 0u;
-        s_rowmask[0] = s_rowmask[1] = s_rowmask[2] = s_rowmask[3] = 0u;
     }
+    if (tid < NW) s_rowmask[tid] = 0u;
 
     float ex[NR], ey[NR], hz[NR], ca[NR], cb[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const int li = w + 4 * k;
+        const int li = w + NW * k;
         const int gi = gi0 + li;
         ex[k] = ey[k] = hz[k] = 0.f;
         ca[k] = cb[k] = 0.f;
@@ -206,7 +218,7 @@ k_tb(wf_tb_args a) {
             if (k < kMaxGroups) {
                 s_grp[k] = g;
                 s_grp_cell[k] = li * kW + lj;
-                atomicOr(&s_rowmask[li & 3], 1u << (li >> 2));
+                atomicOr(&s_rowmask[li % NW], 1u << (li / NW));
                 if (e0.comp == WF_COMP_EY) s_ey_src = 1;
             }
         }
@@ -254,7 +266,7 @@ k_tb(wf_tb_args a) {
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 if (rm & (1u << k)) {
-                    const int li = w + 4 * k;
+                    const int li = w + NW * k;
                     for (int q = 0; q < ngrp; ++q) {
                         const int cl = s_grp_cell[q];
                         if (cl == li * kW + lane) {
@@ -299,7 +311,7 @@ k_tb(wf_tb_args a) {
         }
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const int li = w + 4 * k;
+            const int li = w + NW * k;
             if (li >= s - 1 && li < RI - s) {
                 const int gi = gib + li;
                 const float ex_next = lane_next(ex[k]);
@@ -363,7 +375,7 @@ k_tb(wf_tb_args a) {
             // which Mur overwrites), so whole rows are reloaded.
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const int li = w + 4 * k;
+                const int li = w + NW * k;
                 if (li < RI) hz[k] = s_hz[li * kW + lane];
             }
         }
@@ -374,7 +386,7 @@ k_tb(wf_tb_args a) {
         }
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const int li = w + 4 * k;
+            const int li = w + NW * k;
             if (li >= s && li < RI - s) {
                 const int gi = gib + li;
                 const float hz_left = lane_prev(hz[k]);
@@ -408,7 +420,7 @@ k_tb(wf_tb_args a) {
             const int64_t h = (int64_t)(s - 1) * a.hist_stride;
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const int li = w + 4 * k;
+                const int li = w + NW * k;
                 const int gi = gi0 + li;
                 if (li >= D && li < D + TI && gi < nx && lane_core) {
                     const int64_t g = h + (int64_t)gi * ny + gj;
@@ -425,7 +437,7 @@ k_tb(wf_tb_args a) {
     unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const int li = w + 4 * k;
+        const int li = w + NW * k;
         const int gi = gi0 + li;
         if (li >= D && li < D + TI && gi < nx && lane_core) {
             const size_t g = (size_t)gi * ny + gj;
@@ -452,7 +464,7 @@ hipError_t launch_depth(const wf_tb_args &a, hipStream_t st) {
     switch (a.depth) {
 #define WF_TB_CASE(DD)                                                                         \
     case DD:                                                                                   \
-        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST>), dim3(blocks), dim3(kNT), 0, st, a);      \
+        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST>), dim3(blocks), dim3(TileGeom<TI, DD>::NT), 0, st, a); \
         break;
         WF_TB_CASE(1)
         WF_TB_CASE(2)
@@ -476,7 +488,7 @@ hipError_t launch_depth_hist(const wf_tb_args &a, hipStream_t st) {
     switch (a.depth) {
 #define WF_TB_CASE(DD)                                                                         \
     case DD:                                                                                   \
-        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST, true>), dim3(blocks), dim3(kNT), 0, st, a); \
+        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST, true>), dim3(blocks), dim3(TileGeom<TI, DD>::NT), 0, st, a); \
         break;
         WF_TB_CASE(1)
         WF_TB_CASE(2)
@@ -510,6 +522,7 @@ WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32-row tiles on
             case 16: return wf_tb::launch_depth<16, MAT, FAST>(a, st);                  \
             case 32: return wf_tb::launch_depth<32, MAT, FAST>(a, st);                  \
             case 48: return wf_tb::launch_depth<48, MAT, FAST>(a, st);                  \
+            case 64: return wf_tb::launch_depth<64, MAT, FAST>(a, st);                  \
             default: return hipErrorInvalidValue;                                       \
         }                                                                               \
     }
