@@ -76,6 +76,12 @@ __device__ __forceinline__ Divider make_divider(float d, float r, unsigned int s
     return v;
 }
 
+__device__ __forceinline__ float fast_quotient(float x, const Divider &v) {
+    const float q0 = x * v.r;
+    const float t = __builtin_fmaf(q0, v.d2, -(x * v.s));
+    return __builtin_fmaf(-t, v.rk, q0);
+}
+
 // x / d. FAST: the scaled corrected quotient for every lane; if any lane of
 // the wavefront holds an x outside the verified range, the whole wavefront
 // takes the IEEE division and those lanes keep its result. The ballot makes
@@ -83,15 +89,34 @@ __device__ __forceinline__ Divider make_divider(float d, float r, unsigned int s
 template <bool FAST>
 __device__ __forceinline__ float divide(float x, const Divider &v) {
     if (!FAST) return x / v.d;
-    const float q0 = x * v.r;
-    const float t = __builtin_fmaf(q0, v.d2, -(x * v.s));
-    float q = __builtin_fmaf(-t, v.rk, q0);
+    float q = fast_quotient(x, v);
     const bool slow = !(__builtin_fabsf(x) < v.h);
     if (__builtin_expect(__ballot(slow) != 0ull, 0)) {
         const float qs = x / v.d;
         q = slow ? qs : q;
     }
     return q;
+}
+
+// The two quotients of one Faraday or Ampere row with one range ballot and
+// one branch between them: the stepper's instruction stream is as much
+// scalar as vector, so the branch per division costs as much as its range test.
+template <bool FAST>
+__device__ __forceinline__ void divide2(float x1, const Divider &v1, float x2, const Divider &v2, float &q1, float &q2) {
+    if (!FAST) {
+        q1 = x1 / v1.d;
+        q2 = x2 / v2.d;
+        return;
+    }
+    q1 = fast_quotient(x1, v1);
+    q2 = fast_quotient(x2, v2);
+    const bool s1 = !(__builtin_fabsf(x1) < v1.h);
+    const bool s2 = !(__builtin_fabsf(x2) < v2.h);
+    if (__builtin_expect(__ballot(s1 || s2) != 0ull, 0)) {
+        const float qa = x1 / v1.d, qb = x2 / v2.d;
+        q1 = s1 ? qa : q1;
+        q2 = s2 ? qb : q2;
+    }
 }
 
 // Wavefronts per workgroup, rows per lane and the occupancy the register
@@ -312,15 +337,18 @@ k_tb(wf_tb_args a) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int li = w + NW * k;
-            if (li >= s - 1 && li < RI - s) {
+            // rows s-1 <= li < RI-s, and below 1 <= gi <= nx-2: each range is one
+            // unsigned compare, so the uniform row tests stay short scalar
+            // compare-and-branch pairs rather than combined lane masks
+            if ((unsigned)(li - (s - 1)) < (unsigned)(RI + 1 - 2 * s)) {
                 const int gi = gib + li;
                 const float ex_next = lane_next(ex[k]);
                 const int c = li * kW + lane;
                 const float ey_down = s_ey[c + kW];
-                const float av = divide<FAST>(ex_next - ex[k], divy);
-                const float bv = divide<FAST>(ey_down - ey[k], divx);
+                float av, bv;
+                divide2<FAST>(ex_next - ex[k], divy, ey_down - ey[k], divx, av, bv);
                 const float F = hz[k] + dh * (av - bv);
-                if (gi >= 1 && gi <= nx - 2) {
+                if ((unsigned)(gi - 1) < (unsigned)max(nx - 2, 0)) {
                     const bool lane_int = gj >= 1 && gj <= ny - 2;
                     hz[k] = lane_int ? F : hz[k];
                     if (lane_int) s_hz[c] = F;
@@ -387,14 +415,14 @@ k_tb(wf_tb_args a) {
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int li = w + NW * k;
-            if (li >= s && li < RI - s) {
+            if ((unsigned)(li - s) < (unsigned)(RI - 2 * s)) {  // s <= li < RI-s
                 const int gi = gib + li;
                 const float hz_left = lane_prev(hz[k]);
                 const int c = li * kW + lane;
                 const float hz_up = s_hz[c - kW];
-                const float gyv = divide<FAST>(hz[k] - hz_left, divy);
-                const float gxv = divide<FAST>(hz[k] - hz_up, divx);
-                if (gi >= 0 && gi < nx) {
+                float gyv, gxv;
+                divide2<FAST>(hz[k] - hz_left, divy, hz[k] - hz_up, divx, gyv, gxv);
+                if ((unsigned)gi < (unsigned)nx) {
                     float nex, ney;
                     if (MAT) {
                         const float t1 = ca[k] * ex[k];
