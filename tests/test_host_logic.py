@@ -210,16 +210,17 @@ def test_snapshot_free_fused_step_condition():
     assert not snapshot_is_state(table([("Hx", 5, 0, 0)], nz=1), nx, ny, 1)
 
 
-def test_default_tile_rows_by_workload():
-    """64-row tiles only when the runs make at least 2048 of them."""
+def test_default_launch_shape_by_workload():
+    """48-row tiles 8 steps deep only when the runs make at least 1536 of them."""
     from types import SimpleNamespace
 
-    from waveforge_amd.core.fdtd2d import default_tile_rows
+    from waveforge_amd.core.fdtd2d import default_launch_shape
 
     def runs(*shapes):
         return [SimpleNamespace(nx=a, ny=b) for a, b in shapes]
 
-    assert default_tile_rows(runs((4096, 4096)), 6) == 64   # 64 x 79 tiles
-    assert default_tile_rows(runs((1024, 1024)), 6) == 32   # 16 x 20 tiles
-    assert default_tile_rows(runs(*[(256, 256)] * 512), 6) == 64
-    assert default_tile_rows(runs((256, 256)), 6) == 32
+    assert default_launch_shape(runs((4096, 4096))) == (48, 8)   # 86 x 86 tiles
+    assert default_launch_shape(runs((1024, 1024))) == (32, 6)   # 22 x 22 tiles
+    assert default_launch_shape(runs(*[(256, 256)] * 32)) == (32, 6)   # 36 tiles each
+    assert default_launch_shape(runs(*[(256, 256)] * 64)) == (48, 8)
+    assert default_launch_shape(runs((256, 256))) == (32, 6)

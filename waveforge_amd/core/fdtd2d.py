@@ -47,18 +47,22 @@ EPS0: float = 8.8541878128e-12
 MU0: float = 1.2566370614e-6
 
 DEFAULT_TILE_ROWS = 32
-TALL_TILE_ROWS = 64
-TALL_TILE_MIN_TILES = 2048
 DEFAULT_DEPTH = 6
+TALL_TILE_ROWS = 48
+TALL_TILE_DEPTH = 8
+TALL_TILE_MIN_TILES = 1536
 
 
-def default_tile_rows(runs, depth: int) -> int:
-    """Tile height when none is given: 64-row tiles (8-wavefront workgroups,
-    a smaller halo share) once the runs make at least 2048 of them, enough
-    workgroups to keep 256 CUs busy; 32-row tiles otherwise."""
-    tj = 64 - 2 * depth
+def default_launch_shape(runs) -> tuple[int, int]:
+    """(tile rows, depth) when none is given: 48-row tiles (8-wavefront
+    workgroups at 6 waves per SIMD, a small halo share) 8 steps deep once the
+    runs make at least 1536 such tiles, two full rounds of workgroups on 256
+    CUs; 32-row tiles 6 steps deep otherwise."""
+    tj = 64 - 2 * TALL_TILE_DEPTH
     n = sum(-(-r.nx // TALL_TILE_ROWS) * -(-r.ny // tj) for r in runs)
-    return TALL_TILE_ROWS if n >= TALL_TILE_MIN_TILES else DEFAULT_TILE_ROWS
+    if n >= TALL_TILE_MIN_TILES:
+        return TALL_TILE_ROWS, TALL_TILE_DEPTH
+    return DEFAULT_TILE_ROWS, DEFAULT_DEPTH
 
 
 class SimulationDivergedError(RuntimeError):
@@ -315,8 +319,9 @@ class FDTD2D:
     def _launch_shape(self, runs, inert) -> tuple[int, int]:
         """(tile rows, depth): as given, the defaults, or, for ``"auto"``, the
         autotuner's choice for these grids (timed once per shape set)."""
-        depth = self._depth if isinstance(self._depth, int) else DEFAULT_DEPTH
-        tile = self._tile if isinstance(self._tile, int) else default_tile_rows(runs, depth)
+        tile0, depth0 = default_launch_shape(runs)
+        depth = self._depth if isinstance(self._depth, int) else depth0
+        tile = self._tile if isinstance(self._tile, int) else tile0
         if "auto" in (self._tile, self._depth) and not inert:
             if getattr(self, "_tuned_key", None) != self._cache_key:
                 from .. import autotune

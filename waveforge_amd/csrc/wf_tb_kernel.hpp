@@ -119,14 +119,26 @@ __device__ __forceinline__ void divide2(float x1, const Divider &v1, float x2, c
     }
 }
 
+// Bits k (k < NR) of the rows w + NW*k of wavefront w that lie in [lo, hi).
+template <int NW, int NR>
+__device__ __forceinline__ unsigned int row_bits(int w, int lo, int hi) {
+    const int a = lo - w, b = hi - w;
+    const unsigned int klo = a <= 0 ? 0u : ((unsigned)a + NW - 1) / NW;
+    const unsigned int khi = b <= 0 ? 0u : min(((unsigned)b + NW - 1) / NW, (unsigned)NR);
+    if (khi <= klo) return 0u;
+    return (unsigned int)(((1ull << khi) - 1ull) & ~((1ull << klo) - 1ull));
+}
+
 // Wavefronts per workgroup, rows per lane and the occupancy the register
-// budget is sized for. Tiles of 64 rows use 8 wavefronts (the halo rows are a
-// smaller share of a taller region), shorter tiles 4. Each lane keeps 5 values
-// per row (Ex, Ey, Hz, Ca, Cb) plus ~64 registers of temporaries; without the
-// cap the scheduler hoists every row's loads and runs at one or two waves per
-// SIMD.
+// budget is sized for. Tiles of 48 or more rows use 8 wavefronts (the halo
+// rows are a smaller share of a taller region), shorter tiles 4. Each lane
+// keeps 5 values per row (Ex, Ey, Hz, Ca, Cb) plus 40-48 registers of
+// temporaries; without the cap the scheduler hoists every row's loads and runs
+// at one or two waves per SIMD. An 8-wavefront workgroup puts two waves on
+// each SIMD, so its occupancy is 4 or 6 waves per SIMD: 6 (80 registers) when
+// the rows fit.
 template <int TI>
-constexpr int kWavesOf = TI >= 64 ? 8 : 4;
+constexpr int kWavesOf = TI >= 48 ? 8 : 4;
 template <int TI>
 constexpr int kThreadsOf = kWavesOf<TI> * kW;
 
@@ -135,8 +147,8 @@ struct TileGeom {
     static constexpr int NW = kWavesOf<TI>;
     static constexpr int NT = NW * kW;
     static constexpr int NR = (TI + 2 * D + NW - 1) / NW;
-    static constexpr int kRegs = 5 * NR + 64;
-    static constexpr int kWaves = 512 / kRegs < 1 ? 1 : (512 / kRegs > 8 ? 8 : 512 / kRegs);
+    static constexpr int kRegs = 5 * NR + (NW == 8 ? 40 : 48);
+    static constexpr int kWaves = NW == 8 ? (kRegs <= 80 ? 6 : 4) : (512 / kRegs > 8 ? 8 : 512 / kRegs);
     static_assert(NR <= 32, "row flags are one 32-bit word per wavefront");
     static_assert(NT >= 2 * kW + TI + 2 * D, "edge passes need a thread per face cell");
 };
@@ -334,25 +346,23 @@ k_tb(wf_tb_args a) {
             }
             __syncthreads();
         }
+        // rows this step's Faraday updates: s-1 <= li < RI-s and 1 <= gi <= nx-2,
+        // one bit per row, so each row's test is one scalar bit test
+        unsigned int fmask = row_bits<NW, NR>(w, max(s - 1, 1 - gib), min(RI - s, nx - 1 - gib));
+        asm volatile("" : "+s"(fmask));
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int li = w + NW * k;
-            // rows s-1 <= li < RI-s, and below 1 <= gi <= nx-2: each range is one
-            // unsigned compare, so the uniform row tests stay short scalar
-            // compare-and-branch pairs rather than combined lane masks
-            if ((unsigned)(li - (s - 1)) < (unsigned)(RI + 1 - 2 * s)) {
-                const int gi = gib + li;
+            if (fmask & (1u << k)) {
                 const float ex_next = lane_next(ex[k]);
                 const int c = li * kW + lane;
                 const float ey_down = s_ey[c + kW];
                 float av, bv;
                 divide2<FAST>(ex_next - ex[k], divy, ey_down - ey[k], divx, av, bv);
                 const float F = hz[k] + dh * (av - bv);
-                if ((unsigned)(gi - 1) < (unsigned)max(nx - 2, 0)) {
-                    const bool lane_int = gj >= 1 && gj <= ny - 2;
-                    hz[k] = lane_int ? F : hz[k];
-                    if (lane_int) s_hz[c] = F;
-                }
+                const bool lane_int = gj >= 1 && gj <= ny - 2;
+                hz[k] = lane_int ? F : hz[k];
+                if (lane_int) s_hz[c] = F;
             }
         }
         __syncthreads();
@@ -412,17 +422,21 @@ k_tb(wf_tb_args a) {
             const int tt = n - R.trace_n0;
             if (tt >= 0 && tt < R.trace_len) R.traces[s_det[tid] * R.trace_stride + tt] = s_hz[s_det_cell[tid]];
         }
+        // rows this step's Ampere updates: s <= li < RI-s and 0 <= gi < nx; Ey
+        // only on rows with gi >= 1
+        unsigned int amask = row_bits<NW, NR>(w, max(s, -gib), min(RI - s, nx - gib));
+        unsigned int eymask = row_bits<NW, NR>(w, 1 - gib, RI);
+        asm volatile("" : "+s"(amask), "+s"(eymask));
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int li = w + NW * k;
-            if ((unsigned)(li - s) < (unsigned)(RI - 2 * s)) {  // s <= li < RI-s
-                const int gi = gib + li;
+            if (amask & (1u << k)) {
                 const float hz_left = lane_prev(hz[k]);
                 const int c = li * kW + lane;
                 const float hz_up = s_hz[c - kW];
                 float gyv, gxv;
                 divide2<FAST>(hz[k] - hz_left, divy, hz[k] - hz_up, divx, gyv, gxv);
-                if ((unsigned)gi < (unsigned)nx) {
+                {
                     float nex, ney;
                     if (MAT) {
                         const float t1 = ca[k] * ex[k];
@@ -438,7 +452,7 @@ k_tb(wf_tb_args a) {
                         ney = ey[k] - u2;
                     }
                     ex[k] = (col_in && gj >= 1) ? nex : ex[k];
-                    if (gi >= 1) ey[k] = col_in ? ney : ey[k];
+                    if (eymask & (1u << k)) ey[k] = col_in ? ney : ey[k];
                     s_ey[c] = ey[k];
                 }
             }
