@@ -272,7 +272,8 @@ int wf_fdtd2d_judge(const uint32_t *maxabs, int32_t n_runs, double threshold, in
  * serves the graph-captured form of the run loop (src/core/fdtd2d.py:328-329). */
 int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, hipStream_t stream);
 
-/* Supported tile_i values for wf_tb_args, written into out as (tile_i, 0)
+/* Supported tile_i values for wf_tb_args (16 and 32 run 4-wavefront
+ * workgroups, 48 and 64 run 8-wavefront ones), written into out as (tile_i, 0)
  * pairs (host, 2*count int32); returns count (<= cap). */
 int wf_tb_tile_shapes(int32_t *out, int32_t cap);
 
