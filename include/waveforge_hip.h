@@ -313,6 +313,12 @@ typedef struct wf_adj_args {
     float dh, dx, dy, cmx, cmy;
     int32_t n_steps;                 /* adjoint steps in this call; 0 or 1: one */
     int64_t hist_stride;             /* floats between the state slots of consecutive steps */
+    const int32_t *rx_first;         /* NULL, or (nx, ny): -1, or the position in rx_order of the
+                                        first receiver at that cell */
+    const int32_t *rx_order;         /* NULL, or the n_rx receiver indices ordered by cell (receivers
+                                        of one cell in index order); with both, a cell's residuals
+                                        are added in that order and all steps of a call after the
+                                        first record inside the previous step's last kernel */
 } wf_adj_args;
 
 int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream);

@@ -117,7 +117,8 @@ class WfAdjArgs(ctypes.Structure):
                                    "ley2", "lhz2", "lp", "gca", "gcb", "dl_dd", "rx_cell")] + \
         [(n, _i64) for n in ("run_stride", "state_stride", "dd_run_stride", "dd_rx_stride")] + \
         [(n, _i32) for n in ("nx", "ny", "n_runs", "n_rx", "step")] + \
-        [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("n_steps", _i32), ("hist_stride", _i64)]
+        [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("n_steps", _i32), ("hist_stride", _i64)] + \
+        [("rx_first", ctypes.c_void_p), ("rx_order", ctypes.c_void_p)]
 
 
 class WfInj2dF64(ctypes.Structure):

@@ -49,6 +49,7 @@ class TraceProblem:
     n_steps: int
     checkpoint: int = 100
     depth: int = 6
+    receiver_map: bool = True   # False: one record launch per adjoint step (atomic adds)
 
     def __post_init__(self):
         self.device = self.wave.device
@@ -56,7 +57,16 @@ class TraceProblem:
             raise RuntimeError("waveforge_amd: the adjoint runs on the ROCm GPU; there is no CPU path")
         _native.load()
         self.coef = StepCoefficients.from_grid(self.dt, self.dx, self.dx)
-        self.rx_cells = torch.tensor([i * self.ny + j for i, j in self.rx], dtype=torch.int32, device=self.device)
+        cells = [i * self.ny + j for i, j in self.rx]
+        self.rx_cells = torch.tensor(cells, dtype=torch.int32, device=self.device)
+        # receiver map: receivers ordered by cell (index order within a cell) and,
+        # per cell, the position of its first receiver in that order (-1: none)
+        order = sorted(range(len(cells)), key=lambda k: (cells[k], k))
+        first = torch.full((self.nx * self.ny,), -1, dtype=torch.int32)
+        for p in range(len(order) - 1, -1, -1):
+            first[cells[order[p]]] = p
+        self.rx_order = torch.tensor(order, dtype=torch.int32, device=self.device)
+        self.rx_first = first.to(self.device)
 
     # ------------------------------------------------------------------
     def forward(self, ca: torch.Tensor, cb: torch.Tensor):
@@ -186,6 +196,8 @@ class TraceProblem:
         a.lex2, a.ley2, a.lhz2, a.lp = lex2.data_ptr(), ley2.data_ptr(), lhz2.data_ptr(), lp.data_ptr()
         a.gca, a.gcb = gca.data_ptr(), gcb.data_ptr()
         a.dl_dd, a.rx_cell = dl_dd.data_ptr(), self.rx_cells.data_ptr()
+        if len(self.rx) and self.receiver_map:
+            a.rx_first, a.rx_order = self.rx_first.data_ptr(), self.rx_order.data_ptr()
         a.run_stride = nx * ny
         a.state_stride = 3 * nx * ny
         a.dd_run_stride, a.dd_rx_stride = dl_dd.stride(0), dl_dd.stride(1)

@@ -10,7 +10,8 @@
 // the adjoint state lam = (lEx, lEy, lHz) of u_{n+1} and yields the adjoint of
 // u_n, accumulating dL/dCa = sum_n lE_{n+1} * E_n and dL/dCb = sum_n lE_{n+1} * curl_n:
 //
-//   k_adj_record  lHz[rx] += dL/dd_n[rx]
+//   k_adj_record  lHz[rx] += dL/dd_n[rx] (with a receiver map, only for the
+//                 first step of a call; later steps add theirs in k_adj_h)
 //   k_adj_e       transposed Ampere (S9, S8) + coefficient gradients;
 //                 lHz gathers the four Ampere neighbours (out of place)
 //   k_adj_h       transposed Mur (S7..S4) and Faraday (S3) gathered per
@@ -28,13 +29,30 @@ namespace {
 constexpr int kBX = 64;
 constexpr int kBY = 4;
 
+__device__ __forceinline__ float residual(const wf_adj_args &a, int r, int k, int step) {
+    return a.dl_dd[(int64_t)r * a.dd_run_stride + (int64_t)k * a.dd_rx_stride + step];
+}
+
+// lHz at one cell plus the residuals of the receivers there, in receiver order
+// (rx_order lists them by cell, rx_first[c] is the first position of cell c).
+__device__ __forceinline__ float add_residuals(const wf_adj_args &a, int r, int64_t c, int p, int step, float v) {
+    for (; p < a.n_rx && a.rx_cell[a.rx_order[p]] == c; ++p) v = v + residual(a, r, a.rx_order[p], step);
+    return v;
+}
+
 __global__ void k_adj_record(wf_adj_args a) {
     const int r = blockIdx.x;
-    for (int k = threadIdx.x; k < a.n_rx; k += blockDim.x) {
-        const int64_t off = (int64_t)r * a.run_stride;
-        const float g = a.dl_dd[(int64_t)r * a.dd_run_stride + (int64_t)k * a.dd_rx_stride + a.step];
-        atomicAdd(&a.lhz[off + a.rx_cell[k]], g);  // receivers may share a cell
+    const int64_t off = (int64_t)r * a.run_stride;
+    if (a.rx_first) {  // one thread per receiver cell, its receivers in order
+        for (int p = threadIdx.x; p < a.n_rx; p += blockDim.x) {
+            const int c = a.rx_cell[a.rx_order[p]];
+            if (p > 0 && a.rx_cell[a.rx_order[p - 1]] == c) continue;
+            a.lhz[off + c] = add_residuals(a, r, c, p, a.step, a.lhz[off + c]);
+        }
+        return;
     }
+    for (int k = threadIdx.x; k < a.n_rx; k += blockDim.x)
+        atomicAdd(&a.lhz[off + a.rx_cell[k]], residual(a, r, k, a.step));  // receivers may share a cell
 }
 
 __global__ void __launch_bounds__(kBX * kBY) k_adj_e(wf_adj_args a) {
@@ -85,7 +103,9 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_e(wf_adj_args a) {
 //   lEx  = lEx' + (Dh/dy) (F(i, j-1) - F(i, j)),  lEy = lEy' + (Dh/dx) (F(i, j) - F(i-1, j))
 // with F the post-Mur lHz' on Faraday's range [0, nx-1) x [0, ny-1), and
 // lHz = post-Mur lHz' + lP (the snapshot's transpose, S1).
-__global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a) {
+// next_step >= 0: also record the receivers of adjoint step next_step (the
+// one after this in the backward sweep) into the lHz written here.
+__global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_step) {
     const int j = blockIdx.x * kBX + threadIdx.x;
     const int i = blockIdx.y * kBY + threadIdx.y;
     const int nx = a.nx, ny = a.ny;
@@ -145,7 +165,12 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a) {
     }
     a.lex[off + c] = a.lex2[off + c] + (a.dh / a.dy) * (fl - fij);
     a.ley[off + c] = a.ley2[off + c] + (a.dh / a.dx) * (fij - fd);
-    a.lhz[off + c] = hz + pv;
+    float out = hz + pv;
+    if (next_step >= 0) {
+        const int p = a.rx_first[c];
+        if (p >= 0) out = add_residuals(a, blockIdx.z, c, p, next_step, out);
+    }
+    a.lhz[off + c] = out;
 }
 
 }  // namespace
@@ -158,7 +183,7 @@ extern "C" int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream) {
     }
     const wf_adj_args &a = *args;
     if (a.nx < 4 || a.ny < 4 || a.n_runs < 1 || a.n_runs > 65535 || a.n_rx < 0 ||
-        (a.n_rx > 0 && (!a.rx_cell || !a.dl_dd))) {
+        (a.n_rx > 0 && (!a.rx_cell || !a.dl_dd)) || (a.rx_first != nullptr) != (a.rx_order != nullptr)) {
         wf::set_error("wf_adjoint_step: bad sizes");
         return WF_ERR_ARG;
     }
@@ -175,9 +200,13 @@ extern "C" int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream) {
         b.ex_prev = a.ex_prev - back;
         b.ey_prev = a.ey_prev - back;
         b.hz_next = a.hz_next - back;
-        if (b.n_rx > 0) hipLaunchKernelGGL(k_adj_record, dim3(b.n_runs), dim3(64), 0, stream, b);
+        // with the receiver map, only the call's first step records on its
+        // own; every later step's record rides in the k_adj_h before it
+        const bool fold = b.n_rx > 0 && b.rx_first != nullptr;
+        if (b.n_rx > 0 && (t == 0 || !fold)) hipLaunchKernelGGL(k_adj_record, dim3(b.n_runs), dim3(64), 0, stream, b);
         hipLaunchKernelGGL(k_adj_e, g, dim3(kBX, kBY), 0, stream, b);
-        hipLaunchKernelGGL(k_adj_h, g, dim3(kBX, kBY), 0, stream, b);
+        const int next = fold && t + 1 < n_steps ? b.step - 1 : -1;
+        hipLaunchKernelGGL(k_adj_h, g, dim3(kBX, kBY), 0, stream, b, next);
     }
     return wf::check_launch("wf_adjoint_step");
 }
