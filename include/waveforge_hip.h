@@ -141,8 +141,8 @@ typedef struct wf_tb_args {
     int64_t hist_stride;       /* 0, or floats between consecutive state slots: the launch
                                   then writes the state after each of its `depth` steps s
                                   to the written state + (s-1)*hist_stride instead of only
-                                  the last one (materials, IEEE division, tile_i 32, no
-                                  maxabs_out or stop); the adjoint's recompute of a
+                                  the last one (materials, IEEE division, tile_i 32 or 48,
+                                  no maxabs_out or stop); the adjoint's recompute of a
                                   segment's state history */
 } wf_tb_args;
 

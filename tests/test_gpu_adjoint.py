@@ -120,3 +120,30 @@ def test_receiver_map_folds_records_bit_for_bit(device):
     prob.receiver_map = False
     _, ge0, gs0 = misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device))
     assert torch.equal(ge1, ge0) and torch.equal(gs1, gs0)
+
+
+@pytest.mark.parametrize("tile,depth", [(32, 6), (48, 8), (48, 3)])
+def test_recompute_history_equals_oracle_states(device, tile, depth):
+    """The recompute's state-history launches write the state after every one
+    of their steps, bit for bit as the fp32 oracle steps from the same start."""
+    from cases import assert_bitwise
+    from waveforge_amd.adjoint import coefficients
+
+    prob, wave, dt = _setup(device, nx=70, ny=66, steps=60, ck=30, tx=((6, 5), (50, 40)), rx=((3, 20),))
+    prob.recompute_tile_rows, prob.recompute_depth = tile, depth
+    eps, sig = _media(prob.nx, prob.ny, 2)
+    ca64, cb64 = coefficients(eps.double(), sig.double(), dt)
+    ca, cb = ca64.float().contiguous(), cb64.float().contiguous()
+    _, ckpts = prob.forward(ca.to(device), cb.to(device))
+    s0, L = 30, 30
+    H = prob._recompute(prob._history(ca.to(device), cb.to(device)), ckpts[1], s0, L)
+    torch.cuda.synchronize()
+    for r, (i, j) in enumerate(prob.tx):
+        src = OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0]))
+        o = OracleFDTD2D(prob.nx, prob.ny, 1e-3, 1e-3, dt, Ca=ca.unsqueeze(2), Cb=cb.unsqueeze(2), sources=[src],
+                         n_check=10 ** 9)
+        o.run(s0)
+        for k in range(1, L + 1):
+            o.step()
+            for ci, name in enumerate(("Ex", "Ey", "Hz")):
+                assert_bitwise(H[k, r, ci], o.fields[name], f"run {r} step {s0 + k} {name}")

@@ -140,8 +140,8 @@ class TraceProblem:
         table = torch.from_numpy(np.ascontiguousarray(tbl).reshape(-1)).to(self.device)
         return H, table
 
-    recompute_tile_rows = 32  # the state-history launches exist for 32-row tiles
-    recompute_depth = 6       # steps per recompute launch (each step's state is written)
+    recompute_tile_rows = 48  # the state-history launches exist for 32- and 48-row tiles
+    recompute_depth = 8       # steps per recompute launch (each step's state is written)
 
     def _tile_list(self, depth: int) -> torch.Tensor:
         """(run, tile) pairs of every run for ``depth``-step launches (cached)."""

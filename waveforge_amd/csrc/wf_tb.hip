@@ -59,10 +59,10 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d", a.tile_i);
         return WF_ERR_UNSUPPORTED;
     }
-    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div || a.tile_i != 32 ||
-                                                      a.maxabs_out || a.stop))) {
+    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div ||
+                                                      (a.tile_i != 32 && a.tile_i != 48) || a.maxabs_out || a.stop))) {
         wf::set_error("wf_fdtd2d_tb_advance: a state history (hist_stride %lld) needs materials, IEEE division, "
-                      "tile_i 32 and no maxabs_out or stop flags", (long long)a.hist_stride);
+                      "tile_i 32 or 48 and no maxabs_out or stop flags", (long long)a.hist_stride);
         return WF_ERR_UNSUPPORTED;
     }
     if (a.fast_div && (a.div_lo_x != 0u || a.div_lo_y != 0u || a.div_span_x > 0x7f800000u ||

@@ -556,7 +556,7 @@ WF_TB_DECLARE(wf_tb_launch_mat_fast);
 WF_TB_DECLARE(wf_tb_launch_mat_ieee);
 WF_TB_DECLARE(wf_tb_launch_free_fast);
 WF_TB_DECLARE(wf_tb_launch_free_ieee);
-WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32-row tiles only
+WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32- or 48-row tiles
 
 #define WF_TB_DEFINE(NAME, MAT, FAST)                                                   \
     WF_TB_DECLARE(NAME) {                                                               \
