@@ -174,6 +174,7 @@ k_tb(wf_tb_args a) {
     __shared__ unsigned int s_max[3];
     __shared__ float s_prow[2][kW];  // edge tiles: snapshot of rows i = 1 and i = nx-2
     __shared__ float s_pcol[2][RI];  // edge tiles: snapshot of columns j = 1 and j = ny-2
+    __shared__ float s_fcol[2][RI];  // edge tiles: snapshot of the face columns j = 0 and j = ny-1
 
     const int tid = threadIdx.x;
     const int lane = tid & (kW - 1);
@@ -343,6 +344,8 @@ k_tb(wf_tb_args a) {
                 const int li = tid - 2 * kW;
                 if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
                 if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
+                if (c1 >= 1 && c1 <= kW) s_fcol[0][li] = s_hz[li * kW + c1 - 1];
+                if (c2 >= -1 && c2 + 1 < kW) s_fcol[1][li] = s_hz[li * kW + c2 + 1];
             }
             __syncthreads();
         }
@@ -360,9 +363,11 @@ k_tb(wf_tb_args a) {
                 float av, bv;
                 divide2<FAST>(ex_next - ex[k], divy, ey_down - ey[k], divx, av, bv);
                 const float F = hz[k] + dh * (av - bv);
-                const bool lane_int = gj >= 1 && gj <= ny - 2;
-                hz[k] = lane_int ? F : hz[k];
-                if (lane_int) s_hz[c] = F;
+                // every lane takes F: on the face columns the Mur pass below
+                // overwrites it (from the snapshot kept above), and lanes
+                // outside the grid feed only each other
+                hz[k] = F;
+                s_hz[c] = F;
             }
         }
         __syncthreads();
@@ -397,11 +402,11 @@ k_tb(wf_tb_args a) {
                 const int li = tid - 2 * kW, gil = e_i0 + li;
                 if (li >= s - 1 && li < RI - s && gil >= 1 && gil <= nx - 2) {
                     if (c1 >= 1 && c1 < kW) {
-                        const float pf = s_hz[li * kW + c1 - 1];
+                        const float pf = s_fcol[0][li];
                         s_hz[li * kW + c1 - 1] = s_pcol[0][li] + cy * (s_hz[li * kW + c1] - pf);
                     }
                     if (c2 >= 0 && c2 + 1 < kW) {
-                        const float pf = s_hz[li * kW + c2 + 1];
+                        const float pf = s_fcol[1][li];
                         s_hz[li * kW + c2 + 1] = s_pcol[1][li] + cy * (s_hz[li * kW + c2] - pf);
                     }
                 }
