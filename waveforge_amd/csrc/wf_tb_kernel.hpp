@@ -456,8 +456,10 @@ k_tb(wf_tb_args a) {
                         const float u2 = de * gxv;
                         ney = ey[k] - u2;
                     }
-                    ex[k] = (col_in && gj >= 1) ? nex : ex[k];
-                    if (eymask & (1u << k)) ey[k] = col_in ? ney : ey[k];
+                    // lanes outside the grid update too: they feed only each
+                    // other and the discarded Faraday values of the face columns
+                    ex[k] = gj >= 1 ? nex : ex[k];
+                    if (eymask & (1u << k)) ey[k] = ney;
                     s_ey[c] = ey[k];
                 }
             }
