@@ -73,13 +73,14 @@ def test_batched_runs_each_equal_standalone_oracle(device, depth, tile):
         assert_bitwise(br.traces(k), o.trace_array(), f"run {k} traces")
 
 
-def test_diverging_run_is_isolated(device):
+@pytest.mark.parametrize("tile", [32, 48])
+def test_diverging_run_is_isolated(device, tile):
     from waveforge_amd.mimo import BatchedRuns
 
     steps = 60
     dt = stable_dt(DX, DX)
     specs, oracles = _specs(device, steps, dt, sizes=((30, 30), (40, 35), (25, 50)), amps=(1e-3, 1e6, 1e-3))
-    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=5, n_check=10,
+    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=5, tile_rows=tile, n_check=10,
                      stability_threshold=10.0)
     br.advance(steps)
     st = br.status()
