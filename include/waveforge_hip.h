@@ -296,9 +296,15 @@ int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, h
  * fields before step n, hz_next the Hz after it. Sources are affine and drop
  * out. With n_steps > 1 the call runs steps step, step-1, ..., step-n_steps+1
  * in turn, the state pointers moving back by hist_stride floats per step (a
- * segment of a state history laid out slot after slot). No reference code counterpart: the reference's differentiable-FDTD
- * plan is design prose (docs/architecture/DIFFERENTIABLE_PHYSICS.md:33); the
- * forward it differentiates is FDTD2D.step (src/core/fdtd2d.py:187-266). */
+ * segment of a state history laid out slot after slot). Each step is one
+ * launch that reads the adjoint state from one set and writes the other
+ * (lex2/ley2/lhz2 are the second set, overwritten); the state is back in
+ * (lex, ley, lhz) when the call returns. No reference code counterpart: the
+ * reference's differentiable-FDTD plan is design prose
+ * (docs/architecture/DIFFERENTIABLE_PHYSICS.md:33); the forward it
+ * differentiates is FDTD2D.step (src/core/fdtd2d.py:187-266). */
+#define WF_ADJ_TWO_PASS 1
+
 typedef struct wf_adj_args {
     const float *ca, *cb;            /* (nx, ny), shared by all runs */
     const float *ex_prev, *ey_prev;  /* run r at r*state_stride */
@@ -319,6 +325,9 @@ typedef struct wf_adj_args {
                                         of one cell in index order); with both, a cell's residuals
                                         are added in that order and all steps of a call after the
                                         first record inside the previous step's last kernel */
+    int32_t flags;                   /* WF_ADJ_TWO_PASS: two launches per step (k_adj_e, k_adj_h)
+                                        instead of one; the results are the same bits */
+    int32_t pad2;
 } wf_adj_args;
 
 int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream);

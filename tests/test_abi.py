@@ -35,6 +35,7 @@ def test_header_constants_match_binding():
     text = HEADER.read_text()
     defs = dict(re.findall(r"^#define (WF_\w+) \(?(-?\d+)\)?", text, flags=re.M))
     assert int(defs["WF_RUN3D_SNAP_FROM_STATE"]) == _native.WF_RUN3D_SNAP_FROM_STATE
+    assert int(defs["WF_ADJ_TWO_PASS"]) == _native.WF_ADJ_TWO_PASS
     for name, code in _native.WF_COMP3.items():
         assert int(defs[f"WF_COMP_{name.upper()}"]) == code
 
@@ -83,7 +84,7 @@ int main(void) {
   printf("wf_adj_args size %zu\n", sizeof(wf_adj_args));
   O(wf_adj_args, ca) O(wf_adj_args, hz_next) O(wf_adj_args, lp) O(wf_adj_args, gcb) O(wf_adj_args, dl_dd)
   O(wf_adj_args, rx_cell) O(wf_adj_args, run_stride) O(wf_adj_args, dd_rx_stride) O(wf_adj_args, nx)
-  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, n_steps) O(wf_adj_args, hist_stride) O(wf_adj_args, rx_first) O(wf_adj_args, rx_order)
+  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, n_steps) O(wf_adj_args, hist_stride) O(wf_adj_args, rx_first) O(wf_adj_args, rx_order) O(wf_adj_args, flags)
   printf("wf_inj2d_f64 size %zu\nwf_run2d_f64 size %zu\nwf_f64_args size %zu\n",
          sizeof(wf_inj2d_f64), sizeof(wf_run2d_f64), sizeof(wf_f64_args));
   O(wf_inj2d_f64, wave) O(wf_inj2d_f64, cell) O(wf_inj2d_f64, amp)

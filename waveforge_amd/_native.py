@@ -16,7 +16,8 @@ ABI_VERSION = 1
 
 WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 WF_COMP3 = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
-WF_RUN3D_SNAP_FROM_STATE = 1  # wf_run3d.flags: the Mur snapshot equals the read state
+WF_RUN3D_SNAP_FROM_STATE = 1
+WF_ADJ_TWO_PASS = 1  # wf_run3d.flags: the Mur snapshot equals the read state
 TB_MAX_DEPTH = 8
 
 # Every symbol the header declares; tests check the built library exports them.
@@ -118,7 +119,7 @@ class WfAdjArgs(ctypes.Structure):
         [(n, _i64) for n in ("run_stride", "state_stride", "dd_run_stride", "dd_rx_stride")] + \
         [(n, _i32) for n in ("nx", "ny", "n_runs", "n_rx", "step")] + \
         [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("n_steps", _i32), ("hist_stride", _i64)] + \
-        [("rx_first", ctypes.c_void_p), ("rx_order", ctypes.c_void_p)]
+        [("rx_first", ctypes.c_void_p), ("rx_order", ctypes.c_void_p), ("flags", _i32), ("pad2", _i32)]
 
 
 class WfInj2dF64(ctypes.Structure):

@@ -50,6 +50,7 @@ class TraceProblem:
     checkpoint: int = 100
     depth: int = 6
     receiver_map: bool = True   # False: one record launch per adjoint step (atomic adds)
+    two_pass: bool = False      # True: two launches per adjoint step (same results)
 
     def __post_init__(self):
         self.device = self.wave.device
@@ -198,6 +199,7 @@ class TraceProblem:
         a.dl_dd, a.rx_cell = dl_dd.data_ptr(), self.rx_cells.data_ptr()
         if len(self.rx) and self.receiver_map:
             a.rx_first, a.rx_order = self.rx_first.data_ptr(), self.rx_order.data_ptr()
+        a.flags = _native.WF_ADJ_TWO_PASS if self.two_pass else 0
         a.run_stride = nx * ny
         a.state_stride = 3 * nx * ny
         a.dd_run_stride, a.dd_rx_stride = dl_dd.stride(0), dl_dd.stride(1)

@@ -11,7 +11,10 @@
 // u_n, accumulating dL/dCa = sum_n lE_{n+1} * E_n and dL/dCb = sum_n lE_{n+1} * curl_n:
 //
 //   k_adj_record  lHz[rx] += dL/dd_n[rx] (with a receiver map, only for the
-//                 first step of a call; later steps add theirs in k_adj_h)
+//                 first step of a call; later steps add theirs in the step
+//                 kernel before)
+//   k_adj_eh      the whole step in one launch (the default): the two stages
+//                 below over a tile and a ring around it, through LDS
 //   k_adj_e       transposed Ampere (S9, S8) + coefficient gradients;
 //                 lHz gathers the four Ampere neighbours (out of place)
 //   k_adj_h       transposed Mur (S7..S4) and Faraday (S3) gathered per
@@ -95,26 +98,17 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_e(wf_adj_args a) {
 
 // Transposed Faraday (S3) with the transposed Mur (S7..S4) folded in: the Mur
 // transpose only rewrites the two outer rows and columns of lHz' (the output
-// of k_adj_e) and accumulates the snapshot's adjoint lP there, with lP zero
-// beforehand, so each cell's post-Mur lHz' and lP are short expressions in at
-// most four lHz' values, formed here in the same fp32 operation order as the
-// sequential transpose: top then bottom face of every row (S7, S6), then right
-// then left face of every column (S5, S4). Then
-//   lEx  = lEx' + (Dh/dy) (F(i, j-1) - F(i, j)),  lEy = lEy' + (Dh/dx) (F(i, j) - F(i-1, j))
-// with F the post-Mur lHz' on Faraday's range [0, nx-1) x [0, ny-1), and
-// lHz = post-Mur lHz' + lP (the snapshot's transpose, S1).
-// next_step >= 0: also record the receivers of adjoint step next_step (the
-// one after this in the backward sweep) into the lHz written here.
-__global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_step) {
-    const int j = blockIdx.x * kBX + threadIdx.x;
-    const int i = blockIdx.y * kBY + threadIdx.y;
-    const int nx = a.nx, ny = a.ny;
-    if (i >= nx || j >= ny) return;
-    const int64_t off = (int64_t)blockIdx.z * a.run_stride;
-    const int64_t c = (int64_t)i * ny + j;
-    const float *h = a.lhz2 + off;
-    const float cx = a.cmx, cy = a.cmy;
-    auto H = [&](int ii, int jj) -> float { return h[(int64_t)ii * ny + jj]; };
+// of the transposed Ampere) and accumulates the snapshot's adjoint lP there,
+// with lP zero beforehand, so each cell's post-Mur lHz' and lP are short
+// expressions in at most four lHz' values, formed here in the same fp32
+// operation order as the sequential transpose: top then bottom face of every
+// row (S7, S6), then right then left face of every column (S5, S4). Returns
+// F(i, j), F(i, j-1), F(i-1, j) with F the post-Mur lHz' on Faraday's range
+// [0, nx-1) x [0, ny-1), the post-Mur lHz'(i, j) and lP(i, j); H(ii, jj)
+// reads lHz' and is called only within two cells of (i, j).
+template <class HF>
+__device__ __forceinline__ void mur_faraday_t(const HF &H, int i, int j, int nx, int ny, float cx, float cy,
+                                              float &fij, float &fl, float &fd, float &hz, float &pv) {
     // after the row faces (S7 then S6 transposed)
     auto H1 = [&](int ii, int jj) -> float {
         if (jj == ny - 1 || jj == 0) return 0.f;
@@ -132,12 +126,12 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_ste
     auto F = [&](int ii, int jj) -> float {
         return (ii >= 0 && jj >= 0 && ii <= nx - 2 && jj <= ny - 2) ? H2(ii, jj) : 0.f;
     };
-    float fij, fl, fd, hz, pv = 0.f;
+    pv = 0.f;
     const bool mid_rows = i >= 3 && i <= nx - 3;  // (i, j), (i-1, j) on no column face
     if (mid_rows && j >= 3 && j <= ny - 3) {      // no face term reaches (i, j), (i, j-1), (i-1, j)
-        fij = h[c];
-        fl = h[c - 1];
-        fd = h[c - ny];
+        fij = H(i, j);
+        fl = H(i, j - 1);
+        fd = H(i - 1, j);
         hz = fij;
     } else if (mid_rows) {  // row faces only: cheap, for the lanes at the ends of every row
         auto F1 = [&](int ii, int jj) -> float { return (jj >= 0 && jj <= ny - 2) ? H1(ii, jj) : 0.f; };
@@ -163,6 +157,21 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_ste
         if (i == 1) pv = pv + H1(0, j);
         if (i == 0) pv = pv - cx * H1(0, j);
     }
+}
+
+// next_step >= 0: also record the receivers of adjoint step next_step (the
+// one after this in the backward sweep) into the lHz written here.
+__global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_step) {
+    const int j = blockIdx.x * kBX + threadIdx.x;
+    const int i = blockIdx.y * kBY + threadIdx.y;
+    const int nx = a.nx, ny = a.ny;
+    if (i >= nx || j >= ny) return;
+    const int64_t off = (int64_t)blockIdx.z * a.run_stride;
+    const int64_t c = (int64_t)i * ny + j;
+    const float *h = a.lhz2 + off;
+    auto H = [&](int ii, int jj) -> float { return h[(int64_t)ii * ny + jj]; };
+    float fij, fl, fd, hz, pv;
+    mur_faraday_t(H, i, j, nx, ny, a.cmx, a.cmy, fij, fl, fd, hz, pv);
     a.lex[off + c] = a.lex2[off + c] + (a.dh / a.dy) * (fl - fij);
     a.ley[off + c] = a.ley2[off + c] + (a.dh / a.dx) * (fij - fd);
     float out = hz + pv;
@@ -171,6 +180,152 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_ste
         if (p >= 0) out = add_residuals(a, blockIdx.z, c, p, next_step, out);
     }
     a.lhz[off + c] = out;
+}
+
+// One whole adjoint step in one launch: the transposed Ampere (k_adj_e's
+// arithmetic, in its order) over the tile and a two-cell ring around it into
+// LDS, then the transposed Mur and Faraday (k_adj_h's) over the tile from LDS.
+// Reads the adjoint state from `in` and writes it to `out` (the ring reads
+// cells that neighbouring tiles rewrite, so the step cannot be in place); the
+// coefficient gradients accumulate as in k_adj_e. Saves k_adj_e's three
+// intermediate fields: their write and k_adj_h's read of them.
+constexpr int kEhRows = 16;               // tile rows
+constexpr int kEhCols = kBX - 4;          // tile columns (lanes 2..61 of the 64-lane region)
+constexpr int kEhRegion = kEhRows + 4;    // region rows
+constexpr int kEhPerWave = (kEhRegion + kBY - 1) / kBY;
+
+struct AdjState {
+    const float *lex, *ley, *lhz;
+    float *olex, *oley, *olhz;
+};
+
+__global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st, int next_step) {
+    __shared__ float s_h[kEhRegion][kBX];
+    const int q = threadIdx.x, w = threadIdx.y;
+    const int nx = a.nx, ny = a.ny;
+    const int i0 = blockIdx.y * kEhRows - 2, j0 = blockIdx.x * kEhCols - 2;  // region origin
+    const int j = j0 + q;
+    const int64_t off = (int64_t)blockIdx.z * a.run_stride;
+    const int64_t soff = (int64_t)blockIdx.z * a.state_stride;
+    const float *hz1 = a.hz_next + soff;
+    const bool core_col = q >= 2 && q < 2 + kEhCols && j < ny;
+    // The transposed Ampere's neighbour terms are the neighbours' own terms:
+    // vx = (Cb lEx)/dy of cell (i, j+1) and vy = (Cb lEy)/dx of (i+1, j).
+    // Each cell forms its vx, vy once; vx reaches the cell to its left by a
+    // lane shift, vy the cell above through LDS. Loads come first (the
+    // gradient stores below could alias them as far as the compiler knows).
+    float lex[kEhPerWave], ley[kEhPerWave], lhz[kEhPerWave], ca[kEhPerWave];
+    float vx[kEhPerWave], vy[kEhPerWave];
+#pragma unroll
+    for (int t = 0; t < kEhPerWave; ++t) {
+        const int p = w + kBY * t;
+        const int i = i0 + p;
+        lex[t] = ley[t] = lhz[t] = ca[t] = 0.f;
+        float cb = 0.f;
+        if (p < kEhRegion && i >= 0 && i < nx && j >= 0 && j < ny) {
+            const int64_t c = (int64_t)i * ny + j;
+            lex[t] = st.lex[off + c];
+            ley[t] = st.ley[off + c];
+            lhz[t] = st.lhz[off + c];
+            ca[t] = a.ca[c];
+            cb = a.cb[c];
+        }
+        vx[t] = cb * lex[t] / a.dy;
+        vy[t] = cb * ley[t] / a.dx;
+        if (p < kEhRegion) s_h[p][q] = vy[t];
+    }
+    // the inputs of the tile cells' coefficient gradients, in flight with the
+    // loads above (the sums and stores come after the stages below)
+    float h0[kEhPerWave], hl[kEhPerWave], hu[kEhPerWave], exp_[kEhPerWave], eyp[kEhPerWave];
+    double g0[kEhPerWave], g1[kEhPerWave];
+#pragma unroll
+    for (int t = 0; t < kEhPerWave; ++t) {
+        const int p = w + kBY * t;
+        const int i = i0 + p;
+        h0[t] = hl[t] = hu[t] = exp_[t] = eyp[t] = 0.f;
+        g0[t] = g1[t] = 0.0;
+        if (core_col && p >= 2 && p < 2 + kEhRows && i < nx) {
+            const int64_t c = (int64_t)i * ny + j;
+            h0[t] = hz1[c];
+            if (j >= 1) {
+                hl[t] = hz1[c - 1];
+                exp_[t] = a.ex_prev[soff + c];
+            }
+            if (i >= 1) {
+                hu[t] = hz1[c - ny];
+                eyp[t] = a.ey_prev[soff + c];
+            }
+            g0[t] = a.gca[off + c];
+            g1[t] = a.gcb[off + c];
+        }
+    }
+    __syncthreads();
+    float lz[kEhPerWave];
+#pragma unroll
+    for (int t = 0; t < kEhPerWave; ++t) {
+        const int p = w + kBY * t;
+        lz[t] = lhz[t];
+        const float vx_r = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(vx[t]), 0x130, 0xf, 0xf, false));
+        const float vy_d = p + 1 < kEhRegion ? s_h[p + 1][q] : 0.f;
+        const int i = i0 + p;
+        if (j >= 1) lz[t] += vx[t];
+        if (j + 1 < ny) lz[t] -= vx_r;
+        if (i >= 1) lz[t] -= vy[t];
+        if (i + 1 < nx) lz[t] += vy_d;
+    }
+    __syncthreads();
+    float lex_o[kEhPerWave], ley_o[kEhPerWave];
+#pragma unroll
+    for (int t = 0; t < kEhPerWave; ++t) {
+        const int p = w + kBY * t;
+        const int i = i0 + p;
+        lex_o[t] = ley_o[t] = 0.f;
+        if (p >= kEhRegion) continue;
+        float out = 0.f;
+        if (i >= 0 && i < nx && j >= 0 && j < ny) {
+            const int64_t c = (int64_t)i * ny + j;
+            out = lz[t];
+            if (core_col && p >= 2 && p < 2 + kEhRows) {
+                double gca = 0.0, gcb = 0.0;
+                lex_o[t] = lex[t];
+                ley_o[t] = ley[t];
+                if (j >= 1) {  // S8: Ex' = Ca Ex + Cb (Hz1[i,j] - Hz1[i,j-1]) / dy
+                    const float gy = (h0[t] - hl[t]) / a.dy;
+                    gca += (double)lex[t] * (double)exp_[t];
+                    gcb += (double)lex[t] * (double)gy;
+                    lex_o[t] = ca[t] * lex[t];
+                }
+                if (i >= 1) {  // S9: Ey' = Ca Ey - Cb (Hz1[i,j] - Hz1[i-1,j]) / dx
+                    const float gx = (h0[t] - hu[t]) / a.dx;
+                    gca += (double)ley[t] * (double)eyp[t];
+                    gcb -= (double)ley[t] * (double)gx;
+                    ley_o[t] = ca[t] * ley[t];
+                }
+                a.gca[off + c] = g0[t] + gca;
+                a.gcb[off + c] = g1[t] + gcb;
+            }
+        }
+        s_h[p][q] = out;
+    }
+    __syncthreads();
+    auto H = [&](int ii, int jj) -> float { return s_h[ii - i0][jj - j0]; };
+#pragma unroll
+    for (int t = 0; t < kEhPerWave; ++t) {
+        const int p = w + kBY * t;
+        const int i = i0 + p;
+        if (p < 2 || p >= 2 + kEhRows || i >= nx || !core_col) continue;
+        const int64_t c = (int64_t)i * ny + j;
+        float fij, fl, fd, hz, pv;
+        mur_faraday_t(H, i, j, nx, ny, a.cmx, a.cmy, fij, fl, fd, hz, pv);
+        st.olex[off + c] = lex_o[t] + (a.dh / a.dy) * (fl - fij);
+        st.oley[off + c] = ley_o[t] + (a.dh / a.dx) * (fij - fd);
+        float o = hz + pv;
+        if (next_step >= 0) {
+            const int r0 = a.rx_first[c];
+            if (r0 >= 0) o = add_residuals(a, blockIdx.z, c, r0, next_step, o);
+        }
+        st.olhz[off + c] = o;
+    }
 }
 
 }  // namespace
@@ -192,8 +347,38 @@ extern "C" int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream) {
         wf::set_error("wf_adjoint_step: %d steps need a positive hist_stride and step >= n_steps-1", n_steps);
         return WF_ERR_ARG;
     }
-    const dim3 g((a.ny + kBX - 1) / kBX, (a.nx + kBY - 1) / kBY, a.n_runs);
     wf_adj_args b = a;
+    if (!(a.flags & WF_ADJ_TWO_PASS)) {
+        // one launch per step, the adjoint state alternating between the
+        // primary fields (lex, ley, lhz) and the second set (lex2, ley2, lhz2)
+        const dim3 ge((a.ny + kEhCols - 1) / kEhCols, (a.nx + kEhRows - 1) / kEhRows, a.n_runs);
+        const AdjState pri{a.lex, a.ley, a.lhz, a.lex2, a.ley2, a.lhz2};
+        const AdjState sec{a.lex2, a.ley2, a.lhz2, a.lex, a.ley, a.lhz};
+        for (int t = 0; t < n_steps; ++t) {
+            const int64_t back = (int64_t)t * a.hist_stride;
+            b.step = a.step - t;
+            b.ex_prev = a.ex_prev - back;
+            b.ey_prev = a.ey_prev - back;
+            b.hz_next = a.hz_next - back;
+            const bool fold = b.n_rx > 0 && b.rx_first != nullptr;
+            if (b.n_rx > 0 && (t == 0 || !fold)) {
+                wf_adj_args r = b;
+                if (t & 1) r.lhz = a.lhz2;  // the state this step reads
+                hipLaunchKernelGGL(k_adj_record, dim3(b.n_runs), dim3(64), 0, stream, r);
+            }
+            const int next = fold && t + 1 < n_steps ? b.step - 1 : -1;
+            hipLaunchKernelGGL(k_adj_eh, ge, dim3(kBX, kBY), 0, stream, b, (t & 1) ? sec : pri, next);
+        }
+        const int rc = wf::check_launch("wf_adjoint_step");
+        if (rc || !(n_steps & 1)) return rc;
+        // the state ended in the second set
+        const size_t bytes = (size_t)a.n_runs * (size_t)a.run_stride * sizeof(float);
+        hipError_t e = hipMemcpyAsync(a.lex, a.lex2, bytes, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(a.ley, a.ley2, bytes, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(a.lhz, a.lhz2, bytes, hipMemcpyDeviceToDevice, stream);
+        return wf::check_hip(e, "wf_adjoint_step");
+    }
+    const dim3 g((a.ny + kBX - 1) / kBX, (a.nx + kBY - 1) / kBY, a.n_runs);
     for (int t = 0; t < n_steps; ++t) {
         const int64_t back = (int64_t)t * a.hist_stride;
         b.step = a.step - t;
