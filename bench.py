@@ -63,6 +63,9 @@ def parse():
     p.add_argument("--size3d", type=int, default=256)
     p.add_argument("--steps3d", type=int, default=200)
     p.add_argument("--fd-cells", type=int, default=32, help="seeded cells of the central-difference check")
+    p.add_argument("--serving-full", action="store_true",
+                   help="run only the serving workload at its full size (2000 Poisson requests, grids 128..1024) "
+                        "on the continuous server, with capacity and the plain path sampled; prints one JSON line")
     p.add_argument("--copy-probe", action="store_true",
                    help="run only a plain 512 MiB device copy (3 dispatches): the reference kernel of known "
                         "traffic used to scale rocprofv3 FETCH_SIZE/WRITE_SIZE readings; prints its rate")
@@ -501,10 +504,69 @@ def adjoint_suite(dev, n_fd: int) -> dict:
     return out
 
 
+def serving_full() -> int:
+    """The serving workload at its full size: 2000 imaging requests (head
+    phantoms, grid log-uniform in 128..1024, E in {8, 16, 32}, steps = 8 x grid
+    diagonal, DAS each), Poisson arrivals at 80 % of the continuous server's
+    capacity, served in real time. Capacity comes from a 64-request burst of
+    the same mix (all at t = 0); the plain one-run-per-launch path's capacity
+    from the first 12 of those (its full stream would take hours)."""
+    import torch
+
+    from waveforge_amd import serving, workloads
+
+    dev = torch.device("cuda:0")
+    dt = workloads.imaging_dt()
+
+    def log(msg):
+        print(f"[serving-full] {msg}", file=sys.stderr, flush=True)
+
+    def server():
+        return serving.ImagingServer(dt=dt, device=dev, hbm_budget=192e9, quantum=96)
+
+    warm = serving.poisson_requests(2, 100.0, n_min=64, n_max=96, seed=1)
+    server().serve(warm, realtime=False)
+    serving.serve_plain(serving.poisson_requests(1, 100.0, n_min=64, n_max=96, seed=1), dt=dt, device=dev,
+                        realtime=False)
+    torch.cuda.synchronize()
+    burst = serving.poisson_requests(64, 1.0, n_min=128, n_max=1024, seed=11)
+    for q in burst:
+        q.arrival = 0.0
+    log("capacity burst: 64 requests")
+    cap = server().serve(burst, realtime=False, progress=lambda d, a, t: log(f"burst {d} done, {t:.0f} s"))
+    log(f"capacity {cap['images_per_s']:.2f} images/s, {cap['mcell_updates_per_s'] / 1e3:.1f} G cell-updates/s")
+    plain = serving.poisson_requests(64, 1.0, n_min=128, n_max=1024, seed=11)[:12]
+    for q in plain:
+        q.arrival = 0.0
+    log("plain path: 12 requests of the burst")
+    cap_p = serving.serve_plain(plain, dt=dt, device=dev, realtime=False)
+    log(f"plain {cap_p['mcell_updates_per_s'] / 1e3:.1f} G cell-updates/s")
+    rate = 0.8 * cap["images_per_s"]
+    stream = serving.poisson_requests(2000, rate, n_min=128, n_max=1024, seed=7)
+    work = sum(q.cell_updates for q in stream)
+    log(f"stream: 2000 requests at {rate:.2f} images/s, {work:.3e} cell-updates")
+    sus = server().serve(stream, realtime=True,
+                         progress=lambda d, a, t: log(f"stream {d} done, {a} admitted, {t:.0f} s"))
+    print(json.dumps({
+        "serving_full": {
+            "workload": "2000 requests, head phantoms, grid 128..1024 log-uniform, E in {8,16,32}, "
+                        "steps = 8*diag, DAS per request, Poisson arrivals at 80 % of capacity",
+            "cell_updates": work,
+            "capacity_burst_64": cap,
+            "plain_run_per_launch_12": cap_p,
+            "offered_images_per_s": rate,
+            "sustained": sus,
+            "capacity_ratio_vs_plain": round(cap["mcell_updates_per_s"] / cap_p["mcell_updates_per_s"], 2),
+        }}))
+    return 0
+
+
 def main() -> int:
     args = parse()
     if args.copy_probe:
         return copy_probe()
+    if args.serving_full:
+        return serving_full()
     if args.gpus != 1:
         print(json.dumps({"error": "waveforge_amd runs on one GPU; multi-GPU scaling is not part of this build",
                           "n_gpus": args.gpus}))

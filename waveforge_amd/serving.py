@@ -24,7 +24,7 @@ import heapq
 import math
 import time
 from dataclasses import dataclass, field
-from typing import Optional, Sequence
+from typing import Callable, Optional, Sequence
 
 import numpy as np
 import torch
@@ -249,15 +249,21 @@ class ImagingServer:
         self.retiring = still
 
     # ------------------------------------------------------------------
-    def serve(self, requests: Sequence[ImagingRequest], *, realtime: bool = True) -> dict:
+    def serve(self, requests: Sequence[ImagingRequest], *, realtime: bool = True,
+              progress: Optional[Callable[[int, int, float], None]] = None, progress_every: float = 20.0) -> dict:
         """Serve a stream of requests (sorted by arrival). With ``realtime``
         the arrivals follow the wall clock from the call; returns latency and
-        throughput statistics."""
+        throughput statistics. ``progress(done, admitted, seconds)`` is
+        called about every ``progress_every`` seconds of a long stream."""
         pending = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         k = 0
+        last = 0.0
         while k < len(pending) or self.active or self.retiring:
+            if progress is not None and now() - last >= progress_every:
+                last = now()
+                progress(len(self.done), k, last)
             t = now()
             while k < len(pending) and (pending[k].arrival <= t or not realtime) and self.can_admit(pending[k]):
                 self.admit(pending[k], t)
