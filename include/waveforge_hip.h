@@ -92,7 +92,9 @@ const char *wf_last_error(void);
  * the callers' detector gather (examples/brain_mimo_imaging.py:196-198).
  * `runs` is a HOST array of n_runs descriptors (batch members). The caller
  * checks waveform lengths (stepping past a table end raises in the
- * reference, sources.py:882-887). */
+ * reference, sources.py:882-887). Grids need 4 <= nx, ny and nx*ny < 2^31
+ * (cells are int32 in the source and detector tables; WF_ERR_UNSUPPORTED
+ * otherwise). */
 int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps,
                    hipStream_t stream);
 

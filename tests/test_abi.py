@@ -55,6 +55,12 @@ def test_argument_validation_without_gpu():
     assert lib.wf_fdtd2d_tb_advance(ctypes.byref(a), None) == -1
     d = _native.WfDas2dArgs()
     assert lib.wf_das2d(ctypes.byref(d), None) == -1
+    # a grid beyond the int32 cell range is refused before any launch
+    r = _native.WfRun2d()
+    r.ex = r.ey = r.hz = r.pstrip = 16  # never dereferenced: validation fails first
+    r.nx = r.ny = 46341
+    rc = lib.wf_fdtd2d_step(ctypes.byref(r), 1, 1, None)
+    assert rc == -3 and b"2^31-cell index range" in lib.wf_last_error()
 
 
 LAYOUT_C = r"""

@@ -224,3 +224,17 @@ def test_default_launch_shape_by_workload():
     assert default_launch_shape(runs(*[(256, 256)] * 32)) == (32, 6)   # 36 tiles each
     assert default_launch_shape(runs(*[(256, 256)] * 64)) == (48, 8)
     assert default_launch_shape(runs((256, 256))) == (32, 6)
+
+
+def test_grids_beyond_int32_cells_are_rejected():
+    """Cells are int32 in the source and detector tables: 46341^2 > 2^31-1 is
+    refused with a clear error, 46340^2 is accepted."""
+    import pytest as _pytest
+
+    from waveforge_amd.engine import check_grid_cells
+
+    check_grid_cells(46340, 46340)
+    check_grid_cells(4, 2 ** 29 - 1)
+    for nx, ny in ((46341, 46341), (4, 2 ** 29)):
+        with _pytest.raises(ValueError, match="2\\^31-cell index range"):
+            check_grid_cells(nx, ny)

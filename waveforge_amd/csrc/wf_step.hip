@@ -229,6 +229,10 @@ __global__ void __launch_bounds__(256) k_maxabs_final(const unsigned int *__rest
 int validate_run(const wf_run2d &r) {
     if (!r.ex || !r.ey || !r.hz) { wf::set_error("run: null field pointer"); return WF_ERR_ARG; }
     if (r.nx < 4 || r.ny < 4) { wf::set_error("run: grid must be at least 4x4, got %dx%d", r.nx, r.ny); return WF_ERR_ARG; }
+    if ((int64_t)r.nx * r.ny > INT32_MAX) {  // cells are int32 in the source and detector tables
+        wf::set_error("run: %dx%d cells exceed the 2^31-cell index range of the source and detector tables", r.nx, r.ny);
+        return WF_ERR_UNSUPPORTED;
+    }
     if ((r.ca == nullptr) != (r.cb == nullptr)) { wf::set_error("run: ca and cb must both be set or both NULL"); return WF_ERR_ARG; }
     if (r.n_grp < 0 || r.n_det < 0) { wf::set_error("run: negative table size"); return WF_ERR_ARG; }
     if (r.n_grp > 0 && (!r.inj || !r.grp_start)) { wf::set_error("run: source tables missing"); return WF_ERR_ARG; }

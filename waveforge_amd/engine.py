@@ -190,6 +190,16 @@ class SourceTable:
                 self.grp_start.data_ptr() if self.grp_start is not None else 0)
 
 
+MAX_GRID_CELLS = 2 ** 31 - 1
+
+
+def check_grid_cells(nx: int, ny: int) -> None:
+    """The kernels' source and detector tables hold cells as int32 (i*ny + j)."""
+    if nx * ny > MAX_GRID_CELLS:
+        raise ValueError(f"a {nx}x{ny} grid has {nx * ny} cells, beyond the 2^31-cell index range of the "
+                         f"stepping kernels' source and detector tables")
+
+
 @dataclass
 class RunState:
     """One simulation's device buffers and scalars, as read by the kernels."""
@@ -210,6 +220,9 @@ class RunState:
     pstrip: Optional[torch.Tensor] = None
     second: Optional[tuple] = None   # (ex2, ey2, hz2)
     extra: dict = field(default_factory=dict)
+
+    def __post_init__(self) -> None:
+        check_grid_cells(self.nx, self.ny)
 
     def struct(self) -> WfRun2d:
         r = WfRun2d()
