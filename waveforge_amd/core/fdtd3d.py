@@ -33,7 +33,7 @@ import numpy as np
 import torch
 
 from .. import _native
-from ..engine import SourceTable, f32, maxabs, require_gpu_tensor
+from ..engine import SourceTable, check_grid_cells, f32, maxabs, require_gpu_tensor
 from .detectors import DetectorSet
 from .fields import FieldSet
 from .grid import YeeGrid
@@ -89,6 +89,7 @@ class FDTD3D:
             raise ValueError(
                 f"grid.Nz must be >= 4 when using 3D mode, got {grid.Nz}. Use Nz=1 for 2D-equivalent mode."
             )
+        check_grid_cells(grid.Nx * grid.Ny, grid.Nz)
         self._grid, self._fields, self._boundary, self._sources = grid, fields, boundary, sources
         self._Dh: float = grid.dt / MU0
         self._De: float = grid.dt / EPS0

@@ -610,6 +610,11 @@ int validate3(const wf_run3d &r) {
         wf::set_error("run3d: grid must be at least 4x4 with nz = 1 or nz >= 4, got %dx%dx%d", r.nx, r.ny, r.nz);
         return WF_ERR_ARG;
     }
+    if ((int64_t)r.nx * r.ny * r.nz > INT32_MAX) {  // cells are int32 in the source and detector tables
+        wf::set_error("run3d: %dx%dx%d cells exceed the 2^31-cell index range of the source and detector tables",
+                      r.nx, r.ny, r.nz);
+        return WF_ERR_UNSUPPORTED;
+    }
     if ((r.ca == nullptr) != (r.cb == nullptr)) {
         wf::set_error("run3d: ca and cb must both be set or both NULL");
         return WF_ERR_ARG;

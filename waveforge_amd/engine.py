@@ -194,9 +194,10 @@ MAX_GRID_CELLS = 2 ** 31 - 1
 
 
 def check_grid_cells(nx: int, ny: int) -> None:
-    """The kernels' source and detector tables hold cells as int32 (i*ny + j)."""
+    """The kernels' source and detector tables hold cells as int32 (i*ny + j;
+    in 3D, called with nx = Nx*Ny and ny = Nz)."""
     if nx * ny > MAX_GRID_CELLS:
-        raise ValueError(f"a {nx}x{ny} grid has {nx * ny} cells, beyond the 2^31-cell index range of the "
+        raise ValueError(f"a grid of {nx * ny} cells is beyond the 2^31-cell index range of the "
                          f"stepping kernels' source and detector tables")
 
 
