@@ -288,6 +288,28 @@ k_tb(wf_tb_args a) {
     // be a global load and wait on every step
     const int n0 = __builtin_amdgcn_readfirstlane(R.step + a.step_offset);
 
+    if (edge) {  // the first step's Mur snapshot (LDS holds the state before any source)
+            {
+                // the edge passes' values are formed inside their branches (the
+                // empty asm keeps the compiler from hoisting them out of the step
+                // loop, where they would hold scalar registers for every tile)
+                int e_i0 = gi0, e_j0 = gj0;
+                asm volatile("" : "+s"(e_i0), "+s"(e_j0));
+                const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
+                const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;  // region lanes of j = 1, ny-2
+                if (tid < kW) {
+                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * kW + tid];
+                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * kW + tid];
+                } else if (tid >= 2 * kW && tid < 2 * kW + RI) {
+                    const int li = tid - 2 * kW;
+                    if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
+                    if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
+                    if (c1 >= 1 && c1 <= kW) s_fcol[0][li] = s_hz[li * kW + c1 - 1];
+                    if (c2 >= -1 && c2 + 1 < kW) s_fcol[1][li] = s_hz[li * kW + c2 + 1];
+                }
+            }
+        __syncthreads();
+    }
 #pragma unroll 1
     for (int s = 1; s <= D; ++s) {
         const int n = n0 + s - 1;
@@ -326,29 +348,10 @@ k_tb(wf_tb_args a) {
             if (ey_src) __syncthreads();
         }
         // 2. Faraday. Row conditions are wavefront-uniform; per-lane conditions
-        // become selects. Edge tiles first keep the snapshot of the rows and
-        // columns next to the boundary (Faraday overwrites them in LDS), which
-        // the Mur pass below reads.
-        if (edge) {
-            // the edge passes' values are formed inside their branches (the
-            // empty asm keeps the compiler from hoisting them out of the step
-            // loop, where they would hold scalar registers for every tile)
-            int e_i0 = gi0, e_j0 = gj0;
-            asm volatile("" : "+s"(e_i0), "+s"(e_j0));
-            const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
-            const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;  // region lanes of j = 1, ny-2
-            if (tid < kW) {
-                if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * kW + tid];
-                if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * kW + tid];
-            } else if (tid >= 2 * kW && tid < 2 * kW + RI) {
-                const int li = tid - 2 * kW;
-                if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
-                if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
-                if (c1 >= 1 && c1 <= kW) s_fcol[0][li] = s_hz[li * kW + c1 - 1];
-                if (c2 >= -1 && c2 + 1 < kW) s_fcol[1][li] = s_hz[li * kW + c2 + 1];
-            }
-            __syncthreads();
-        }
+        // become selects. On edge tiles the snapshot of the rows and columns
+        // next to the boundary, which the Mur pass below reads, was kept in
+        // LDS before Faraday overwrites them (before the loop for the first
+        // step, after the previous step's Mur pass for the others).
         // rows this step's Faraday updates: s-1 <= li < RI-s and 1 <= gi <= nx-2,
         // one bit per row, so each row's test is one scalar bit test
         unsigned int fmask = row_bits<NW, NR>(w, max(s - 1, 1 - gib), min(RI - s, nx - 1 - gib));
@@ -420,6 +423,29 @@ k_tb(wf_tb_args a) {
             for (int k = 0; k < NR; ++k) {
                 const int li = w + NW * k;
                 if (li < RI) hz[k] = s_hz[li * kW + lane];
+            }
+            // the next step's snapshot: LDS holds this step's final Hz, which
+            // nothing rewrites before the next Faraday (Ampere and the
+            // detectors only read it; its barrier orders these copies first)
+            if (s < D)
+            {
+                // the edge passes' values are formed inside their branches (the
+                // empty asm keeps the compiler from hoisting them out of the step
+                // loop, where they would hold scalar registers for every tile)
+                int e_i0 = gi0, e_j0 = gj0;
+                asm volatile("" : "+s"(e_i0), "+s"(e_j0));
+                const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
+                const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;  // region lanes of j = 1, ny-2
+                if (tid < kW) {
+                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * kW + tid];
+                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * kW + tid];
+                } else if (tid >= 2 * kW && tid < 2 * kW + RI) {
+                    const int li = tid - 2 * kW;
+                    if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
+                    if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
+                    if (c1 >= 1 && c1 <= kW) s_fcol[0][li] = s_hz[li * kW + c1 - 1];
+                    if (c2 >= -1 && c2 + 1 < kW) s_fcol[1][li] = s_hz[li * kW + c2 + 1];
+                }
             }
         }
         // 3. Ampere + detectors.
