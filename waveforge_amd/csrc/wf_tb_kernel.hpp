@@ -1,10 +1,11 @@
 // Temporally blocked, multi-run 2D TM FDTD advance (the throughput path).
 //
-// Work decomposition. One workgroup (4 wavefronts) owns one core tile of one
-// run: TI rows x TJ = 64 - 2D columns, plus a halo of D cells per side, so
-// every row of the staged region is exactly 64 cells = one wavefront. Wave w
-// holds rows w, w+4, w+8, ... of the region in registers (Ex, Ey, Hz and, on
-// the material path, Ca/Cb), one column per lane. The workgroup advances the
+// Work decomposition. One workgroup (4 wavefronts for 16- and 32-row tiles,
+// 8 for 48 and 64 rows) owns one core tile of one run: TI rows x TJ = 64 - 2D
+// columns, plus a halo of D cells per side, so every row of the staged region
+// is exactly 64 cells = one wavefront. Of W wavefronts, wave w holds rows w,
+// w+W, w+2W, ... of the region in registers (Ex, Ey, Hz and, on the material
+// path, Ca/Cb), one column per lane. The workgroup advances the
 // region D leapfrog steps without touching HBM, shrinking the valid region by
 // one cell per side per step, then writes the core to the second state buffer
 // (ping-pong: no workgroup reads what another writes in the same launch).
@@ -21,11 +22,13 @@
 //   1. soft sources inside the region, in collection order, added to the
 //      owning lane's registers (src/core/sources.py:1060-1071); LDS keeps the
 //      pre-injection Hz, which is the Mur snapshot (src/core/boundaries.py:237);
-//   2. Faraday on interior cells (src/core/fdtd2d.py:232-235); on tiles that
-//      touch the grid boundary, a short pass after it (one thread per face cell,
-//      from the Faraday values and the snapshot rows and columns next to the
-//      boundary kept in LDS) writes the Mur faces (src/core/boundaries.py:259-274),
-//      and the face lanes reload them; interior tiles skip both;
+//   2. Faraday on the grid rows 1..nx-2 of the computed range, every lane
+//      (src/core/fdtd2d.py:232-235; values on face columns and outside the grid
+//      are discarded); on tiles that touch the grid boundary, a short pass
+//      after it (one thread per face cell, from the Faraday values and the
+//      snapshot of the face rows and columns and those next to them, kept in
+//      LDS) writes the Mur faces (src/core/boundaries.py:259-274), and the rows
+//      are reloaded from LDS; interior tiles skip both;
 //   3. Ampere (src/core/fdtd2d.py:244-257); Hz detectors owned by the core are
 //      read from LDS (examples/brain_mimo_imaging.py:196-198).
 //
@@ -347,8 +350,8 @@ k_tb(wf_tb_args a) {
             }
             if (ey_src) __syncthreads();
         }
-        // 2. Faraday. Row conditions are wavefront-uniform; per-lane conditions
-        // become selects. On edge tiles the snapshot of the rows and columns
+        // 2. Faraday. Row conditions are wavefront-uniform (one bit per row of a
+        // per-step mask). On edge tiles the snapshot of the rows and columns
         // next to the boundary, which the Mur pass below reads, was kept in
         // LDS before Faraday overwrites them (before the loop for the first
         // step, after the previous step's Mur pass for the others).
