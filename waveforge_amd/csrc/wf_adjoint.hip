@@ -185,10 +185,12 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_ste
 // One whole adjoint step in one launch: the transposed Ampere (k_adj_e's
 // arithmetic, in its order) over the tile and a two-cell ring around it into
 // LDS, then the transposed Mur and Faraday (k_adj_h's) over the tile from LDS.
-// Reads the adjoint state from `in` and writes it to `out` (the ring reads
-// cells that neighbouring tiles rewrite, so the step cannot be in place); the
-// coefficient gradients accumulate as in k_adj_e. Saves k_adj_e's three
-// intermediate fields: their write and k_adj_h's read of them.
+// Reads the adjoint state from st.lex/ley/lhz and writes it to
+// st.olex/oley/olhz (the ring reads cells that neighbouring tiles rewrite, so
+// the step cannot be in place); the coefficient gradients accumulate as in
+// k_adj_e. Saves k_adj_e's three intermediate fields: their write and
+// k_adj_h's read of them. A wavefront holds region rows w, w+4, ... (one
+// 64-lane row each).
 constexpr int kEhRows = 16;               // tile rows
 constexpr int kEhCols = kBX - 4;          // tile columns (lanes 2..61 of the 64-lane region)
 constexpr int kEhRegion = kEhRows + 4;    // region rows
