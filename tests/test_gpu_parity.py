@@ -107,3 +107,28 @@ def test_mixed_step_and_run(device):
     assert_bitwise(f.Ey, o.Ey, "Ey")
     assert_bitwise(f.Hz, o.Hz, "Hz")
     assert sim.steps_completed == o.steps_completed == 32
+
+
+def test_fast_division_is_licensed_on_the_device(device):
+    """The exhaustive device check passes for the divisors the tests use, so
+    the fast_div=True cases above really run the fast quotient; stretched to
+    every finite float, the same check counts the mismatches above the range
+    (where x * 2^(31-e) overflows)."""
+    import ctypes
+
+    from waveforge_amd import _native
+    from waveforge_amd.engine import division_range, f32, verified_division
+
+    for d in (1e-3, 5e-4, 2.5e-3):
+        v = verified_division(d, device)
+        assert v is not None, d
+        r, lo, span = v
+        assert (lo, span) == division_range(f32(d))
+    d = f32(1e-3)
+    r, lo, span = verified_division(d, device)
+    bad = torch.zeros(1, dtype=torch.int32, device=device)
+    lib = _native.load()
+    _native.check(lib.wf_div_verify(d, r, 0, 0x7F800000, bad.data_ptr(),
+                                    ctypes.c_void_p(_native.stream_handle(device))), "wf_div_verify")
+    print(f"[fast-div] mismatches over every finite float: {int(bad.item())}", flush=True)
+    assert int(bad.item()) > 0
