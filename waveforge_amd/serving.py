@@ -89,7 +89,7 @@ class ImagingServer:
     """Admit, co-schedule and retire imaging requests on one GPU."""
 
     def __init__(self, *, dt: float, dx: float = 1e-3, device=None, hbm_budget: float = 64e9, quantum: int = 96,
-                 depth: int = 6, tile_rows: int = 32, max_requests: Optional[int] = None,
+                 depth: int = 8, tile_rows: int = 48, max_requests: Optional[int] = None,
                  stability_threshold: float = 1e10, v_medium: Optional[float] = None) -> None:
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         if self.device.type != "cuda":
