@@ -146,3 +146,26 @@ def test_run_mimo_matches_oracle_per_tx(device):
                          detectors=ants, n_check=500)
         o.run(steps)
         np.testing.assert_array_equal(S[tx], o.trace_array().numpy())
+
+
+def test_default_launch_shape_follows_the_batch_size(device):
+    """With no shape given, a batch of at least 1536 tiles of 48 x 48 cells runs
+    48-row tiles 8 steps deep and a smaller one 32-row tiles 6 deep; one of
+    the two given keeps the other at the 32 x 6 pair."""
+    from waveforge_amd.mimo import BatchedRuns, RunSpec
+
+    dt = stable_dt(DX, DX)
+
+    def batch(n, count, **kw):
+        ca = torch.full((n, n), 0.99, device=device)
+        cb = torch.full((n, n), 0.05, device=device)
+        specs = [RunSpec(n, n, ca, cb, sources=[], receivers=[], n_record=0) for _ in range(count)]
+        return BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, **kw)
+
+    big = batch(512, 16)     # 16 x 11 x 11 = 1936 tiles of 48 rows, 48 columns
+    assert (big.tile_rows, big.depth) == (48, 8)
+    small = batch(256, 16)   # 16 x 6 x 6 = 576 such tiles
+    assert (small.tile_rows, small.depth) == (32, 6)
+    assert (batch(512, 16, depth=4).tile_rows, batch(512, 16, tile_rows=16).depth) == (32, 6)
+    big.advance(10)
+    assert big.steps_completed == 10

@@ -61,7 +61,7 @@ class BatchedRuns:
     """Device state and launch plan of a set of independent runs."""
 
     def __init__(self, specs: Sequence[RunSpec], *, dx: float, dy: float, dt: float, device=None,
-                 tile_rows=32, depth=6, n_check: int = 500, stability_threshold: float = 1e10,
+                 tile_rows=None, depth=None, n_check: int = 500, stability_threshold: float = 1e10,
                  fast_div: bool = True) -> None:
         if not specs:
             raise ValueError("no runs")
@@ -116,6 +116,15 @@ class BatchedRuns:
             rs.step = 0
             self.runs.append(rs)
         SourceTable.upload_many(tables, dev)
+        if tile_rows is None or depth is None:
+            from .core import fdtd2d
+
+            # neither given: the size-aware shape of FDTD2D; one given: the 32 x 6 pair
+            t0, d0 = (fdtd2d.default_launch_shape(self.runs) if tile_rows is None and depth is None
+                      else (fdtd2d.DEFAULT_TILE_ROWS, fdtd2d.DEFAULT_DEPTH))
+            tile_rows = t0 if tile_rows is None else tile_rows
+            depth = d0 if depth is None else depth
+            self.tile_rows, self.depth = tile_rows, depth
         self.tuned = None
         if "auto" in (tile_rows, depth):
             from . import autotune
@@ -246,8 +255,8 @@ def mimo_specs(ca: torch.Tensor, cb: torch.Tensor, antennas, wave: torch.Tensor,
                     n_record=n_steps, tag=(tag, t)) for t, (i, j) in enumerate(antennas)]
 
 
-def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, depth: int = 6,
-             tile_rows: int = 32, n_check: int = 500, return_tensor: bool = False):
+def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, depth=None,
+             tile_rows=None, n_check: int = 500, return_tensor: bool = False):
     """All TX runs of one array in one batched launch sequence.
 
     Same inputs and output as the reference's ``run_mimo``: returns the
@@ -274,8 +283,8 @@ def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, de
     return out if return_tensor else out.cpu().numpy()
 
 
-def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, device, depth: int = 6,
-                   tile_rows: int = 32, n_check: int = 500, v_medium: float | None = None, gate_start: int = 0,
+def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, device, depth=None,
+                   tile_rows=None, n_check: int = 500, v_medium: float | None = None, gate_start: int = 0,
                    use_graph: bool = False, chunk: int = 240):
     """Batched MIMO imaging of many phantoms: all TX runs of all phantoms in
     one batched launch sequence, then one DAS image per phantom (raw traces).
