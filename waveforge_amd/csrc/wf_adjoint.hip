@@ -189,19 +189,20 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_h(wf_adj_args a, int next_ste
 // st.olex/oley/olhz (the ring reads cells that neighbouring tiles rewrite, so
 // the step cannot be in place); the coefficient gradients accumulate as in
 // k_adj_e. Saves k_adj_e's three intermediate fields: their write and
-// k_adj_h's read of them. A wavefront holds region rows w, w+4, ... (one
-// 64-lane row each).
-constexpr int kEhRows = 16;               // tile rows
+// k_adj_h's read of them. Wavefront w holds region rows w, w + kEhWaves, ...
+// (one 64-lane row each).
+constexpr int kEhRows = 8;                // tile rows
 constexpr int kEhCols = kBX - 4;          // tile columns (lanes 2..61 of the 64-lane region)
 constexpr int kEhRegion = kEhRows + 4;    // region rows
-constexpr int kEhPerWave = (kEhRegion + kBY - 1) / kBY;
+constexpr int kEhWaves = 4;               // wavefronts per workgroup
+constexpr int kEhPerWave = (kEhRegion + kEhWaves - 1) / kEhWaves;
 
 struct AdjState {
     const float *lex, *ley, *lhz;
     float *olex, *oley, *olhz;
 };
 
-__global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st, int next_step) {
+__global__ void __launch_bounds__(kBX * kEhWaves) k_adj_eh(wf_adj_args a, AdjState st, int next_step) {
     __shared__ float s_h[kEhRegion][kBX];
     const int q = threadIdx.x, w = threadIdx.y;
     const int nx = a.nx, ny = a.ny;
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st
     float vx[kEhPerWave], vy[kEhPerWave];
 #pragma unroll
     for (int t = 0; t < kEhPerWave; ++t) {
-        const int p = w + kBY * t;
+        const int p = w + kEhWaves * t;
         const int i = i0 + p;
         lex[t] = ley[t] = lhz[t] = ca[t] = 0.f;
         float cb = 0.f;
@@ -242,7 +243,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st
     double g0[kEhPerWave], g1[kEhPerWave];
 #pragma unroll
     for (int t = 0; t < kEhPerWave; ++t) {
-        const int p = w + kBY * t;
+        const int p = w + kEhWaves * t;
         const int i = i0 + p;
         h0[t] = hl[t] = hu[t] = exp_[t] = eyp[t] = 0.f;
         g0[t] = g1[t] = 0.0;
@@ -265,7 +266,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st
     float lz[kEhPerWave];
 #pragma unroll
     for (int t = 0; t < kEhPerWave; ++t) {
-        const int p = w + kBY * t;
+        const int p = w + kEhWaves * t;
         lz[t] = lhz[t];
         const float vx_r = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(vx[t]), 0x130, 0xf, 0xf, false));
         const float vy_d = p + 1 < kEhRegion ? s_h[p + 1][q] : 0.f;
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st
     float lex_o[kEhPerWave], ley_o[kEhPerWave];
 #pragma unroll
     for (int t = 0; t < kEhPerWave; ++t) {
-        const int p = w + kBY * t;
+        const int p = w + kEhWaves * t;
         const int i = i0 + p;
         lex_o[t] = ley_o[t] = 0.f;
         if (p >= kEhRegion) continue;
@@ -313,7 +314,7 @@ __global__ void __launch_bounds__(kBX * kBY) k_adj_eh(wf_adj_args a, AdjState st
     auto H = [&](int ii, int jj) -> float { return s_h[ii - i0][jj - j0]; };
 #pragma unroll
     for (int t = 0; t < kEhPerWave; ++t) {
-        const int p = w + kBY * t;
+        const int p = w + kEhWaves * t;
         const int i = i0 + p;
         if (p < 2 || p >= 2 + kEhRows || i >= nx || !core_col) continue;
         const int64_t c = (int64_t)i * ny + j;
@@ -369,7 +370,7 @@ extern "C" int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream) {
                 hipLaunchKernelGGL(k_adj_record, dim3(b.n_runs), dim3(64), 0, stream, r);
             }
             const int next = fold && t + 1 < n_steps ? b.step - 1 : -1;
-            hipLaunchKernelGGL(k_adj_eh, ge, dim3(kBX, kBY), 0, stream, b, (t & 1) ? sec : pri, next);
+            hipLaunchKernelGGL(k_adj_eh, ge, dim3(kBX, kEhWaves), 0, stream, b, (t & 1) ? sec : pri, next);
         }
         const int rc = wf::check_launch("wf_adjoint_step");
         if (rc || !(n_steps & 1)) return rc;
