@@ -149,9 +149,9 @@ def test_run_mimo_matches_oracle_per_tx(device):
 
 
 def test_default_launch_shape_follows_the_batch_size(device):
-    """With no shape given, a batch of at least 1536 tiles of 48 x 48 cells runs
-    48-row tiles 8 steps deep and a smaller one 32-row tiles 6 deep; one of
-    the two given keeps the other at the 32 x 6 pair."""
+    """With no shape given, a batch runs 48-row tiles 8 steps deep, or 16-row
+    tiles when it makes fewer than 64 tiles of 48 x 48 cells; one of the two
+    given takes the other from that default."""
     from waveforge_amd.mimo import BatchedRuns, RunSpec
 
     dt = stable_dt(DX, DX)
@@ -162,10 +162,11 @@ def test_default_launch_shape_follows_the_batch_size(device):
         specs = [RunSpec(n, n, ca, cb, sources=[], receivers=[], n_record=0) for _ in range(count)]
         return BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, **kw)
 
-    big = batch(512, 16)     # 16 x 11 x 11 = 1936 tiles of 48 rows, 48 columns
+    big = batch(256, 16)     # 16 x 6 x 6 = 576 tiles of 48 rows, 48 columns
     assert (big.tile_rows, big.depth) == (48, 8)
-    small = batch(256, 16)   # 16 x 6 x 6 = 576 such tiles
-    assert (small.tile_rows, small.depth) == (32, 6)
-    assert (batch(512, 16, depth=4).tile_rows, batch(512, 16, tile_rows=16).depth) == (32, 6)
+    small = batch(128, 2)    # 2 x 3 x 3 = 18 such tiles
+    assert (small.tile_rows, small.depth) == (16, 8)
+    assert (batch(256, 16, depth=6).tile_rows, batch(256, 16, tile_rows=32).depth) == (48, 8)
     big.advance(10)
-    assert big.steps_completed == 10
+    small.advance(10)
+    assert big.steps_completed == small.steps_completed == 10

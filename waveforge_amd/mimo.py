@@ -116,12 +116,10 @@ class BatchedRuns:
             rs.step = 0
             self.runs.append(rs)
         SourceTable.upload_many(tables, dev)
-        if tile_rows is None or depth is None:
-            from .core import fdtd2d
+        if tile_rows is None or depth is None:  # what is not given comes from FDTD2D's default
+            from .core.fdtd2d import default_launch_shape
 
-            # neither given: the size-aware shape of FDTD2D; one given: the 32 x 6 pair
-            t0, d0 = (fdtd2d.default_launch_shape(self.runs) if tile_rows is None and depth is None
-                      else (fdtd2d.DEFAULT_TILE_ROWS, fdtd2d.DEFAULT_DEPTH))
+            t0, d0 = default_launch_shape(self.runs)
             tile_rows = t0 if tile_rows is None else tile_rows
             depth = d0 if depth is None else depth
             self.tile_rows, self.depth = tile_rows, depth
