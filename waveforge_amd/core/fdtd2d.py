@@ -59,7 +59,7 @@ def default_launch_shape(runs) -> tuple[int, int]:
     48-row tiles (8-wavefront workgroups at 6 waves per SIMD, a small halo
     share) 8 steps deep: on every run set timed on the MI355X (single grids
     from 128^2 to 4096^2, 16 to 64 runs of 128^2 to 512^2) within 10 % of the
-    autotuner's best shape and 15-30 % ahead of 32-row tiles 6 deep. Sets of
+    autotuner's best shape and 10-36 % ahead of 32-row tiles 6 deep. Sets of
     fewer than 64 such tiles (one grid up to about 300^2) fill the GPU better
     with 16-row tiles, also 8 deep (about 10 % ahead there). If a run has more
     sources or detectors in one tile than the kernel stages, smaller tiles
