@@ -33,6 +33,8 @@ def _grid(nx, ny, dev):
         (50, 70, 6, 200, (25, 35), dict(v_medium=299_792_458.0 / math.sqrt(13.1))),
         (32, 32, 5, 80, (40, 20), dict(out_of_range="zero")),
         (20, 20, 1, 50, None, {}),
+        (60, 60, 32, 150, (24, 24), {}),  # the most receive distances a thread keeps in registers
+        (60, 60, 40, 150, (20, 28), dict(out_of_range="zero")),  # beyond that: recomputed per pair
     ],
 )
 def test_das_matches_oracle_bitwise(device, nx, ny, n_ant, T, img, kw):

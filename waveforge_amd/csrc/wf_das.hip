@@ -11,14 +11,39 @@
 // The library is built with -ffp-contract=off, so no product is fused into a
 // sum; fp64 sqrt and division are the correctly rounded device routines.
 //
-// Work per image: nx_img*ny_img*n_tx*n_rx gathers. The per-pixel distances to
-// each receive antenna are recomputed per pair, as the reference does; the
-// trace set (n_tx*n_rx*n_t*4 bytes, 2 MB for 16x16x2000) stays in L2, and
-// the 64 neighbouring pixels of a wavefront read nearby samples.
+// Work per image: nx_img*ny_img*n_tx*n_rx gathers. The reference recomputes
+// the pixel's distance to each receive antenna for every pair; the same fp64
+// operations on the same inputs give the same value, so for arrays of up to
+// kRegRx elements each thread forms the n_rx distances once, in registers
+// (n_tx + n_rx square roots per pixel instead of n_tx*(n_rx + 1)). The trace
+// set (n_tx*n_rx*n_t*4 bytes, 2 MB for 16x16x2000) stays in L2, and the 64
+// neighbouring pixels of a wavefront read nearby samples.
 #include "wf_common.hpp"
 
 namespace {
 
+constexpr int kRegRx = 32;  // receive elements whose distances a thread keeps in registers
+
+__device__ __forceinline__ double distance(double px, double py, double ax, double ay) {
+    const double ex = px - ax;
+    const double ey = py - ay;
+    return sqrt(ex * ex + ey * ey);
+}
+
+// One pair's sample: n = int32(rint(((d_tx + d_rx) / v) / dt)), clipped or
+// zeroed outside [0, n_t), zero before the gate.
+__device__ __forceinline__ float sample(const wf_das2d_args &a, const float *row, double dtx, double drx) {
+    const int nt = a.n_t;
+    const double tau = (dtx + drx) / a.v_medium;
+    const int n = (int)rint(tau / a.dt);
+    if (a.clip) {
+        const int nc = n < 0 ? 0 : (n > nt - 1 ? nt - 1 : n);
+        return nc < a.gate_start ? 0.0f : row[nc];
+    }
+    return (n < 0 || n >= nt || n < a.gate_start) ? 0.0f : row[n];
+}
+
+template <bool kReg>
 __global__ void __launch_bounds__(256) k_das(wf_das2d_args a) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t npx = (int64_t)a.nx_img * a.ny_img;
@@ -28,26 +53,25 @@ __global__ void __launch_bounds__(256) k_das(wf_das2d_args a) {
     const double px = (double)u * a.pitch_x;
     const double py = (double)v * a.pitch_y;
     const int nt = a.n_t;
+    double drx_r[kReg ? kRegRx : 1];
+    if (kReg) {
+#pragma unroll
+        for (int rx = 0; rx < kRegRx; ++rx)
+            drx_r[rx] = rx < a.n_rx ? distance(px, py, a.ant_x[rx], a.ant_y[rx]) : 0.0;
+    }
     double img = 0.0;
     for (int tx = 0; tx < a.n_tx; ++tx) {
-        const double ex = px - a.ant_x[tx];
-        const double ey = py - a.ant_y[tx];
-        const double dtx = sqrt(ex * ex + ey * ey);
+        const double dtx = distance(px, py, a.ant_x[tx], a.ant_y[tx]);
         const float *row = a.traces + (int64_t)tx * a.n_rx * nt;
-        for (int rx = 0; rx < a.n_rx; ++rx) {
-            const double fx = px - a.ant_x[rx];
-            const double fy = py - a.ant_y[rx];
-            const double drx = sqrt(fx * fx + fy * fy);
-            const double tau = (dtx + drx) / a.v_medium;
-            const int n = (int)rint(tau / a.dt);
-            float s;
-            if (a.clip) {
-                const int nc = n < 0 ? 0 : (n > nt - 1 ? nt - 1 : n);
-                s = nc < a.gate_start ? 0.0f : row[(int64_t)rx * nt + nc];
-            } else {
-                s = (n < 0 || n >= nt || n < a.gate_start) ? 0.0f : row[(int64_t)rx * nt + n];
+        if (kReg) {
+#pragma unroll
+            for (int rx = 0; rx < kRegRx; ++rx)
+                if (rx < a.n_rx) img = img + (double)sample(a, row + (int64_t)rx * nt, dtx, drx_r[rx]);
+        } else {
+            for (int rx = 0; rx < a.n_rx; ++rx) {
+                const double drx = distance(px, py, a.ant_x[rx], a.ant_y[rx]);
+                img = img + (double)sample(a, row + (int64_t)rx * nt, dtx, drx);
             }
-            img = img + (double)s;
         }
     }
     a.image[p] = img * img;
@@ -76,6 +100,9 @@ extern "C" int wf_das2d(const wf_das2d_args *args, hipStream_t stream) {
         wf::set_error("wf_das2d: image too large");
         return WF_ERR_ARG;
     }
-    hipLaunchKernelGGL(k_das, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    if (a.n_rx <= kRegRx)
+        hipLaunchKernelGGL(k_das<true>, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_das<false>, dim3((unsigned)blocks), dim3(256), 0, stream, a);
     return wf::check_launch("wf_das2d");
 }
