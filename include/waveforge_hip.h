@@ -29,12 +29,27 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 1
+#define WF_ABI_VERSION 2
 
 #define WF_OK 0
 #define WF_ERR_ARG (-1)
 #define WF_ERR_HIP (-2)
 #define WF_ERR_UNSUPPORTED (-3)
+
+/* Bits of a run's device status word (wf_run2d.status, wf_run2d_f64.status,
+ * wf_run3d.status). Kernels OR them in and never clear them; the caller
+ * zeroes the word and reads it back through wf_status_check.
+ *   WF_STATUS_WAVE_END     a step past the end of a source's waveform table:
+ *                          that contribution was not applied (the reference
+ *                          raises there, src/core/sources.py:882-887);
+ *   WF_STATUS_TILE_GROUPS  a tile region of wf_fdtd2d_tb_advance held more
+ *                          than 64 source groups (the extra ones were not
+ *                          applied);
+ *   WF_STATUS_TILE_DETS    a tile core of wf_fdtd2d_tb_advance held more than
+ *                          64 detectors (the extra ones were not recorded). */
+#define WF_STATUS_WAVE_END 1u
+#define WF_STATUS_TILE_GROUPS 2u
+#define WF_STATUS_TILE_DETS 4u
 
 /* Field component codes of an injection (src/core/sources.py:49). The 2D
  * TM path uses the first three; the 3D path all six. */
@@ -63,6 +78,7 @@ typedef struct wf_inj2d {
 typedef struct wf_run2d {
     float *ex, *ey, *hz;          /* live state, (nx, ny) fp32 each */
     float *ex2, *ey2, *hz2;       /* second state buffer (temporally blocked path) */
+    float *ex3, *ey3, *hz3;       /* third state buffer: NULL unless launches use parity 2 or 3 */
     const float *ca, *cb;         /* per-cell coefficients; NULL => free space */
     float *pstrip;                /* 4*(nx+ny) floats: Mur snapshot strips (per-step path) */
     const wf_inj2d *inj;          /* contributions sorted by group */
@@ -77,10 +93,20 @@ typedef struct wf_run2d {
     float cmx, cmy;               /* fp32 Mur coefficients (src/core/boundaries.py:177-178) */
     int32_t step;
     int32_t flags;                /* reserved, 0 */
+    uint32_t *status;             /* NULL or device word: WF_STATUS_* bits (see above) */
 } wf_run2d;
 
 int wf_abi_version(void);
 const char *wf_last_error(void);
+
+/* Wait for `stream`, read n_words device status words (WF_STATUS_* bits, as
+ * the stepping kernels leave them) and return WF_OK if all are zero, else
+ * WF_ERR_ARG with a message naming the first nonzero word and its
+ * conditions. The words are not cleared. This is how a direct caller learns
+ * of the errors the reference raises synchronously: stepping past a
+ * waveform table (src/core/sources.py:882-887), and the blocked kernel's
+ * per-tile source and detector limits. */
+int wf_status_check(const uint32_t *status, int32_t n_words, hipStream_t stream);
 
 /* Advance `n_steps` reference-exact leapfrog steps in place, starting at
  * run->step, one step per launch group (snapshot+inject, Faraday+Mur,
@@ -90,9 +116,9 @@ const char *wf_last_error(void);
  * the Faraday slice update (fdtd2d.py:232-235), MurABC.apply
  * (boundaries.py:239-274), the Ampere slice updates (fdtd2d.py:244-257), and
  * the callers' detector gather (examples/brain_mimo_imaging.py:196-198).
- * `runs` is a HOST array of n_runs descriptors (batch members). The caller
- * checks waveform lengths (stepping past a table end raises in the
- * reference, sources.py:882-887). Grids need 4 <= nx, ny and nx*ny < 2^31
+ * `runs` is a HOST array of n_runs descriptors (batch members). A step past
+ * a waveform table's end (the reference raises there, sources.py:882-887)
+ * injects nothing and sets WF_STATUS_WAVE_END in run->status. Grids need 4 <= nx, ny and nx*ny < 2^31
  * (cells are int32 in the source and detector tables; WF_ERR_UNSUPPORTED
  * otherwise). */
 int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps,
@@ -108,14 +134,22 @@ int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz,
                      float *out3, hipStream_t stream);
 
 /* Temporally blocked advance of many runs in one launch per block of steps.
- * Each launch advances every run by `depth` steps (1..WF_TB_MAX_DEPTH):
- * reads state A (ex/ey/hz) and writes state B (ex2/ey2/hz2) when parity is 0,
- * the reverse when parity is 1. Tiles of each run are listed in `tiles`
+ * Each launch advances every run by `depth` steps (1..WF_TB_MAX_DEPTH),
+ * reading one state buffer and writing another: parity 0 reads A (ex/ey/hz)
+ * and writes B (ex2/ey2/hz2), 1 reads B and writes A, 2 reads A and writes C
+ * (ex3/ey3/hz3), 3 reads C and writes B. An even number of launches
+ * alternates 0, 1; an odd number (3 or more) runs 2, 3, 1 and then alternates,
+ * so the state ends in A either way. Tiles of each run are listed in `tiles`
  * (device, 2 int32 per tile: run index, tile index within the run); a tile is
- * tile_i rows x tile_j columns with tile_j = 64 - 2*depth, numbered row-major
- * over ceil(nx/tile_i) x ceil(ny/tile_j). Source, Mur and detector handling is
- * exactly the per-step semantics above; step indices are run->step +
- * step_offset + level. If maxabs_out is non-NULL, the launch also folds
+ * tile_i rows x tile_j columns, numbered row-major over ceil(nx/tile_i) x
+ * ceil(ny/tile_j). tile_j = 64 - 2*depth stages one column per lane of each
+ * 64-lane row (tile_i 16, 32, 48 or 64); tile_j = 128 - 2*depth stages two
+ * (tile_i 16, 32 or 48), which halves the share of recomputed halo columns.
+ * Source, Mur and detector handling is exactly the per-step semantics above;
+ * step indices are run->step + step_offset + level. A tile region may hold at
+ * most 64 source groups and a tile core at most 64 detectors (more set
+ * WF_STATUS_TILE_GROUPS / WF_STATUS_TILE_DETS in run->status and are not
+ * applied); a step past a waveform table's end sets WF_STATUS_WAVE_END. If maxabs_out is non-NULL, the launch also folds
  * max|E|,|H| of its final level into maxabs_out[3*run + c] (as uint bit
  * patterns; the caller zeroes it first). fast_div selects the fast quotient
  * of wf_div_verify for every |x| < bits(div_span) (IEEE division above it and
@@ -128,8 +162,9 @@ typedef struct wf_tb_args {
     const wf_run2d *runs;      /* device array of n_runs descriptors */
     const int32_t *tiles;      /* device, 2*n_tiles */
     int32_t n_runs, n_tiles;
-    int32_t tile_i, tile_j;    /* core tile shape: tile_i from wf_tb_tile_shapes, tile_j = 64 - 2*depth */
-    int32_t depth, parity;
+    int32_t tile_i, tile_j;    /* core tile shape: tile_i from wf_tb_tile_shapes, tile_j = 64*cpl - 2*depth
+                                  with cpl = 1 or 2 columns per lane */
+    int32_t depth, parity;     /* parity 0..3: which state buffers the launch reads and writes */
     int32_t step_offset;
     int32_t has_materials;     /* 1 if every run has ca/cb, 0 if none has */
     uint32_t *maxabs_out;      /* NULL or device 3*n_runs words */
@@ -182,6 +217,7 @@ typedef struct wf_run2d_f64 {
     int32_t step;
     double dh, de, dx, dy;        /* dt/MU0, dt/EPS0, dx, dy in fp64 */
     double cmx, cmy;
+    uint32_t *status;             /* NULL or device word: WF_STATUS_* bits */
 } wf_run2d_f64;
 
 typedef struct wf_f64_args {
@@ -222,6 +258,7 @@ typedef struct wf_run3d {
     float cmx, cmy, cmz;          /* fp32 Mur coefficients (boundaries.py:468-470) */
     int32_t step;
     int32_t flags;                /* WF_RUN3D_* bits, else 0 */
+    uint32_t *status;             /* NULL or device word: WF_STATUS_* bits */
 } wf_run3d;
 
 /* wf_run3d.flags bit, honoured by wf_fdtd3d_fused_step only: the Mur snapshot
@@ -274,9 +311,11 @@ int wf_fdtd2d_judge(const uint32_t *maxabs, int32_t n_runs, double threshold, in
  * serves the graph-captured form of the run loop (src/core/fdtd2d.py:328-329). */
 int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, hipStream_t stream);
 
-/* Supported tile_i values for wf_tb_args (16 and 32 run 4-wavefront
- * workgroups, 48 and 64 run 8-wavefront ones), written into out as (tile_i, 0)
- * pairs (host, 2*count int32); returns count (<= cap). */
+/* Supported tile shapes of wf_tb_args, written into out as (tile_i, cpl)
+ * pairs (host, 2*count int32; cpl = columns per lane, so tile_j =
+ * 64*cpl - 2*depth); returns count (<= cap). One column per lane: 16 and 32
+ * rows run 4-wavefront workgroups, 48 and 64 rows 8-wavefront ones; two
+ * columns per lane: 16, 32 and 48 rows, 8-wavefront workgroups. */
 int wf_tb_tile_shapes(int32_t *out, int32_t cap);
 
 /* Exhaustive device check that the fast quotient equals IEEE x/d bit for bit

@@ -41,9 +41,11 @@ def test_header_constants_match_binding():
 
 
 def test_tile_shapes_listed():
-    rows = _native.tile_rows()
-    assert 32 in rows and all(r > 0 for r in rows)
+    shapes = _native.tile_shapes()
+    assert (32, 1) in shapes and (48, 2) in shapes and all(r > 0 and c in (1, 2) for r, c in shapes)
+    assert _native.tile_rows() == [r for r, c in shapes if c == 1]
     assert [_native.tile_cols(d) for d in (1, 4, 8)] == [62, 56, 48]
+    assert [_native.tile_cols(d, 2) for d in (1, 4, 8)] == [126, 120, 112]
 
 
 def test_argument_validation_without_gpu():
@@ -73,11 +75,11 @@ int main(void) {
          sizeof(wf_inj2d), sizeof(wf_run2d), sizeof(wf_tb_args), sizeof(wf_das2d_args));
   O(wf_inj2d, wave) O(wf_inj2d, wave_len) O(wf_inj2d, comp) O(wf_inj2d, cell) O(wf_inj2d, amp)
   O(wf_run2d, ex) O(wf_run2d, ey) O(wf_run2d, hz) O(wf_run2d, ex2) O(wf_run2d, ey2) O(wf_run2d, hz2)
-  O(wf_run2d, ca) O(wf_run2d, cb) O(wf_run2d, pstrip) O(wf_run2d, inj) O(wf_run2d, grp_start)
+  O(wf_run2d, ex3) O(wf_run2d, ey3) O(wf_run2d, hz3) O(wf_run2d, ca) O(wf_run2d, cb) O(wf_run2d, pstrip) O(wf_run2d, inj) O(wf_run2d, grp_start)
   O(wf_run2d, det_cell) O(wf_run2d, traces) O(wf_run2d, trace_stride) O(wf_run2d, n_grp) O(wf_run2d, n_inj)
   O(wf_run2d, n_det) O(wf_run2d, trace_n0) O(wf_run2d, trace_len) O(wf_run2d, nx) O(wf_run2d, ny)
   O(wf_run2d, dh) O(wf_run2d, de) O(wf_run2d, dx) O(wf_run2d, dy) O(wf_run2d, cmx) O(wf_run2d, cmy)
-  O(wf_run2d, step) O(wf_run2d, flags)
+  O(wf_run2d, step) O(wf_run2d, flags) O(wf_run2d, status)
   O(wf_tb_args, runs) O(wf_tb_args, tiles) O(wf_tb_args, n_runs) O(wf_tb_args, n_tiles) O(wf_tb_args, tile_i)
   O(wf_tb_args, tile_j) O(wf_tb_args, depth) O(wf_tb_args, parity) O(wf_tb_args, step_offset)
   O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out) O(wf_tb_args, rdx) O(wf_tb_args, rdy)
@@ -95,7 +97,7 @@ int main(void) {
          sizeof(wf_inj2d_f64), sizeof(wf_run2d_f64), sizeof(wf_f64_args));
   O(wf_inj2d_f64, wave) O(wf_inj2d_f64, cell) O(wf_inj2d_f64, amp)
   O(wf_run2d_f64, ex) O(wf_run2d_f64, pstrip) O(wf_run2d_f64, traces) O(wf_run2d_f64, trace_stride)
-  O(wf_run2d_f64, n_grp) O(wf_run2d_f64, step) O(wf_run2d_f64, dh) O(wf_run2d_f64, cmy)
+  O(wf_run2d_f64, n_grp) O(wf_run2d_f64, step) O(wf_run2d_f64, dh) O(wf_run2d_f64, cmy) O(wf_run2d_f64, status)
   O(wf_f64_args, runs) O(wf_f64_args, n_runs) O(wf_f64_args, step_offset) O(wf_f64_args, pad)
   printf("wf_das_env_args size %zu\n", sizeof(wf_das_env_args));
   O(wf_das_env_args, traces) O(wf_das_env_args, layer_eps) O(wf_das_env_args, cx) O(wf_das_env_args, dt)
@@ -103,7 +105,7 @@ int main(void) {
   printf("wf_run3d size %zu\n", sizeof(wf_run3d));
   O(wf_run3d, ex) O(wf_run3d, hz) O(wf_run3d, ex2) O(wf_run3d, hz2) O(wf_run3d, ca) O(wf_run3d, pstrip) O(wf_run3d, det_comp) O(wf_run3d, traces)
   O(wf_run3d, trace_stride) O(wf_run3d, n_grp) O(wf_run3d, nz) O(wf_run3d, dh) O(wf_run3d, cmz)
-  O(wf_run3d, step) O(wf_run3d, flags)
+  O(wf_run3d, step) O(wf_run3d, flags) O(wf_run3d, status)
   return 0;
 }
 """

@@ -17,9 +17,9 @@ def test_tuner_times_every_candidate_and_caches(device, tmp_path):
     sim, _ = build_product(c, device, depth="auto", tile="auto")
     sim.run(30)
     res = sim._tuned
-    assert sim.launch_shape == (res.tile_rows, res.depth)
-    assert len(res.rates) == len(_native.tile_rows()) * 8
-    assert max(res.rates.values()) == res.rates[f"{res.tile_rows},{res.depth}"]
+    assert sim.launch_shape == (res.tile_rows, res.depth, res.cpl)
+    assert len(res.rates) == len(_native.tile_shapes()) * 8
+    assert max(res.rates.values()) == res.rates[autotune.shape_key(res.tile_rows, res.cpl, res.depth)]
     again = autotune.tune_runs(sim._build_runs(), fast_div=True)
     assert again is res
     path = tmp_path / "tune.json"
@@ -56,7 +56,8 @@ def test_batched_runs_auto_shape(device):
     dt = stable_dt(1e-3, 1e-3)
     specs, oracles = _specs(device, 40, dt)
     br = BatchedRuns(specs, dx=1e-3, dy=1e-3, dt=dt, device=device, tile_rows="auto", depth="auto")
-    assert br.tuned is not None and (br.tile_rows, br.depth) == (br.tuned.tile_rows, br.tuned.depth)
+    assert br.tuned is not None
+    assert (br.tile_rows, br.depth, br.cpl) == (br.tuned.tile_rows, br.tuned.depth, br.tuned.cpl)
     br.advance(40)
     torch.cuda.synchronize()
     for k, o in enumerate(oracles):

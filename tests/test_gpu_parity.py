@@ -42,6 +42,14 @@ CASES = {
         ],
         detectors=[(0, 6), (1, 5), (19, 23)],
     ),
+    # the cells Ampere never writes (Ex of column 0, Ey of row 0) carry values
+    # and take soft sources, on a grid wider than one two-column region
+    "face_cells": case(
+        23, 150, steps=20, material=True, init=True, seed=3,
+        sources=[("point", "Ex", 9, 0, "sin"), ("point", "Ey", 0, 7, "gauss"), ("point", "Ey", 0, 140, "sin"),
+                 ("point", "Ex", 0, 0, "gauss"), ("line", "Ex", "x", 0, 2, 20, "dgauss", None)],
+        detectors=[(0, 0), (11, 1), (22, 149)],
+    ),
     "odd_tiles_mat": case(133, 71, steps=41, material=True, sources=[("point", "Hz", 66, 35, "dgauss")],
                           detectors=[(10, 10), (120, 60), (66, 35)]),
     "batch3": case(17, 19, steps=25, init=True, batch=3, sources=[("point", "Hz", 8, 9, "gauss")],
@@ -74,13 +82,13 @@ def test_per_step_path_bitwise(device, name):
 
 
 @pytest.mark.parametrize("fast_div", [True, False])
-@pytest.mark.parametrize("tile", [32, 16, 48, 64])
+@pytest.mark.parametrize("tile", [32, 16, 48, 64, (16, 2), (32, 2), (48, 2)])
 @pytest.mark.parametrize("depth", [1, 2, 3, 4, 5, 8])
-@pytest.mark.parametrize("name", ["tiny5x7_mat", "mat_all_sources", "edge_sources", "odd_tiles_mat", "batch3",
-                                  "free_point_hz", "extreme_mat", "extreme_free"])
+@pytest.mark.parametrize("name", ["tiny5x7_mat", "mat_all_sources", "edge_sources", "face_cells", "odd_tiles_mat",
+                                  "batch3", "free_point_hz", "extreme_mat", "extreme_free"])
 def test_tb_path_bitwise(device, name, tile, depth, fast_div):
     c = CASES[name]
-    print(f"[tb] case={name} tile_rows={tile} depth={depth} fast_div={fast_div}", flush=True)
+    print(f"[tb] case={name} tile={tile} depth={depth} fast_div={fast_div}", flush=True)
     sim, det = build_product(c, device, depth=depth, tile=tile, fast_div=fast_div)
     o = build_oracle(c)
     sim.run(c["steps"])

@@ -224,17 +224,17 @@ def test_default_launch_shape_by_workload():
     def runs(*shapes, det=None):
         return [SimpleNamespace(nx=a, ny=b, sources=None, det_cell=det, extra={}) for a, b in shapes]
 
-    assert default_launch_shape(runs((4096, 4096))) == (48, 8)   # 86 x 86 tiles
-    assert default_launch_shape(runs((1024, 1024))) == (48, 8)   # 22 x 22 tiles
-    assert default_launch_shape(runs(*[(256, 256)] * 16)) == (48, 8)   # 36 tiles each
-    assert default_launch_shape(runs((256, 256))) == (16, 8)
-    assert default_launch_shape(runs((128, 128), (200, 120))) == (16, 8)   # 9 + 15 tiles
+    assert default_launch_shape(runs((4096, 4096))) == (48, 8, 1)   # 86 x 86 tiles
+    assert default_launch_shape(runs((1024, 1024))) == (48, 8, 1)   # 22 x 22 tiles
+    assert default_launch_shape(runs(*[(256, 256)] * 16)) == (48, 8, 1)   # 36 tiles each
+    assert default_launch_shape(runs((256, 256))) == (16, 8, 1)
+    assert default_launch_shape(runs((128, 128), (200, 120))) == (16, 8, 1)   # 9 + 15 tiles
     # a 9 x 9 block of detectors (81 > 64 per tile) in one 48-row tile, split
     # 4 + 5 rows by the 32-row tile boundary at row 160
     side = int(np.sqrt(TB_MAX_DETS_PER_TILE)) + 1
     cells = [(156 + a) * 1024 + 100 + b for a in range(side) for b in range(side)]
     det = torch.tensor(cells, dtype=torch.int32)
-    assert default_launch_shape(runs((1024, 1024), det=det)) == (32, 6)
+    assert default_launch_shape(runs((1024, 1024), det=det)) == (32, 6, 1)
 
 
 def test_grids_beyond_int32_cells_are_rejected():

@@ -12,7 +12,7 @@ import threading
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libwaveforge_hip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 WF_COMP3 = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
@@ -24,6 +24,7 @@ TB_MAX_DEPTH = 8
 EXPORTED_SYMBOLS = (
     "wf_abi_version",
     "wf_last_error",
+    "wf_status_check",
     "wf_fdtd2d_step",
     "wf_fdtd2d_maxabs",
     "wf_fdtd2d_tb_advance",
@@ -65,6 +66,7 @@ class WfRun2d(ctypes.Structure):
     _fields_ = [
         ("ex", _vp), ("ey", _vp), ("hz", _vp),
         ("ex2", _vp), ("ey2", _vp), ("hz2", _vp),
+        ("ex3", _vp), ("ey3", _vp), ("hz3", _vp),
         ("ca", _vp), ("cb", _vp),
         ("pstrip", _vp),
         ("inj", _vp), ("grp_start", _vp),
@@ -76,6 +78,7 @@ class WfRun2d(ctypes.Structure):
         ("dh", _f32), ("de", _f32), ("dx", _f32), ("dy", _f32),
         ("cmx", _f32), ("cmy", _f32),
         ("step", _i32), ("flags", _i32),
+        ("status", _vp),
     ]
 
 
@@ -135,7 +138,7 @@ class WfRun2dF64(ctypes.Structure):
                                    "traces")] + \
         [("trace_stride", _i64)] + \
         [(n, _i32) for n in ("n_grp", "n_inj", "n_det", "trace_n0", "trace_len", "nx", "ny", "step")] + \
-        [(n, _f64) for n in ("dh", "de", "dx", "dy", "cmx", "cmy")]
+        [(n, _f64) for n in ("dh", "de", "dx", "dy", "cmx", "cmy")] + [("status", _vp)]
 
 
 class WfF64Args(ctypes.Structure):
@@ -162,7 +165,7 @@ class WfRun3d(ctypes.Structure):
         [("trace_stride", _i64)] + \
         [(n, _i32) for n in ("n_grp", "n_inj", "n_det", "trace_n0", "trace_len", "nx", "ny", "nz")] + \
         [(n, _f32) for n in ("dh", "de", "dx", "dy", "dz", "cmx", "cmy", "cmz")] + \
-        [("step", _i32), ("flags", _i32)]
+        [("step", _i32), ("flags", _i32), ("status", _vp)]
 
 
 _lock = threading.Lock()
@@ -188,6 +191,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
             raise RuntimeError(f"waveforge_amd: cannot load {p}: {exc}") from exc
         lib.wf_abi_version.restype = ctypes.c_int
         lib.wf_last_error.restype = ctypes.c_char_p
+        lib.wf_status_check.argtypes = [_vp, _i32, _vp]
+        lib.wf_status_check.restype = ctypes.c_int
         lib.wf_fdtd2d_step.argtypes = [ctypes.POINTER(WfRun2d), _i32, _i32, _vp]
         lib.wf_fdtd2d_step.restype = ctypes.c_int
         lib.wf_fdtd2d_maxabs.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp]
@@ -230,17 +235,24 @@ def check(rc: int, what: str) -> None:
         raise RuntimeError(f"{what} failed ({rc}): {msg}")
 
 
-def tile_rows() -> list[int]:
-    """Tile heights (tile_i) compiled into the temporally blocked kernel."""
+def tile_shapes() -> list[tuple[int, int]]:
+    """(tile rows, columns per lane) of every shape compiled into the
+    temporally blocked kernel; the tile width is ``64*cpl - 2*depth``."""
     lib = load()
     buf = (_i32 * 64)()
     n = lib.wf_tb_tile_shapes(buf, 32)
-    return [int(buf[2 * k]) for k in range(n)]
+    return [(int(buf[2 * k]), int(buf[2 * k + 1])) for k in range(n)]
 
 
-def tile_cols(depth: int) -> int:
-    """Tile width for a launch of ``depth`` steps: one 64-lane wavefront per row minus the halo."""
-    return 64 - 2 * depth
+def tile_rows() -> list[int]:
+    """Tile heights (tile_i) compiled for one column per lane."""
+    return [ti for ti, cpl in tile_shapes() if cpl == 1]
+
+
+def tile_cols(depth: int, cpl: int = 1) -> int:
+    """Tile width for a launch of ``depth`` steps: one 64-lane wavefront per
+    row with ``cpl`` columns per lane, minus the halo."""
+    return 64 * cpl - 2 * depth
 
 
 def stream_handle(device) -> int:

@@ -28,6 +28,7 @@ from .. import _native
 from ..engine import (
     RunState,
     judge,
+    launch_parities,
     schedule_launches,
     SourceTable,
     StepCoefficients,
@@ -48,13 +49,24 @@ MU0: float = 1.2566370614e-6
 
 DEFAULT_TILE_ROWS = 48
 DEFAULT_DEPTH = 8
+DEFAULT_CPL = 1
 SMALL_SET_TILE_ROWS = 16
 SMALL_SET_MAX_TILES = 64
-FALLBACK_SHAPES = ((32, 6), (16, 6))
+FALLBACK_SHAPES = ((32, 6, 1), (16, 6, 1))
 
 
-def default_launch_shape(runs) -> tuple[int, int]:
-    """(tile rows, depth) when none is given.
+def tile_shape(tile) -> tuple[int, int]:
+    """``(tile rows, columns per lane)`` of a tile argument: an int is a tile
+    height with one column per lane, a pair names both."""
+    if isinstance(tile, (tuple, list)) and len(tile) == 2 and all(isinstance(v, int) for v in tile):
+        return int(tile[0]), int(tile[1])
+    if isinstance(tile, int) and not isinstance(tile, bool):
+        return int(tile), 1
+    raise ValueError(f"tile must be an int (rows), a (rows, columns per lane) pair, None or 'auto'; got {tile!r}")
+
+
+def default_launch_shape(runs) -> tuple[int, int, int]:
+    """(tile rows, depth, columns per lane) when none is given.
 
     48-row tiles (8-wavefront workgroups at 6 waves per SIMD, a small halo
     share) 8 steps deep: on every run set timed on the MI355X (single grids
@@ -64,12 +76,13 @@ def default_launch_shape(runs) -> tuple[int, int]:
     with 16-row tiles, also 8 deep (about 10 % ahead there). If a run has more
     sources or detectors in one tile than the kernel stages, smaller tiles
     are tried before the caller falls back to the per-step path."""
-    tj = 64 - 2 * DEFAULT_DEPTH
+    tj = 64 * DEFAULT_CPL - 2 * DEFAULT_DEPTH
     n = sum(-(-r.nx // DEFAULT_TILE_ROWS) * -(-r.ny // tj) for r in runs)
-    first = (SMALL_SET_TILE_ROWS if n < SMALL_SET_MAX_TILES else DEFAULT_TILE_ROWS, DEFAULT_DEPTH)
-    for tile, depth in (first,) + FALLBACK_SHAPES:
-        if all(tb_limits_ok(r, tile, d) for r in runs for d in range(1, depth + 1)):
-            return tile, depth
+    first = ((SMALL_SET_TILE_ROWS, DEFAULT_DEPTH, 1) if n < SMALL_SET_MAX_TILES
+             else (DEFAULT_TILE_ROWS, DEFAULT_DEPTH, DEFAULT_CPL))
+    for tile, depth, cpl in (first,) + FALLBACK_SHAPES:
+        if all(tb_limits_ok(r, tile, d, cpl) for r in runs for d in range(1, depth + 1)):
+            return tile, depth, cpl
     return first
 
 
@@ -126,11 +139,12 @@ class FDTD2D:
         self._threshold = stability_threshold
         self._n_check = n_check
         self._detectors = detectors
-        for name, v, ok in (("depth", depth, range(1, _native.TB_MAX_DEPTH + 1)), ("tile", tile, None)):
-            if v is not None and v != "auto" and not isinstance(v, int):
-                raise ValueError(f"{name} must be an int, None or 'auto', got {v!r}")
-            if ok is not None and isinstance(v, int) and v not in ok:
-                raise ValueError(f"{name} must be in 1..{_native.TB_MAX_DEPTH}, got {v}")
+        if depth is not None and depth != "auto" and not isinstance(depth, int):
+            raise ValueError(f"depth must be an int, None or 'auto', got {depth!r}")
+        if isinstance(depth, int) and depth not in range(1, _native.TB_MAX_DEPTH + 1):
+            raise ValueError(f"depth must be in 1..{_native.TB_MAX_DEPTH}, got {depth}")
+        if tile is not None and tile != "auto":
+            tile = tile_shape(tile)
         self._depth = depth
         self._tile = tile
         self.launch_shape = None
@@ -145,8 +159,13 @@ class FDTD2D:
         self._src_table: Optional[SourceTable] = None
         self._src_key = None
         self._second = None
+        self._third = None
         self._tb_plan: Optional[TBPlan] = None
         self._tb_key = None
+        self._shape_key = None
+        self._status: Optional[torch.Tensor] = None   # device word the kernels raise WF_STATUS_* bits in
+        self._slots: Optional[torch.Tensor] = None    # check slots (+ a copy of the status word), reused
+        self._stop: Optional[torch.Tensor] = None     # per-run stop flags, all zero between calls
 
     # ------------------------------------------------------------------
     # device state
@@ -213,9 +232,11 @@ class FDTD2D:
         coef = StepCoefficients.from_grid(self._grid.dt, self._grid.dx, self._grid.dy,
                                           self._boundary.C_mur_x, self._boundary.C_mur_y)
         Nx, Ny = self._grid.Nx, self._grid.Ny
+        if self._status is None or self._status.device != hzs[0].device:
+            self._status = torch.zeros(1, dtype=torch.int32, device=hzs[0].device)
         runs = []
         for b in range(B):
-            rs = RunState(exs[b], eys[b], hzs[b], cas[b], cbs[b], Nx, Ny, coef, sources=srcs)
+            rs = RunState(exs[b], eys[b], hzs[b], cas[b], cbs[b], Nx, Ny, coef, sources=srcs, status=self._status)
             rs.pstrip = torch.empty(4 * (Nx + Ny), dtype=torch.float32, device=hzs[b].device)
             if det is not None and len(det):
                 rs.det_cell = det.cell_index
@@ -263,7 +284,7 @@ class FDTD2D:
                 first += 1
                 n_steps -= 1
             if n_steps > 0:
-                plan = self._tb_plan_for(runs, tile)
+                plan = self._tb_plan_for(runs, tile, third=len(split_launches(n_steps, depth, odd_ok=True)) % 2 == 1)
                 plan.advance(n_steps, depth, stream, base_step=first)
         else:
             for r in runs:
@@ -271,19 +292,29 @@ class FDTD2D:
             per_step_advance(runs, n_steps, stream)
             self._apply_inert(self.steps_completed, n_steps)
 
-    def _tb_plan_for(self, runs, tile) -> TBPlan:
-        if self._second is None or self._second[0].shape != self._fields.Hz.shape:
-            shape = self._fields.Hz.shape
-            dev = self._fields.Hz.device
+    def _tb_plan_for(self, runs, tile: tuple[int, int], third: bool = False) -> TBPlan:
+        """The blocked-kernel plan for these runs: a second state buffer always,
+        a third one (allocated on first need, then kept) for odd launch counts."""
+        shape = self._fields.Hz.shape
+        dev = self._fields.Hz.device
+        if self._second is None or self._second[0].shape != shape:
             self._second = tuple(torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3))
+            self._third = None
             self._tb_key = None
-        ex2, ey2, hz2 = (self._members(t) for t in self._second)
-        key = (self._cache_key, tile, self._fast_div)
+        if third and (self._third is None or self._third[0].shape != shape):
+            self._third = tuple(torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3))
+            self._tb_key = None
+        key = (self._cache_key, tile, self._fast_div, self._third is not None)
         if key != self._tb_key:
+            ex2, ey2, hz2 = (self._members(t) for t in self._second)
+            ex3 = ey3 = hz3 = None
+            if self._third is not None:
+                ex3, ey3, hz3 = (self._members(t) for t in self._third)
             for b, r in enumerate(runs):
                 r.second = (ex2[b], ey2[b], hz2[b])
+                r.third = None if ex3 is None else (ex3[b], ey3[b], hz3[b])
                 r.step = 0  # launches pass the global step as step_offset
-            self._tb_plan = TBPlan(runs, tile, self._fields.Hz.device, fast_div=self._fast_div)
+            self._tb_plan = TBPlan(runs, tile[0], dev, fast_div=self._fast_div, cpl=tile[1])
             self._tb_key = key
         return self._tb_plan
 
@@ -317,19 +348,23 @@ class FDTD2D:
         self._device_check()
         runs = self._build_runs()
         inert = self._src_table.inert if self._src_table is not None else []
-        tile, depth = self._launch_shape(runs, inert)
-        key = (self._cache_key, tile, depth)
-        if getattr(self, "_tb_ok_key", None) != key:
-            self._tb_ok = not inert and all(tb_limits_ok(r, tile, d) for r in runs for d in range(1, depth + 1))
-            self._tb_ok_key = key
-        return (runs, tile, depth) if self._tb_ok else None
+        key = (self._cache_key, self._tile, self._depth, self._fast_div)
+        if self._shape_key != key:
+            tile, depth = self._launch_shape(runs, inert)
+            ok = not inert and all(tb_limits_ok(r, tile[0], d, tile[1]) for r in runs for d in range(1, depth + 1))
+            self._shape_cached = (tile, depth, ok)
+            self._shape_key = key
+        tile, depth, ok = self._shape_cached
+        self.launch_shape = (tile[0], depth, tile[1])
+        return (runs, tile, depth) if ok else None
 
-    def _launch_shape(self, runs, inert) -> tuple[int, int]:
-        """(tile rows, depth): as given, the defaults, or, for ``"auto"``, the
-        autotuner's choice for these grids (timed once per shape set)."""
-        tile0, depth0 = default_launch_shape(runs)
+    def _launch_shape(self, runs, inert) -> tuple[tuple[int, int], int]:
+        """((tile rows, columns per lane), depth): as given, the defaults, or,
+        for ``"auto"``, the autotuner's choice for these grids (timed once per
+        shape set)."""
+        tile0, depth0, cpl0 = default_launch_shape(runs)
         depth = self._depth if isinstance(self._depth, int) else depth0
-        tile = self._tile if isinstance(self._tile, int) else tile0
+        tile = self._tile if isinstance(self._tile, tuple) else (tile0, cpl0)
         if "auto" in (self._tile, self._depth) and not inert:
             if getattr(self, "_tuned_key", None) != self._cache_key:
                 from .. import autotune
@@ -339,9 +374,7 @@ class FDTD2D:
                     tile_rows=None if self._tile == "auto" else [tile],
                     depths=autotune.DEPTHS if self._depth == "auto" else [depth])
                 self._tuned, self._tuned_key = res, self._cache_key
-            tile, depth = self._tuned.tile_rows, self._tuned.depth
-
-        self.launch_shape = (tile, depth)
+            tile, depth = (self._tuned.tile_rows, self._tuned.cpl), self._tuned.depth
         return tile, depth
 
     def _run_async(self, n_steps: int, setup) -> None:
@@ -351,32 +384,55 @@ class FDTD2D:
         step the launch ending there folds max|Ex|, |Ey|, |Hz| into a slot, a
         one-thread kernel compares it with the threshold and raises a device
         flag, and every later launch carries the flagged state through
-        unchanged. One synchronisation at the end reads the slots; a failed
-        check raises ``SimulationDivergedError`` with the fields and
-        ``steps_completed`` of that check step, as the reference does."""
+        unchanged. One synchronisation at the end reads the slots (and the
+        kernels' status word); a failed check raises
+        ``SimulationDivergedError`` with the fields and ``steps_completed``
+        of that check step, as the reference does. Slots, stop flags and the
+        launch arguments persist across calls, so a call costs the launches
+        and little else."""
         runs, tile, depth = setup
         start = self.steps_completed
         limit = self._wave_limit()
         n_exec = n_steps if limit is None else max(0, min(n_steps, limit - start))
-        launches, checks = schedule_launches(start, n_exec, depth, self._n_check)
+        launches, checks = schedule_launches(start, n_exec, depth, self._n_check, odd_ok=True)
         if n_exec and launches is None:
             return self._run_sync(n_steps)
-        stream = self._stream()
-        dev = self._fields.Hz.device
         if n_exec:
-            plan = self._tb_plan_for(runs, tile)
-            slots = torch.zeros((max(1, len(checks)), len(runs), 3), dtype=torch.int32, device=dev)
-            stop = torch.zeros(len(runs), dtype=torch.int32, device=dev)
+            stream = self._stream()
+            dev = self._fields.Hz.device
+            plan = self._tb_plan_for(runs, tile, third=len(launches) % 2 == 1)
+            n_runs = len(runs)
+            if self._stop is None or self._stop.numel() != n_runs or self._stop.device != dev:
+                self._stop = torch.zeros(n_runs, dtype=torch.int32, device=dev)
+            need = max(1, len(checks)) * n_runs * 3 + 1
+            if self._slots is None or self._slots.numel() < need or self._slots.device != dev:
+                self._slots = torch.zeros(need, dtype=torch.int32, device=dev)
+            slots = self._slots[:len(checks) * n_runs * 3].view(len(checks), n_runs, 3) if checks else None
             offset = start
-            for k, (d, slot) in enumerate(launches):
-                plan.launch(d, k % 2, offset, stream, maxabs_out=slots[slot] if slot >= 0 else None, stop=stop)
+            for (d, slot), par in zip(launches, launch_parities(len(launches))):
+                plan.launch(d, par, offset, stream, maxabs_out=slots[slot] if slot >= 0 else None, stop=self._stop)
                 if slot >= 0:
-                    judge(slots[slot], self._threshold, stop, True, stream)
+                    judge(slots[slot], self._threshold, self._stop, True, stream)
                 offset += d
-            vals = slots.view(torch.float32).amax(dim=1).cpu().tolist()
-            for step_m, (vx, vy, vh) in zip(checks, vals):
-                self.steps_completed = step_m
-                self._judge_values((vx, vy, vh))
+            if checks:
+                used = len(checks) * n_runs * 3
+                self._slots[used:used + 1].copy_(self._status)
+                host = self._slots[:used + 1].cpu()
+                self._slots[:used + 1].zero_()   # the next call's slots start at zero
+                status = int(host[used])
+                vals = host[:used].view(torch.float32).view(len(checks), n_runs, 3).amax(dim=1).tolist()
+                if status:
+                    self._status.zero_()
+                    raise RuntimeError(f"waveforge_amd: the stepping kernels reported status 0x{status:x} "
+                                       "(a source past its waveform table, or a tile over its source or "
+                                       "detector limit)")
+                try:
+                    for step_m, (vx, vy, vh) in zip(checks, vals):
+                        self.steps_completed = step_m
+                        self._judge_values((vx, vy, vh))
+                except SimulationDivergedError:
+                    self._stop.zero_()
+                    raise
         self.steps_completed = start + n_exec
         if n_exec < n_steps:
             raise RuntimeError(

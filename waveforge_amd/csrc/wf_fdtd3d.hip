@@ -97,6 +97,10 @@ __global__ void __launch_bounds__(256) k3_src(wf_run3d r, int n) {
         const int64_t cell = r.inj[a].cell;
         float v = f[cell];
         for (int c = a; c < b; ++c) {
+            if ((unsigned)n >= (unsigned)r.inj[c].wave_len) {  // past the table: report, inject nothing
+                if (r.status) atomicOr(r.status, WF_STATUS_WAVE_END);
+                continue;
+            }
             const float w = r.inj[c].wave[n] * r.inj[c].amp;  // values = w_now * amp
             v = v + w;
         }

@@ -66,6 +66,10 @@ __device__ __forceinline__ void pre_body(const R &r, int n) {
         T *f = comp == WF_COMP_EX ? r.ex : (comp == WF_COMP_EY ? r.ey : r.hz);
         T v = f[cell];
         for (int c = a; c < b; ++c) {
+            if ((unsigned)n >= (unsigned)r.inj[c].wave_len) {  // past the table: report, inject nothing
+                if (r.status) atomicOr(r.status, WF_STATUS_WAVE_END);
+                continue;
+            }
             const T w = r.inj[c].wave[n] * r.inj[c].amp;  // values = w_now * amp
             v = v + w;                                    // target[idx] += values
         }

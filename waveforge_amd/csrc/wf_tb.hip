@@ -5,7 +5,8 @@
 
 namespace {
 
-constexpr int kTileRows[] = {32, 16, 48, 64};
+// (tile_i, columns per lane) of every compiled shape
+constexpr int kTileShapes[][2] = {{32, 1}, {16, 1}, {48, 1}, {64, 1}, {16, 2}, {32, 2}, {48, 2}};
 
 // Exhaustive check of the fast quotient (wf_tb::divide<true>) against IEEE
 // division for every x with lo <= |x bits| < lo + span (both signs). Each
@@ -29,10 +30,14 @@ __global__ void __launch_bounds__(256) k_div_verify(float d, float r, unsigned i
 }  // namespace
 
 extern "C" int wf_tb_tile_shapes(int32_t *out, int32_t cap) {
-    const int n = (int)(sizeof(kTileRows) / sizeof(kTileRows[0]));
+    const int n = (int)(sizeof(kTileShapes) / sizeof(kTileShapes[0]));
+    if (!out || cap < 0) {
+        wf::set_error("wf_tb_tile_shapes: bad arguments");
+        return WF_ERR_ARG;
+    }
     for (int k = 0; k < n && k < cap; ++k) {
-        out[2 * k] = kTileRows[k];
-        out[2 * k + 1] = 0;  // tile_j is 64 - 2*depth
+        out[2 * k] = kTileShapes[k][0];
+        out[2 * k + 1] = kTileShapes[k][1];  // tile_j is 64*cpl - 2*depth
     }
     return n;
 }
@@ -47,22 +52,24 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: depth %d outside [1, %d]", a.depth, WF_TB_MAX_DEPTH);
         return WF_ERR_ARG;
     }
-    if (a.tile_j != wf_tb::kW - 2 * a.depth) {
-        wf::set_error("wf_fdtd2d_tb_advance: tile_j must be 64 - 2*depth = %d, got %d", wf_tb::kW - 2 * a.depth, a.tile_j);
+    const int cpl = a.tile_j == wf_tb::kW - 2 * a.depth ? 1 : (a.tile_j == 2 * wf_tb::kW - 2 * a.depth ? 2 : 0);
+    if (cpl == 0) {
+        wf::set_error("wf_fdtd2d_tb_advance: tile_j must be 64*cpl - 2*depth (%d or %d), got %d",
+                      wf_tb::kW - 2 * a.depth, 2 * wf_tb::kW - 2 * a.depth, a.tile_j);
         return WF_ERR_ARG;
     }
-    if (a.parity != 0 && a.parity != 1) {
-        wf::set_error("wf_fdtd2d_tb_advance: parity must be 0 or 1");
+    if (a.parity < 0 || a.parity > 3 || (a.parity > 1 && a.hist_stride != 0)) {
+        wf::set_error("wf_fdtd2d_tb_advance: parity must be 0..3 (2 and 3 not with a state history)");
         return WF_ERR_ARG;
     }
-    if (a.tile_i != 16 && a.tile_i != 32 && a.tile_i != 48 && a.tile_i != 64) {
-        wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d", a.tile_i);
+    if (a.tile_i != 16 && a.tile_i != 32 && a.tile_i != 48 && (a.tile_i != 64 || cpl != 1)) {
+        wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d with %d column(s) per lane", a.tile_i, cpl);
         return WF_ERR_UNSUPPORTED;
     }
-    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div ||
+    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div || cpl != 1 ||
                                                       (a.tile_i != 32 && a.tile_i != 48) || a.maxabs_out || a.stop))) {
         wf::set_error("wf_fdtd2d_tb_advance: a state history (hist_stride %lld) needs materials, IEEE division, "
-                      "tile_i 32 or 48 and no maxabs_out or stop flags", (long long)a.hist_stride);
+                      "tile_i 32 or 48 with one column per lane and no maxabs_out or stop flags", (long long)a.hist_stride);
         return WF_ERR_UNSUPPORTED;
     }
     if (a.fast_div && (a.div_lo_x != 0u || a.div_lo_y != 0u || a.div_span_x > 0x7f800000u ||
@@ -73,6 +80,9 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
     hipError_t e;
     if (a.hist_stride > 0)
         e = wf_tb_launch_mat_ieee_hist(a, stream);
+    else if (cpl == 2)
+        e = a.has_materials ? (a.fast_div ? wf_tb_launch_mat_fast_w2(a, stream) : wf_tb_launch_mat_ieee_w2(a, stream))
+                            : (a.fast_div ? wf_tb_launch_free_fast_w2(a, stream) : wf_tb_launch_free_ieee_w2(a, stream));
     else if (a.has_materials)
         e = a.fast_div ? wf_tb_launch_mat_fast(a, stream) : wf_tb_launch_mat_ieee(a, stream);
     else

@@ -1,4 +1,4 @@
-// Temporally blocked kernels, instantiated for materials=false, fast division=true.
+// Temporally blocked kernels, instantiated for materials=false, fast division=true, one column per lane.
 #include "wf_tb_kernel.hpp"
 
-WF_TB_DEFINE(wf_tb_launch_free_fast, false, true)
+WF_TB_DEFINE(wf_tb_launch_free_fast, false, true, 1)

@@ -1,19 +1,22 @@
 // Temporally blocked, multi-run 2D TM FDTD advance (the throughput path).
 //
-// Work decomposition. One workgroup (4 wavefronts for 16- and 32-row tiles,
-// 8 for 48 and 64 rows) owns one core tile of one run: TI rows x TJ = 64 - 2D
-// columns, plus a halo of D cells per side, so every row of the staged region
-// is exactly 64 cells = one wavefront. Of W wavefronts, wave w holds rows w,
-// w+W, w+2W, ... of the region in registers (Ex, Ey, Hz and, on the material
-// path, Ca/Cb), one column per lane. The workgroup advances the
-// region D leapfrog steps without touching HBM, shrinking the valid region by
-// one cell per side per step, then writes the core to the second state buffer
-// (ping-pong: no workgroup reads what another writes in the same launch).
+// Work decomposition. One workgroup (4 wavefronts for 16- and 32-row tiles of
+// one column per lane, 8 otherwise) owns one core tile of one run: TI rows x
+// TJ = W - 2D columns, plus a halo of D cells per side. Every row of the
+// staged region is W = 64*CPL cells, one wavefront with CPL adjacent columns
+// per lane (lane l holds columns CPL*l .. CPL*l + CPL-1). Of NW wavefronts,
+// wave w holds rows w, w+NW, w+2NW, ... of the region in registers (Ex, Ey, Hz
+// and, on the material path, Ca/Cb). The workgroup advances the region D
+// leapfrog steps without touching HBM, shrinking the valid region by one cell
+// per side per step, then writes the core to another state buffer (no
+// workgroup reads what another writes in the same launch).
 // HBM traffic is ~(20*R + 12)/D bytes per cell-update (R = region/core area)
-// instead of the 32 B of one step per pass.
+// instead of the 32 B of one step per pass. Two columns per lane halve the
+// share of halo columns (a 128-wide region keeps 112 columns at D = 8, a
+// 64-wide one 48) at twice the registers per staged row.
 //
-// Neighbours: the j-neighbours of a cell live in the adjacent lane of the same
-// wavefront and are fetched with DPP whole-wave shifts; the i-neighbours live
+// Neighbours: the j-neighbours of a cell are the lane's other column or the
+// adjacent lane's, fetched with a DPP whole-wave shift; the i-neighbours live
 // in another wavefront and go through LDS, which holds only the Hz and Ey rows
 // (Faraday reads Ey(i+1, j), Ampere reads Hz(i-1, j)).
 //
@@ -29,8 +32,12 @@
 //      snapshot of the face rows and columns and those next to them, kept in
 //      LDS) writes the Mur faces (src/core/boundaries.py:259-274), and the rows
 //      are reloaded from LDS; interior tiles skip both;
-//   3. Ampere (src/core/fdtd2d.py:244-257); Hz detectors owned by the core are
-//      read from LDS (examples/brain_mimo_imaging.py:196-198).
+//   3. Ampere (src/core/fdtd2d.py:244-257) on every lane and row of the
+//      computed range; the cells the reference never writes (Ex of column
+//      j = 0, Ey of row i = 0) are saved before it and put back after it on
+//      the tiles that hold them, so interior tiles pay no per-cell select;
+//      Hz detectors owned by the core are read from LDS
+//      (examples/brain_mimo_imaging.py:196-198).
 //
 // Division by fp32(dx) / fp32(dy): IEEE division, or (FAST) the reciprocal
 // product corrected by one fused residual step taken at a scale s = 2^k that
@@ -40,6 +47,12 @@
 // equal to IEEE x/d for every |x| below a bound h (wf_div_verify, exhaustive
 // per divisor); NaN and |x| >= h fall back to IEEE division.
 //
+// Guards. A source step past the end of its waveform table injects nothing
+// and sets WF_STATUS_WAVE_END in the run's status word; a tile region with
+// more source groups, or a tile core with more detectors, than the kernel
+// stages sets WF_STATUS_TILE_GROUPS / WF_STATUS_TILE_DETS (the extra ones are
+// not applied). wf_status_check reports both as errors.
+//
 // Tiles are mapped XCD-aware: the dispatcher deals workgroups round-robin to
 // the 8 XCDs, so block b runs on XCD b%8; we hand each XCD a contiguous band
 // of tiles so neighbouring halos are read through the same L2.
@@ -48,7 +61,7 @@
 
 namespace wf_tb {
 
-constexpr int kW = 64;          // region width = wavefront size
+constexpr int kW = 64;          // lanes per wavefront
 constexpr int kMaxGroups = 64;  // source groups per tile region (host checks)
 constexpr int kMaxDets = 64;    // detectors per tile core (host checks)
 
@@ -101,24 +114,35 @@ __device__ __forceinline__ float divide(float x, const Divider &v) {
     return q;
 }
 
-// The two quotients of one Faraday or Ampere row with one range ballot and
-// one branch between them: the stepper's instruction stream is as much
-// scalar as vector, so the branch per division costs as much as its range test.
-template <bool FAST>
-__device__ __forceinline__ void divide2(float x1, const Divider &v1, float x2, const Divider &v2, float &q1, float &q2) {
+// The 2*N quotients of one Faraday or Ampere row (N columns per lane, two
+// divisions per cell) with one range ballot and one branch between them: the
+// stepper's instruction stream is as much scalar as vector, so a branch per
+// division would cost as much as its range test.
+template <bool FAST, int N>
+__device__ __forceinline__ void divide_row(const float (&x1)[N], const Divider &v1, const float (&x2)[N],
+                                           const Divider &v2, float (&q1)[N], float (&q2)[N]) {
     if (!FAST) {
-        q1 = x1 / v1.d;
-        q2 = x2 / v2.d;
+#pragma unroll
+        for (int h = 0; h < N; ++h) {
+            q1[h] = x1[h] / v1.d;
+            q2[h] = x2[h] / v2.d;
+        }
         return;
     }
-    q1 = fast_quotient(x1, v1);
-    q2 = fast_quotient(x2, v2);
-    const bool s1 = !(__builtin_fabsf(x1) < v1.h);
-    const bool s2 = !(__builtin_fabsf(x2) < v2.h);
-    if (__builtin_expect(__ballot(s1 || s2) != 0ull, 0)) {
-        const float qa = x1 / v1.d, qb = x2 / v2.d;
-        q1 = s1 ? qa : q1;
-        q2 = s2 ? qb : q2;
+    bool any = false;
+#pragma unroll
+    for (int h = 0; h < N; ++h) {
+        q1[h] = fast_quotient(x1[h], v1);
+        q2[h] = fast_quotient(x2[h], v2);
+        any = any || !(__builtin_fabsf(x1[h]) < v1.h) || !(__builtin_fabsf(x2[h]) < v2.h);
+    }
+    if (__builtin_expect(__ballot(any) != 0ull, 0)) {
+#pragma unroll
+        for (int h = 0; h < N; ++h) {
+            const float qa = x1[h] / v1.d, qb = x2[h] / v2.d;
+            q1[h] = !(__builtin_fabsf(x1[h]) < v1.h) ? qa : q1[h];
+            q2[h] = !(__builtin_fabsf(x2[h]) < v2.h) ? qb : q2[h];
+        }
     }
 }
 
@@ -132,42 +156,67 @@ __device__ __forceinline__ unsigned int row_bits(int w, int lo, int hi) {
     return (unsigned int)(((1ull << khi) - 1ull) & ~((1ull << klo) - 1ull));
 }
 
-// Wavefronts per workgroup, rows per lane and the occupancy the register
-// budget is sized for. Tiles of 48 or more rows use 8 wavefronts (the halo
-// rows are a smaller share of a taller region), shorter tiles 4. Each lane
-// keeps 5 values per row (Ex, Ey, Hz, Ca, Cb) plus 40-48 registers of
-// temporaries; without the cap the scheduler hoists every row's loads and runs
-// at one or two waves per SIMD. An 8-wavefront workgroup puts two waves on
-// each SIMD, so its occupancy is 4 or 6 waves per SIMD: 6 (80 registers) when
-// the rows fit.
-template <int TI>
-constexpr int kWavesOf = TI >= 48 ? 8 : 4;
-template <int TI>
-constexpr int kThreadsOf = kWavesOf<TI> * kW;
+// CPL consecutive floats of one LDS row at a lane's first column (8-byte
+// accesses for two columns per lane).
+template <int CPL>
+__device__ __forceinline__ void lds_get(const float *p, float (&v)[CPL]) {
+    if constexpr (CPL == 2) {
+        const float2 t = *reinterpret_cast<const float2 *>(p);
+        v[0] = t.x;
+        v[1] = t.y;
+    } else {
+        v[0] = p[0];
+    }
+}
+template <int CPL>
+__device__ __forceinline__ void lds_put(float *p, const float (&v)[CPL]) {
+    if constexpr (CPL == 2) {
+        *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
+    } else {
+        p[0] = v[0];
+    }
+}
 
-template <int TI, int D>
+// Wavefronts per workgroup, rows per lane and the occupancy the register
+// budget is sized for. One column per lane: tiles of 48 or more rows use 8
+// wavefronts (the halo rows are a smaller share of a taller region), shorter
+// tiles 4. Two columns per lane: always 8. Each lane keeps 5*CPL values per
+// row (Ex, Ey, Hz, Ca, Cb) plus 40-48 registers of temporaries; without the
+// cap the scheduler hoists every row's loads and runs at one or two waves per
+// SIMD. An 8-wavefront workgroup puts two waves on each SIMD, so its
+// occupancy is 4 or 6 waves per SIMD: 6 (80 registers) when the rows fit.
+template <int TI, int CPL>
+constexpr int kWavesOf = (CPL == 2 || TI >= 32) ? 8 : 4;
+template <int TI, int CPL>
+constexpr int kThreadsOf = kWavesOf<TI, CPL> * kW;
+
+template <int TI, int D, int CPL>
 struct TileGeom {
-    static constexpr int NW = kWavesOf<TI>;
+    static constexpr int NW = kWavesOf<TI, CPL>;
     static constexpr int NT = NW * kW;
+    static constexpr int W = kW * CPL;
     static constexpr int NR = (TI + 2 * D + NW - 1) / NW;
-    static constexpr int kRegs = 5 * NR + (NW == 8 ? 40 : 48);
-    static constexpr int kWaves = NW == 8 ? (kRegs <= 80 ? 6 : 4) : (512 / kRegs > 8 ? 8 : 512 / kRegs);
+    static constexpr int kRegs = 5 * NR * CPL + (NW == 8 ? 40 : 48);
+    static constexpr int kWaves = NW == 8 ? (kRegs <= 72 ? 8 : (kRegs <= 80 ? 6 : (kRegs <= 128 ? 4 : 2)))
+                                          : (512 / kRegs > 8 ? 8 : 512 / kRegs);
     static_assert(NR <= 32, "row flags are one 32-bit word per wavefront");
-    static_assert(NT >= 2 * kW + TI + 2 * D, "edge passes need a thread per face cell");
+    static_assert(NT >= 2 * W + TI + 2 * D, "edge passes need a thread per face cell");
 };
 
 // HIST: write the core after every step s to the written state + (s-1)*hist_stride
 // (a state history for the adjoint's recompute) instead of only after the last.
-template <int TI, int D, bool MAT, bool FAST, bool HIST = false>
-__global__ void __launch_bounds__(kThreadsOf<TI>) __attribute__((amdgpu_waves_per_eu(TileGeom<TI, D>::kWaves)))
+template <int TI, int D, bool MAT, bool FAST, bool HIST = false, int CPL = 1>
+__global__ void __launch_bounds__((kThreadsOf<TI, CPL>)) __attribute__((amdgpu_waves_per_eu((TileGeom<TI, D, CPL>::kWaves))))
 k_tb(wf_tb_args a) {
-    constexpr int TJ = kW - 2 * D;
+    using G = TileGeom<TI, D, CPL>;
+    constexpr int W = G::W;
+    constexpr int TJ = W - 2 * D;
     constexpr int RI = TI + 2 * D;
-    constexpr int NW = TileGeom<TI, D>::NW;
-    constexpr int kNT = TileGeom<TI, D>::NT;
-    constexpr int NR = TileGeom<TI, D>::NR;
-    __shared__ float s_hz[RI * kW];
-    __shared__ float s_ey[RI * kW];
+    constexpr int NW = G::NW;
+    constexpr int kNT = G::NT;
+    constexpr int NR = G::NR;
+    __shared__ __align__(16) float s_hz[RI * W];  // 8-byte accesses with two columns per lane
+    __shared__ __align__(16) float s_ey[RI * W];
     __shared__ int s_grp[kMaxGroups];
     __shared__ int s_grp_cell[kMaxGroups];
     __shared__ int s_det[kMaxDets];
@@ -175,9 +224,11 @@ k_tb(wf_tb_args a) {
     __shared__ int s_ngrp, s_ndet, s_ey_src;
     __shared__ unsigned int s_rowmask[NW];  // per wavefront: bit k set if row w+NW*k holds a source
     __shared__ unsigned int s_max[3];
-    __shared__ float s_prow[2][kW];  // edge tiles: snapshot of rows i = 1 and i = nx-2
+    __shared__ float s_prow[2][W];   // edge tiles: snapshot of rows i = 1 and i = nx-2
     __shared__ float s_pcol[2][RI];  // edge tiles: snapshot of columns j = 1 and j = ny-2
     __shared__ float s_fcol[2][RI];  // edge tiles: snapshot of the face columns j = 0 and j = ny-1
+    __shared__ float s_keep_ex[RI];  // left tiles: Ex of column j = 0 before Ampere
+    __shared__ __align__(16) float s_keep_ey[W];  // top tiles: Ey of row i = 0 before Ampere
 
     const int tid = threadIdx.x;
     const int lane = tid & (kW - 1);
@@ -193,15 +244,16 @@ k_tb(wf_tb_args a) {
     const int I0 = (tile / ntj) * TI;
     const int J0 = (tile % ntj) * TJ;
     const int gi0 = I0 - D, gj0 = J0 - D;  // global coordinates of region (0, 0)
-    const int gj = gj0 + lane;
-    const bool col_in = gj >= 0 && gj < ny;
+    const int c0 = lane * CPL;             // region column of the lane's first cell
 
-    const float *src_ex = a.parity ? R.ex2 : R.ex;
-    const float *src_ey = a.parity ? R.ey2 : R.ey;
-    const float *src_hz = a.parity ? R.hz2 : R.hz;
-    float *dst_ex = a.parity ? R.ex : R.ex2;
-    float *dst_ey = a.parity ? R.ey : R.ey2;
-    float *dst_hz = a.parity ? R.hz : R.hz2;
+    // parity 0: A -> B, 1: B -> A, 2: A -> C, 3: C -> B (A = ex/ey/hz, B = ex2.., C = ex3..)
+    const int par = a.parity;
+    const float *src_ex = par == 1 ? R.ex2 : (par == 3 ? R.ex3 : R.ex);
+    const float *src_ey = par == 1 ? R.ey2 : (par == 3 ? R.ey3 : R.ey);
+    const float *src_hz = par == 1 ? R.hz2 : (par == 3 ? R.hz3 : R.hz);
+    float *dst_ex = par == 1 ? R.ex : (par == 2 ? R.ex3 : R.ex2);
+    float *dst_ey = par == 1 ? R.ey : (par == 2 ? R.ey3 : R.ey2);
+    float *dst_hz = par == 1 ? R.hz : (par == 2 ? R.hz3 : R.hz2);
 
     if (a.stop != nullptr && a.stop[run] != 0) {
         // The run failed a stability check earlier in this stream: carry its
@@ -209,12 +261,14 @@ k_tb(wf_tb_args a) {
         // the failing check.
         for (int k = w; k < TI; k += NW) {
             const int gi = I0 + k;
-            const int cj = J0 + lane;
-            if (gi < nx && lane < TJ && cj < ny) {
-                const size_t g = (size_t)gi * ny + cj;
-                dst_ex[g] = src_ex[g];
-                dst_ey[g] = src_ey[g];
-                dst_hz[g] = src_hz[g];
+            for (int c = lane; c < TJ; c += kW) {
+                const int cj = J0 + c;
+                if (gi < nx && cj < ny) {
+                    const size_t g = (size_t)gi * ny + cj;
+                    dst_ex[g] = src_ex[g];
+                    dst_ey[g] = src_ey[g];
+                    dst_hz[g] = src_hz[g];
+                }
             }
         }
         return;
@@ -228,39 +282,50 @@ k_tb(wf_tb_args a) {
     }
     if (tid < NW) s_rowmask[tid] = 0u;
 
-    float ex[NR], ey[NR], hz[NR], ca[NR], cb[NR];
+    bool col_in[CPL];
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) {
+        const int gj = gj0 + c0 + h;
+        col_in[h] = gj >= 0 && gj < ny;
+    }
+    float ex[NR][CPL], ey[NR][CPL], hz[NR][CPL], ca[NR][CPL], cb[NR][CPL];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int li = w + NW * k;
         const int gi = gi0 + li;
-        ex[k] = ey[k] = hz[k] = 0.f;
-        ca[k] = cb[k] = 0.f;
-        if (li < RI) {
-            if (gi >= 0 && gi < nx && col_in) {
-                const size_t g = (size_t)gi * ny + gj;
-                ex[k] = src_ex[g];
-                ey[k] = src_ey[g];
-                hz[k] = src_hz[g];
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            ex[k][h] = ey[k][h] = hz[k][h] = 0.f;
+            ca[k][h] = cb[k][h] = 0.f;
+            if (li < RI && gi >= 0 && gi < nx && col_in[h]) {
+                const size_t g = (size_t)gi * ny + (gj0 + c0 + h);
+                ex[k][h] = src_ex[g];
+                ey[k][h] = src_ey[g];
+                hz[k][h] = src_hz[g];
                 if (MAT) {
-                    ca[k] = R.ca[g];
-                    cb[k] = R.cb[g];
+                    ca[k][h] = R.ca[g];
+                    cb[k][h] = R.cb[g];
                 }
             }
-            s_hz[li * kW + lane] = hz[k];
-            s_ey[li * kW + lane] = ey[k];
+        }
+        if (li < RI) {
+            lds_put<CPL>(&s_hz[li * W + c0], hz[k]);
+            lds_put<CPL>(&s_ey[li * W + c0], ey[k]);
         }
     }
     __syncthreads();
     for (int g = tid; g < R.n_grp; g += kNT) {
         const wf_inj2d e0 = R.inj[R.grp_start[g]];
         const int li = e0.cell / ny - gi0, lj = e0.cell % ny - gj0;
-        if (li >= 0 && li < RI && lj >= 0 && lj < kW) {
+        if (li >= 0 && li < RI && lj >= 0 && lj < W) {
             const int k = atomicAdd(&s_ngrp, 1);
             if (k < kMaxGroups) {
                 s_grp[k] = g;
-                s_grp_cell[k] = li * kW + lj;
+                s_grp_cell[k] = li * W + lj;
                 atomicOr(&s_rowmask[li % NW], 1u << (li / NW));
                 if (e0.comp == WF_COMP_EY) s_ey_src = 1;
+            } else if (R.status) {
+                atomicOr(R.status, (unsigned int)WF_STATUS_TILE_GROUPS);
             }
         }
     }
@@ -271,7 +336,9 @@ k_tb(wf_tb_args a) {
             const int k = atomicAdd(&s_ndet, 1);
             if (k < kMaxDets) {
                 s_det[k] = d;
-                s_det_cell[k] = (gi - gi0) * kW + (gjd - gj0);
+                s_det_cell[k] = (gi - gi0) * W + (gjd - gj0);
+            } else if (R.status) {
+                atomicOr(R.status, (unsigned int)WF_STATUS_TILE_DETS);
             }
         }
     }
@@ -280,12 +347,14 @@ k_tb(wf_tb_args a) {
     const int ndet = min(s_ndet, kMaxDets);
     const bool ey_src = s_ey_src != 0;
     const unsigned int rowmask = s_rowmask[w];
-    const bool edge = (gi0 <= 0) || (gi0 + RI >= nx) || (gj0 <= 0) || (gj0 + kW >= ny);
+    const bool edge = (gi0 <= 0) || (gi0 + RI >= nx) || (gj0 <= 0) || (gj0 + W >= ny);
     const float dh = R.dh, de = R.de, cx = R.cmx, cy = R.cmy;
     const Divider divx = make_divider(R.dx, a.rdx, a.div_span_x);
     const Divider divy = make_divider(R.dy, a.rdy, a.div_span_y);
     (void)de;
-    const bool lane_core = lane >= D && lane < D + TJ && col_in;
+    bool lane_core[CPL];
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) lane_core[h] = c0 + h >= D && c0 + h < D + TJ && col_in[h];
     // the step index, read once: the kernel's own stores (traces) could alias the
     // descriptor as far as the compiler knows, so a read inside the loop would
     // be a global load and wait on every step
@@ -299,16 +368,16 @@ k_tb(wf_tb_args a) {
                 int e_i0 = gi0, e_j0 = gj0;
                 asm volatile("" : "+s"(e_i0), "+s"(e_j0));
                 const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
-                const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;  // region lanes of j = 1, ny-2
-                if (tid < kW) {
-                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * kW + tid];
-                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * kW + tid];
-                } else if (tid >= 2 * kW && tid < 2 * kW + RI) {
-                    const int li = tid - 2 * kW;
-                    if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
-                    if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
-                    if (c1 >= 1 && c1 <= kW) s_fcol[0][li] = s_hz[li * kW + c1 - 1];
-                    if (c2 >= -1 && c2 + 1 < kW) s_fcol[1][li] = s_hz[li * kW + c2 + 1];
+                const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;  // region columns of j = 1, ny-2
+                if (tid < W) {
+                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * W + tid];
+                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * W + tid];
+                } else if (tid >= 2 * W && tid < 2 * W + RI) {
+                    const int li = tid - 2 * W;
+                    if (q1 >= 0 && q1 < W) s_pcol[0][li] = s_hz[li * W + q1];
+                    if (q2 >= 0 && q2 < W) s_pcol[1][li] = s_hz[li * W + q2];
+                    if (q1 >= 1 && q1 <= W) s_fcol[0][li] = s_hz[li * W + q1 - 1];
+                    if (q2 >= -1 && q2 + 1 < W) s_fcol[1][li] = s_hz[li * W + q2 + 1];
                 }
             }
         __syncthreads();
@@ -332,18 +401,26 @@ k_tb(wf_tb_args a) {
                     const int li = w + NW * k;
                     for (int q = 0; q < ngrp; ++q) {
                         const int cl = s_grp_cell[q];
-                        if (cl == li * kW + lane) {
-                            const int g = s_grp[q];
-                            const int b0 = R.grp_start[g], b1 = R.grp_start[g + 1];
-                            const int comp = R.inj[b0].comp;
-                            float v = comp == WF_COMP_EX ? ex[k] : (comp == WF_COMP_EY ? ey[k] : hz[k]);
-                            for (int e = b0; e < b1; ++e) {
-                                const float wv = R.inj[e].wave[n] * R.inj[e].amp;
-                                v = v + wv;
+#pragma unroll
+                        for (int h = 0; h < CPL; ++h) {
+                            if (cl == li * W + c0 + h) {
+                                const int g = s_grp[q];
+                                const int b0 = R.grp_start[g], b1 = R.grp_start[g + 1];
+                                const int comp = R.inj[b0].comp;
+                                float v = comp == WF_COMP_EX ? ex[k][h] : (comp == WF_COMP_EY ? ey[k][h] : hz[k][h]);
+                                for (int e = b0; e < b1; ++e) {
+                                    const wf_inj2d &inj = R.inj[e];
+                                    if ((unsigned)n < (unsigned)inj.wave_len) {
+                                        const float wv = inj.wave[n] * inj.amp;
+                                        v = v + wv;
+                                    } else if (R.status) {
+                                        atomicOr(R.status, (unsigned int)WF_STATUS_WAVE_END);
+                                    }
+                                }
+                                if (comp == WF_COMP_EX) ex[k][h] = v;
+                                else if (comp == WF_COMP_EY) { ey[k][h] = v; s_ey[cl] = v; }
+                                else hz[k][h] = v;
                             }
-                            if (comp == WF_COMP_EX) ex[k] = v;
-                            else if (comp == WF_COMP_EY) { ey[k] = v; s_ey[cl] = v; }
-                            else hz[k] = v;
                         }
                     }
                 }
@@ -363,17 +440,24 @@ k_tb(wf_tb_args a) {
         for (int k = 0; k < NR; ++k) {
             const int li = w + NW * k;
             if (fmask & (1u << k)) {
-                const float ex_next = lane_next(ex[k]);
-                const int c = li * kW + lane;
-                const float ey_down = s_ey[c + kW];
-                float av, bv;
-                divide2<FAST>(ex_next - ex[k], divy, ey_down - ey[k], divx, av, bv);
-                const float F = hz[k] + dh * (av - bv);
-                // every lane takes F: on the face columns the Mur pass below
-                // overwrites it (from the snapshot kept above), and lanes
-                // outside the grid feed only each other
-                hz[k] = F;
-                s_hz[c] = F;
+                const int c = li * W + c0;
+                float eyd[CPL], x1[CPL], x2[CPL], av[CPL], bv[CPL];
+                lds_get<CPL>(&s_ey[c + W], eyd);
+#pragma unroll
+                for (int h = 0; h < CPL; ++h) {
+                    const float exn = h + 1 < CPL ? ex[k][h + 1] : lane_next(ex[k][0]);
+                    x1[h] = exn - ex[k][h];
+                    x2[h] = eyd[h] - ey[k][h];
+                }
+                divide_row<FAST, CPL>(x1, divy, x2, divx, av, bv);
+#pragma unroll
+                for (int h = 0; h < CPL; ++h) {
+                    // every lane takes F: on the face columns the Mur pass below
+                    // overwrites it (from the snapshot kept above), and lanes
+                    // outside the grid feed only each other
+                    hz[k][h] = hz[k][h] + dh * (av[h] - bv[h]);
+                }
+                lds_put<CPL>(&s_hz[c], hz[k]);
             }
         }
         __syncthreads();
@@ -386,34 +470,34 @@ k_tb(wf_tb_args a) {
             int e_i0 = gi0, e_j0 = gj0;
             asm volatile("" : "+s"(e_i0), "+s"(e_j0));
             const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;
-            const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;
-            if (tid < 2 * kW) {  // x faces: rows i = 0 (from i = 1) and i = nx-1 (from i = nx-2)
-                const int side = tid >> 6, l = tid & (kW - 1);
+            const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;
+            if (tid < 2 * W) {  // x faces: rows i = 0 (from i = 1) and i = nx-1 (from i = nx-2)
+                const int side = tid >= W, l = tid - side * W;
                 const int rin = side ? r2 : r1, rf = side ? rin + 1 : rin - 1;
                 const int gjl = e_j0 + l;
                 if (rin >= s - 1 && rin < RI - s && rf >= 0 && rf < RI && gjl >= 1 && gjl <= ny - 2) {
-                    const float p_in = s_prow[side][l], p_f = s_hz[rf * kW + l];
-                    const float L = p_in + cx * (s_hz[rin * kW + l] - p_f);
+                    const float p_in = s_prow[side][l], p_f = s_hz[rf * W + l];
+                    const float L = p_in + cx * (s_hz[rin * W + l] - p_f);
                     if (gjl == 1 && l >= 1) {
-                        const float pc = s_hz[rf * kW + l - 1];
-                        s_hz[rf * kW + l - 1] = p_f + cy * (L - pc);
+                        const float pc = s_hz[rf * W + l - 1];
+                        s_hz[rf * W + l - 1] = p_f + cy * (L - pc);
                     }
-                    if (gjl == ny - 2 && l + 1 < kW) {
-                        const float pc = s_hz[rf * kW + l + 1];
-                        s_hz[rf * kW + l + 1] = p_f + cy * (L - pc);
+                    if (gjl == ny - 2 && l + 1 < W) {
+                        const float pc = s_hz[rf * W + l + 1];
+                        s_hz[rf * W + l + 1] = p_f + cy * (L - pc);
                     }
-                    s_hz[rf * kW + l] = L;
+                    s_hz[rf * W + l] = L;
                 }
-            } else if (tid - 2 * kW < RI) {  // y faces: j = 0 (from j = 1) and j = ny-1 (from j = ny-2)
-                const int li = tid - 2 * kW, gil = e_i0 + li;
+            } else if (tid - 2 * W < RI) {  // y faces: j = 0 (from j = 1) and j = ny-1 (from j = ny-2)
+                const int li = tid - 2 * W, gil = e_i0 + li;
                 if (li >= s - 1 && li < RI - s && gil >= 1 && gil <= nx - 2) {
-                    if (c1 >= 1 && c1 < kW) {
+                    if (q1 >= 1 && q1 < W) {
                         const float pf = s_fcol[0][li];
-                        s_hz[li * kW + c1 - 1] = s_pcol[0][li] + cy * (s_hz[li * kW + c1] - pf);
+                        s_hz[li * W + q1 - 1] = s_pcol[0][li] + cy * (s_hz[li * W + q1] - pf);
                     }
-                    if (c2 >= 0 && c2 + 1 < kW) {
+                    if (q2 >= 0 && q2 + 1 < W) {
                         const float pf = s_fcol[1][li];
-                        s_hz[li * kW + c2 + 1] = s_pcol[1][li] + cy * (s_hz[li * kW + c2] - pf);
+                        s_hz[li * W + q2 + 1] = s_pcol[1][li] + cy * (s_hz[li * W + q2] - pf);
                     }
                 }
             }
@@ -425,7 +509,7 @@ k_tb(wf_tb_args a) {
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 const int li = w + NW * k;
-                if (li < RI) hz[k] = s_hz[li * kW + lane];
+                if (li < RI) lds_get<CPL>(&s_hz[li * W + c0], hz[k]);
             }
             // the next step's snapshot: LDS holds this step's final Hz, which
             // nothing rewrites before the next Faraday (Ampere and the
@@ -438,17 +522,42 @@ k_tb(wf_tb_args a) {
                 int e_i0 = gi0, e_j0 = gj0;
                 asm volatile("" : "+s"(e_i0), "+s"(e_j0));
                 const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
-                const int c1 = 1 - e_j0, c2 = ny - 2 - e_j0;  // region lanes of j = 1, ny-2
-                if (tid < kW) {
-                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * kW + tid];
-                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * kW + tid];
-                } else if (tid >= 2 * kW && tid < 2 * kW + RI) {
-                    const int li = tid - 2 * kW;
-                    if (c1 >= 0 && c1 < kW) s_pcol[0][li] = s_hz[li * kW + c1];
-                    if (c2 >= 0 && c2 < kW) s_pcol[1][li] = s_hz[li * kW + c2];
-                    if (c1 >= 1 && c1 <= kW) s_fcol[0][li] = s_hz[li * kW + c1 - 1];
-                    if (c2 >= -1 && c2 + 1 < kW) s_fcol[1][li] = s_hz[li * kW + c2 + 1];
+                const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;  // region columns of j = 1, ny-2
+                if (tid < W) {
+                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * W + tid];
+                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * W + tid];
+                } else if (tid >= 2 * W && tid < 2 * W + RI) {
+                    const int li = tid - 2 * W;
+                    if (q1 >= 0 && q1 < W) s_pcol[0][li] = s_hz[li * W + q1];
+                    if (q2 >= 0 && q2 < W) s_pcol[1][li] = s_hz[li * W + q2];
+                    if (q1 >= 1 && q1 <= W) s_fcol[0][li] = s_hz[li * W + q1 - 1];
+                    if (q2 >= -1 && q2 + 1 < W) s_fcol[1][li] = s_hz[li * W + q2 + 1];
                 }
+            }
+            // The cells Ampere must leave as they are: Ex of column j = 0 and
+            // Ey of row i = 0 (src/core/fdtd2d.py:244-257 writes Ex[:, 1:] and
+            // Ey[1:, :]); kept per lane (LDS written and read by the same
+            // lane) and put back after Ampere.
+            int e_j0b = gj0;
+            asm volatile("" : "+s"(e_j0b));
+            if (e_j0b <= 0) {
+                const int cz = -e_j0b;  // region column of j = 0
+                if (lane == cz / CPL) {
+#pragma unroll
+                    for (int k = 0; k < NR; ++k) {
+                        const int li = w + NW * k;
+                        float v = ex[k][0];
+#pragma unroll
+                        for (int h = 1; h < CPL; ++h) v = cz % CPL == h ? ex[k][h] : v;
+                        if (li < RI) s_keep_ex[li] = v;
+                    }
+                }
+            }
+            if (gib <= 0 && w == (-gib) % NW) {
+                const int kz = (-gib) / NW;  // wave w holds region row -gib (i = 0) as its row kz
+#pragma unroll
+                for (int k = 0; k < NR; ++k)
+                    if (k == kz) lds_put<CPL>(&s_keep_ey[c0], ey[k]);
             }
         }
         // 3. Ampere + detectors.
@@ -456,55 +565,87 @@ k_tb(wf_tb_args a) {
             const int tt = n - R.trace_n0;
             if (tt >= 0 && tt < R.trace_len) R.traces[s_det[tid] * R.trace_stride + tt] = s_hz[s_det_cell[tid]];
         }
-        // rows this step's Ampere updates: s <= li < RI-s and 0 <= gi < nx; Ey
-        // only on rows with gi >= 1
+        // rows this step's Ampere updates: s <= li < RI-s and 0 <= gi < nx
         unsigned int amask = row_bits<NW, NR>(w, max(s, -gib), min(RI - s, nx - gib));
-        unsigned int eymask = row_bits<NW, NR>(w, 1 - gib, RI);
-        asm volatile("" : "+s"(amask), "+s"(eymask));
+        asm volatile("" : "+s"(amask));
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const int li = w + NW * k;
             if (amask & (1u << k)) {
-                const float hz_left = lane_prev(hz[k]);
-                const int c = li * kW + lane;
-                const float hz_up = s_hz[c - kW];
-                float gyv, gxv;
-                divide2<FAST>(hz[k] - hz_left, divy, hz[k] - hz_up, divx, gyv, gxv);
-                {
-                    float nex, ney;
-                    if (MAT) {
-                        const float t1 = ca[k] * ex[k];
-                        const float t2 = cb[k] * gyv;
-                        nex = t1 + t2;
-                        const float u1 = ca[k] * ey[k];
-                        const float u2 = cb[k] * gxv;
-                        ney = u1 - u2;
-                    } else {
-                        const float t2 = de * gyv;
-                        nex = ex[k] + t2;
-                        const float u2 = de * gxv;
-                        ney = ey[k] - u2;
-                    }
+                const int c = li * W + c0;
+                float hzu[CPL], x3[CPL], x4[CPL], gyv[CPL], gxv[CPL];
+                lds_get<CPL>(&s_hz[c - W], hzu);
+#pragma unroll
+                for (int h = 0; h < CPL; ++h) {
+                    const float hzl = h > 0 ? hz[k][h - 1] : lane_prev(hz[k][CPL - 1]);
+                    x3[h] = hz[k][h] - hzl;
+                    x4[h] = hz[k][h] - hzu[h];
+                }
+                divide_row<FAST, CPL>(x3, divy, x4, divx, gyv, gxv);
+#pragma unroll
+                for (int h = 0; h < CPL; ++h) {
                     // lanes outside the grid update too: they feed only each
                     // other and the discarded Faraday values of the face columns
-                    ex[k] = gj >= 1 ? nex : ex[k];
-                    if (eymask & (1u << k)) ey[k] = ney;
-                    s_ey[c] = ey[k];
+                    if (MAT) {
+                        const float t1 = ca[k][h] * ex[k][h];
+                        const float t2 = cb[k][h] * gyv[h];
+                        ex[k][h] = t1 + t2;
+                        const float u1 = ca[k][h] * ey[k][h];
+                        const float u2 = cb[k][h] * gxv[h];
+                        ey[k][h] = u1 - u2;
+                    } else {
+                        const float t2 = de * gyv[h];
+                        ex[k][h] = ex[k][h] + t2;
+                        const float u2 = de * gxv[h];
+                        ey[k][h] = ey[k][h] - u2;
+                    }
                 }
+                lds_put<CPL>(&s_ey[c], ey[k]);
+            }
+        }
+        if (edge) {
+            int e_j0b = gj0;
+            asm volatile("" : "+s"(e_j0b));
+            if (e_j0b <= 0) {
+                const int cz = -e_j0b;
+                if (lane == cz / CPL) {
+#pragma unroll
+                    for (int k = 0; k < NR; ++k) {
+                        const int li = w + NW * k;
+                        if (li < RI) {
+                            const float v = s_keep_ex[li];
+#pragma unroll
+                            for (int h = 0; h < CPL; ++h)
+                                if (cz % CPL == h) ex[k][h] = v;
+                        }
+                    }
+                }
+            }
+            if (gib <= 0 && w == (-gib) % NW) {
+                const int kz = (-gib) / NW;
+#pragma unroll
+                for (int k = 0; k < NR; ++k)
+                    if (k == kz) {
+                        lds_get<CPL>(&s_keep_ey[c0], ey[k]);
+                        lds_put<CPL>(&s_ey[(w + NW * k) * W + c0], ey[k]);
+                    }
             }
         }
         __syncthreads();
         if (HIST) {
-            const int64_t h = (int64_t)(s - 1) * a.hist_stride;
+            const int64_t hoff = (int64_t)(s - 1) * a.hist_stride;
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 const int li = w + NW * k;
                 const int gi = gi0 + li;
-                if (li >= D && li < D + TI && gi < nx && lane_core) {
-                    const int64_t g = h + (int64_t)gi * ny + gj;
-                    dst_ex[g] = ex[k];
-                    dst_ey[g] = ey[k];
-                    dst_hz[g] = hz[k];
+#pragma unroll
+                for (int h = 0; h < CPL; ++h) {
+                    if (li >= D && li < D + TI && gi < nx && lane_core[h]) {
+                        const int64_t g = hoff + (int64_t)gi * ny + (gj0 + c0 + h);
+                        dst_ex[g] = ex[k][h];
+                        dst_ey[g] = ey[k][h];
+                        dst_hz[g] = hz[k][h];
+                    }
                 }
             }
         }
@@ -517,14 +658,17 @@ k_tb(wf_tb_args a) {
     for (int k = 0; k < NR; ++k) {
         const int li = w + NW * k;
         const int gi = gi0 + li;
-        if (li >= D && li < D + TI && gi < nx && lane_core) {
-            const size_t g = (size_t)gi * ny + gj;
-            dst_ex[g] = ex[k];
-            dst_ey[g] = ey[k];
-            dst_hz[g] = hz[k];
-            m0 = max(m0, wf::abs_bits(ex[k]));
-            m1 = max(m1, wf::abs_bits(ey[k]));
-            m2 = max(m2, wf::abs_bits(hz[k]));
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            if (li >= D && li < D + TI && gi < nx && lane_core[h]) {
+                const size_t g = (size_t)gi * ny + (gj0 + c0 + h);
+                dst_ex[g] = ex[k][h];
+                dst_ey[g] = ey[k][h];
+                dst_hz[g] = hz[k][h];
+                m0 = max(m0, wf::abs_bits(ex[k][h]));
+                m1 = max(m1, wf::abs_bits(ey[k][h]));
+                m2 = max(m2, wf::abs_bits(hz[k][h]));
+            }
         }
     }
     if (a.maxabs_out) {
@@ -536,37 +680,14 @@ k_tb(wf_tb_args a) {
     }
 }
 
-template <int TI, bool MAT, bool FAST>
+template <int TI, bool MAT, bool FAST, bool HIST = false, int CPL = 1>
 hipError_t launch_depth(const wf_tb_args &a, hipStream_t st) {
     const unsigned blocks = 8u * (unsigned)((a.n_tiles + 7) / 8);
     switch (a.depth) {
-#define WF_TB_CASE(DD)                                                                         \
-    case DD:                                                                                   \
-        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST>), dim3(blocks), dim3(TileGeom<TI, DD>::NT), 0, st, a); \
-        break;
-        WF_TB_CASE(1)
-        WF_TB_CASE(2)
-        WF_TB_CASE(3)
-        WF_TB_CASE(4)
-        WF_TB_CASE(5)
-        WF_TB_CASE(6)
-        WF_TB_CASE(7)
-        WF_TB_CASE(8)
-#undef WF_TB_CASE
-        default:
-            return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-
-template <int TI, bool MAT, bool FAST>
-hipError_t launch_depth_hist(const wf_tb_args &a, hipStream_t st) {
-    const unsigned blocks = 8u * (unsigned)((a.n_tiles + 7) / 8);
-    switch (a.depth) {
-#define WF_TB_CASE(DD)                                                                         \
-    case DD:                                                                                   \
-        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST, true>), dim3(blocks), dim3(TileGeom<TI, DD>::NT), 0, st, a); \
+#define WF_TB_CASE(DD)                                                                                        \
+    case DD:                                                                                                  \
+        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST, HIST, CPL>), dim3(blocks), dim3(TileGeom<TI, DD, CPL>::NT), 0, \
+                           st, a);                                                                            \
         break;
         WF_TB_CASE(1)
         WF_TB_CASE(2)
@@ -585,22 +706,34 @@ hipError_t launch_depth_hist(const wf_tb_args &a, hipStream_t st) {
 
 }  // namespace wf_tb
 
-// One object file per (materials, division) pair instantiates the kernels for
-// every tile height and depth; the entry point in wf_tb.hip dispatches here.
+// One object file per (materials, division, columns per lane) instantiates the
+// kernels for every tile height and depth; the entry point in wf_tb.hip
+// dispatches here. Tile widths: tile_j = 64*CPL - 2*depth.
 #define WF_TB_DECLARE(NAME) hipError_t NAME(const wf_tb_args &a, hipStream_t st)
 WF_TB_DECLARE(wf_tb_launch_mat_fast);
 WF_TB_DECLARE(wf_tb_launch_mat_ieee);
 WF_TB_DECLARE(wf_tb_launch_free_fast);
 WF_TB_DECLARE(wf_tb_launch_free_ieee);
-WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32- or 48-row tiles
+WF_TB_DECLARE(wf_tb_launch_mat_fast_w2);
+WF_TB_DECLARE(wf_tb_launch_mat_ieee_w2);
+WF_TB_DECLARE(wf_tb_launch_free_fast_w2);
+WF_TB_DECLARE(wf_tb_launch_free_ieee_w2);
+WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32- or 48-row tiles, one column per lane
 
-#define WF_TB_DEFINE(NAME, MAT, FAST)                                                   \
-    WF_TB_DECLARE(NAME) {                                                               \
-        switch (a.tile_i) {                                                             \
-            case 16: return wf_tb::launch_depth<16, MAT, FAST>(a, st);                  \
-            case 32: return wf_tb::launch_depth<32, MAT, FAST>(a, st);                  \
-            case 48: return wf_tb::launch_depth<48, MAT, FAST>(a, st);                  \
-            case 64: return wf_tb::launch_depth<64, MAT, FAST>(a, st);                  \
-            default: return hipErrorInvalidValue;                                       \
-        }                                                                               \
+namespace wf_tb {
+template <bool MAT, bool FAST, int CPL>
+hipError_t launch_tile(const wf_tb_args &a, hipStream_t st) {
+    switch (a.tile_i) {
+        case 16: return launch_depth<16, MAT, FAST, false, CPL>(a, st);
+        case 32: return launch_depth<32, MAT, FAST, false, CPL>(a, st);
+        case 48: return launch_depth<48, MAT, FAST, false, CPL>(a, st);
+        case 64:
+            if constexpr (CPL == 1) return launch_depth<64, MAT, FAST, false, CPL>(a, st);
+            return hipErrorInvalidValue;
+        default: return hipErrorInvalidValue;
     }
+}
+}  // namespace wf_tb
+
+#define WF_TB_DEFINE(NAME, MAT, FAST, CPL) \
+    WF_TB_DECLARE(NAME) { return wf_tb::launch_tile<MAT, FAST, CPL>(a, st); }

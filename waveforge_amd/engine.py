@@ -220,6 +220,8 @@ class RunState:
     step: int = 0
     pstrip: Optional[torch.Tensor] = None
     second: Optional[tuple] = None   # (ex2, ey2, hz2)
+    third: Optional[tuple] = None    # (ex3, ey3, hz3): lets a blocked run use an odd launch count
+    status: Optional[torch.Tensor] = None  # device uint32 word: WF_STATUS_* bits the kernels raise
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self) -> None:
@@ -230,6 +232,8 @@ class RunState:
         r.ex, r.ey, r.hz = self.ex.data_ptr(), self.ey.data_ptr(), self.hz.data_ptr()
         if self.second is not None:
             r.ex2, r.ey2, r.hz2 = (t.data_ptr() for t in self.second)
+        if self.third is not None:
+            r.ex3, r.ey3, r.hz3 = (t.data_ptr() for t in self.third)
         if self.ca is not None:
             r.ca, r.cb = self.ca.data_ptr(), self.cb.data_ptr()
         if self.pstrip is not None:
@@ -248,6 +252,8 @@ class RunState:
         c = self.coef
         r.dh, r.de, r.dx, r.dy, r.cmx, r.cmy = c.dh, c.de, c.dx, c.dy, c.cmx, c.cmy
         r.step = self.step
+        if self.status is not None:
+            r.status = self.status.data_ptr()
         return r
 
 
@@ -275,15 +281,15 @@ def maxabs(ex: torch.Tensor, ey: torch.Tensor, hz: torch.Tensor, stream: int) ->
     return out
 
 
-def tile_count(nx: int, ny: int, tile_i: int, depth: int) -> int:
-    tj = _native.tile_cols(depth)
+def tile_count(nx: int, ny: int, tile_i: int, depth: int, cpl: int = 1) -> int:
+    tj = _native.tile_cols(depth, cpl)
     return ((nx + tile_i - 1) // tile_i) * ((ny + tj - 1) // tj)
 
 
-def tb_limits_ok(run: RunState, tile_i: int, depth: int) -> bool:
+def tb_limits_ok(run: RunState, tile_i: int, depth: int, cpl: int = 1) -> bool:
     """True if every tile region of ``run`` holds at most the per-tile source
     groups and core detectors the temporally blocked kernel can stage."""
-    TI, TJ = tile_i, _native.tile_cols(depth)
+    TI, TJ = tile_i, _native.tile_cols(depth, cpl)
     n_grp = run.sources.n_grp if run.sources is not None else 0
     n_det = run.det_cell.numel() if run.det_cell is not None else 0
     if n_grp <= TB_MAX_GROUPS_PER_TILE and n_det <= TB_MAX_DETS_PER_TILE and \
@@ -359,19 +365,33 @@ def verified_division(d: float, device: torch.device) -> Optional[tuple[float, i
     return _DIV_CACHE[key]
 
 
-def split_launches(n_steps: int, depth: int) -> Optional[list[int]]:
-    """Depths (each in ``[1, depth]``) of an even number of launches covering
-    ``n_steps`` steps, or ``None`` when no such split exists (odd ``n_steps``
-    with ``depth == 1``, or ``n_steps < 2``)."""
+def split_launches(n_steps: int, depth: int, odd_ok: bool = False) -> Optional[list[int]]:
+    """Depths (each in ``[1, depth]``, as equal as possible) of the fewest
+    launches covering ``n_steps`` steps; the count is even unless ``odd_ok``
+    (a third state buffer) allows an odd count of 3 or more. ``None`` when no
+    such split exists (``n_steps < 2``, or odd ``n_steps`` at ``depth == 1``
+    without ``odd_ok``)."""
     if n_steps < 2:
         return None
-    n_launch = -(-n_steps // depth)
-    if n_launch % 2:
+    n_launch = max(2, -(-n_steps // depth))
+    if n_launch % 2 and not odd_ok:
         if n_launch == n_steps:
             return None
         n_launch += 1
     base, rem = divmod(n_steps, n_launch)
     return [base + (1 if k < rem else 0) for k in range(n_launch)]
+
+
+def launch_parities(n_launch: int) -> list[int]:
+    """State-buffer codes of ``n_launch`` consecutive launches that start and
+    end in the live buffers A: 0 (A->B), 1 (B->A) alternating for an even
+    count; 2 (A->C), 3 (C->B), 1 (B->A) and then alternating for an odd
+    count of 3 or more (needs the third buffer C)."""
+    if n_launch % 2 == 0:
+        return [k % 2 for k in range(n_launch)]
+    if n_launch < 3:
+        raise ValueError("a single launch cannot end in the buffer it reads")
+    return [2, 3, 1] + [k % 2 for k in range(n_launch - 3)]
 
 
 class TBPlan:
@@ -390,8 +410,14 @@ class TBPlan:
     event_pool: Optional[list] = None
     _event_count: int = 0
 
-    def __init__(self, runs: Sequence[RunState], tile_i: int, device: torch.device, fast_div: bool = True) -> None:
+    def __init__(self, runs: Sequence[RunState], tile_i: int, device: torch.device, fast_div: bool = True,
+                 cpl: int = 1) -> None:
         self.tile_i = int(tile_i)
+        self.cpl = int(cpl)
+        self.odd_ok = all(r.third is not None for r in runs)
+        self._args: dict = {}
+        self._lib = _native.load()
+        self._streams: dict = {}
         self.runs = list(runs)
         self.device = device
         mats = {r.ca is not None for r in runs}
@@ -417,32 +443,43 @@ class TBPlan:
         if depth not in self._tiles:
             parts = []
             for k, r in enumerate(self.runs):
-                nt = tile_count(r.nx, r.ny, self.tile_i, depth)
+                nt = tile_count(r.nx, r.ny, self.tile_i, depth, self.cpl)
                 parts.append(np.stack([np.full(nt, k, dtype=np.int32), np.arange(nt, dtype=np.int32)], axis=1))
             t = np.ascontiguousarray(np.concatenate(parts, axis=0))
             self._tiles[depth] = (torch.from_numpy(t).to(self.device), int(t.shape[0]))
         return self._tiles[depth]
 
+    def _base_args(self, depth: int, parity: int) -> WfTbArgs:
+        """The launch arguments that do not change from launch to launch."""
+        key = (depth, parity)
+        a = self._args.get(key)
+        if a is None:
+            tiles, n_tiles = self.tiles(depth)
+            a = WfTbArgs()
+            a.runs = self.run_table.data_ptr()
+            a.tiles = tiles.data_ptr()
+            a.n_runs = len(self.runs)
+            a.n_tiles = n_tiles
+            a.tile_i = self.tile_i
+            a.tile_j = _native.tile_cols(depth, self.cpl)
+            a.depth = int(depth)
+            a.parity = int(parity)
+            a.has_materials = self.has_materials
+            if self.div is not None:
+                (a.rdx, a.div_lo_x, a.div_span_x), (a.rdy, a.div_lo_y, a.div_span_y) = self.div
+                a.fast_div = 1
+            self._args[key] = a
+        return a
+
     def launch(self, depth: int, parity: int, step_offset: int, stream: int,
                maxabs_out: Optional[torch.Tensor] = None, stop: Optional[torch.Tensor] = None) -> None:
-        tiles, n_tiles = self.tiles(depth)
-        a = WfTbArgs()
-        a.runs = self.run_table.data_ptr()
-        a.tiles = tiles.data_ptr()
-        a.n_runs = len(self.runs)
-        a.n_tiles = n_tiles
-        a.tile_i = self.tile_i
-        a.tile_j = _native.tile_cols(depth)
-        a.depth = int(depth)
-        a.parity = int(parity)
+        a = self._base_args(depth, parity)
         a.step_offset = int(step_offset)
-        a.has_materials = self.has_materials
         a.maxabs_out = maxabs_out.data_ptr() if maxabs_out is not None else 0
         a.stop = stop.data_ptr() if stop is not None else 0
-        if self.div is not None:
-            (a.rdx, a.div_lo_x, a.div_span_x), (a.rdy, a.div_lo_y, a.div_span_y) = self.div
-            a.fast_div = 1
-        lib = _native.load()
+        st = self._streams.get(stream)
+        if st is None:
+            st = self._streams[stream] = ctypes.c_void_p(stream)
         log = TBPlan.event_log
         if log is not None:
             TBPlan._event_count += 1
@@ -456,7 +493,9 @@ class TBPlan:
                 ev0 = torch.cuda.Event(enable_timing=True)
                 ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
-        _native.check(lib.wf_fdtd2d_tb_advance(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_fdtd2d_tb_advance")
+        rc = self._lib.wf_fdtd2d_tb_advance(ctypes.byref(a), st)
+        if rc:
+            _native.check(rc, "wf_fdtd2d_tb_advance")
         if log is not None:
             ev1.record()
             log.append((int(depth), self.n_cells, ev0, ev1))
@@ -464,12 +503,12 @@ class TBPlan:
     def advance(self, n_steps: int, depth: int, stream: int, base_step: int = 0) -> None:
         """Advance every run ``n_steps`` (>= 2) steps, ending in the live buffers.
         Run ``k`` injects and records step ``table step + base_step + offset``."""
-        depths = split_launches(n_steps, depth)
+        depths = split_launches(n_steps, depth, odd_ok=self.odd_ok)
         if depths is None:
             raise ValueError(f"cannot split {n_steps} steps into an even number of launches of depth <= {depth}")
         offset = 0
-        for k, d in enumerate(depths):
-            self.launch(d, k % 2, base_step + offset, stream)
+        for d, par in zip(depths, launch_parities(len(depths))):
+            self.launch(d, par, base_step + offset, stream)
             offset += d
 
 
@@ -481,15 +520,19 @@ def judge(maxabs: torch.Tensor, threshold: float, stop: torch.Tensor, all_runs: 
                                       stop.data_ptr(), ctypes.c_void_p(stream)), "wf_fdtd2d_judge")
 
 
-def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optional[int]):
+def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optional[int], odd_ok: bool = False):
     """Launch plan for ``n_steps`` steps from step ``start``.
 
     Returns ``(launches, checks)``: ``launches`` is a list of ``(depth, slot)``
     where ``slot`` (or -1) is the stability-check slot the launch's final
     level feeds, ``checks`` the global step of each slot. Segments end at every
-    step ``m`` with ``m % check_every == 0``; the launch count is even (so the
-    state ends in the live buffers), or ``None`` is returned if that is
-    impossible (every launch of depth 1 and an odd count)."""
+    step ``m`` with ``m % check_every == 0`` and are split into the fewest
+    launches of at most ``depth`` steps, as equal as possible. The launch count
+    is even (so the state ends in the live buffers) unless ``odd_ok`` (a third
+    state buffer) allows an odd count of 3 or more; to make it even, the
+    segment with the most launches is split into one launch more. ``None`` is
+    returned if that is impossible (every launch of depth 1 and an odd
+    count)."""
     end = start + n_steps
     checks = []
     if check_every:
@@ -498,26 +541,27 @@ def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optiona
             checks.append(m)
             m += check_every
     bounds = sorted(set(checks + [end]))
-    launches = []
+    segs = []
     prev = start
     for b in bounds:
-        seg = b - prev
-        if seg > 0:
-            n_l = -(-seg // depth)
-            base, rem = divmod(seg, n_l)
-            for k in range(n_l):
-                launches.append([base + (1 if k < rem else 0), -1])
-            if b in checks:
-                launches[-1][1] = checks.index(b)
+        if b - prev > 0:
+            segs.append([b - prev, -(-(b - prev) // depth), checks.index(b) if b in checks else -1])
         prev = b
-    if len(launches) % 2:
-        for k in range(len(launches) - 1, -1, -1):
-            d, slot = launches[k]
-            if d >= 2:
-                launches[k:k + 1] = [[d - d // 2, -1], [d // 2, slot]]
-                break
-        else:
+    total = sum(n for _, n, _ in segs)
+    if total % 2 and not (odd_ok and total >= 3):
+        # one more launch in the segment where it costs least: the one whose
+        # launches are deepest (its split gets closest to equal depths)
+        cand = [k for k, (seg, n, _) in enumerate(segs) if n < seg]
+        if not cand:
             return None, checks
+        k = max(cand, key=lambda i: (segs[i][0] / (segs[i][1] + 1), segs[i][0]))
+        segs[k][1] += 1
+    launches = []
+    for seg, n_l, slot in segs:
+        base, rem = divmod(seg, n_l)
+        part = [[base + (1 if k < rem else 0), -1] for k in range(n_l)]
+        part[-1][1] = slot
+        launches += part
     return [tuple(x) for x in launches], checks
 
 

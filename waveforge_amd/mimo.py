@@ -116,26 +116,29 @@ class BatchedRuns:
             rs.step = 0
             self.runs.append(rs)
         SourceTable.upload_many(tables, dev)
-        if tile_rows is None or depth is None:  # what is not given comes from FDTD2D's default
-            from .core.fdtd2d import default_launch_shape
+        from .core.fdtd2d import default_launch_shape, tile_shape
 
-            t0, d0 = default_launch_shape(self.runs)
-            tile_rows = t0 if tile_rows is None else tile_rows
-            depth = d0 if depth is None else depth
-            self.tile_rows, self.depth = tile_rows, depth
+        t0, d0, c0 = default_launch_shape(self.runs)  # what is not given comes from FDTD2D's default
+        cpl = c0
+        if tile_rows is None:
+            tile_rows = t0
+        elif tile_rows != "auto":
+            tile_rows, cpl = tile_shape(tile_rows)
+        depth = d0 if depth is None else depth
         self.tuned = None
         if "auto" in (tile_rows, depth):
             from . import autotune
 
             self.tuned = autotune.tune_runs(self.runs, self.device, fast_div=fast_div,
-                                            tile_rows=None if tile_rows == "auto" else [tile_rows],
+                                            tile_rows=None if tile_rows == "auto" else [(tile_rows, cpl)],
                                             depths=autotune.DEPTHS if depth == "auto" else [depth])
-            self.tile_rows, self.depth = tile_rows, depth = self.tuned.tile_rows, self.tuned.depth
+            tile_rows, depth, cpl = self.tuned.tile_rows, self.tuned.depth, self.tuned.cpl
+        self.tile_rows, self.depth, self.cpl = tile_rows, depth, cpl
         for k, rs in enumerate(self.runs):
             for d in range(1, depth + 1):
-                if not tb_limits_ok(rs, tile_rows, d):
+                if not tb_limits_ok(rs, tile_rows, d, cpl):
                     raise ValueError(f"run {k}: too many sources or receivers per tile for the batched kernel")
-        self.plan = TBPlan(self.runs, tile_rows, self.device, fast_div=fast_div)
+        self.plan = TBPlan(self.runs, tile_rows, self.device, fast_div=fast_div, cpl=cpl)
         self.steps_completed = 0
         self._dev_step = 0  # value of every descriptor's device-side `step` field
         self.stop = torch.zeros(len(self.runs), dtype=torch.int32, device=self.device)
