@@ -124,6 +124,21 @@ int wf_status_check(const uint32_t *status, int32_t n_words, hipStream_t stream)
 int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps,
                    hipStream_t stream);
 
+/* The same steps as wf_fdtd2d_step with no workspace from the caller and no
+ * allocation: no counterpart in the reference. It is for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops, hosts
+ * that forbid allocation). Each run's Mur snapshot strips (its pstrip
+ * workspace, which is ignored here) live in a file-scope device array of the
+ * library, 2^18 floats (1 MiB) per GPU, held until the process ends, so every
+ * run needs 4*(nx+ny) <= 2^18 (nx + ny <= 65536; larger grids are rejected
+ * with WF_ERR_ARG before any launch). The array's address is taken once per
+ * device (hipGetSymbolAddress) on the first call with that device current;
+ * call it with the stream's device current. All calls on one GPU share the
+ * array: calls that overlap in time on one GPU (different streams) race; calls
+ * on one stream do not. The strips need no clearing: every step writes them
+ * before it reads them. Results are bit-identical to wf_fdtd2d_step. */
+int wf_fdtd2d_step_noalloc(const wf_run2d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream);
+
 /* max|ex|, max|ey|, max|hz| over n cells each -> out3[0..2] (device).
  * `partial` is a device workspace of at least 3*n_partial floats; n_partial
  * is the number of reduction blocks (1..4096). Replaces the three

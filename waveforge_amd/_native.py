@@ -26,6 +26,7 @@ EXPORTED_SYMBOLS = (
     "wf_last_error",
     "wf_status_check",
     "wf_fdtd2d_step",
+    "wf_fdtd2d_step_noalloc",
     "wf_fdtd2d_maxabs",
     "wf_fdtd2d_tb_advance",
     "wf_fdtd2d_judge",
@@ -195,6 +196,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_status_check.restype = ctypes.c_int
         lib.wf_fdtd2d_step.argtypes = [ctypes.POINTER(WfRun2d), _i32, _i32, _vp]
         lib.wf_fdtd2d_step.restype = ctypes.c_int
+        lib.wf_fdtd2d_step_noalloc.argtypes = [ctypes.POINTER(WfRun2d), _i32, _i32, _vp]
+        lib.wf_fdtd2d_step_noalloc.restype = ctypes.c_int
         lib.wf_fdtd2d_maxabs.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp]
         lib.wf_fdtd2d_maxabs.restype = ctypes.c_int
         lib.wf_fdtd2d_tb_advance.argtypes = [ctypes.POINTER(WfTbArgs), _vp]

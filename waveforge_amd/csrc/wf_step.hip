@@ -21,6 +21,7 @@
 // CPU `tensor / python_float` is.
 #include "wf_common.hpp"
 
+#include <mutex>
 #include <type_traits>
 
 namespace {
@@ -244,18 +245,15 @@ int validate_run(const wf_run2d &r) {
     return WF_OK;
 }
 
-}  // namespace
-
-extern "C" int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream) {
-    if (!runs || n_runs < 1 || n_steps < 0) { wf::set_error("wf_fdtd2d_step: bad arguments"); return WF_ERR_ARG; }
-    for (int b = 0; b < n_runs; ++b) {
-        const int rc = validate_run(runs[b]);
-        if (rc) return rc;
-        if (!runs[b].pstrip) { wf::set_error("wf_fdtd2d_step: pstrip workspace is NULL"); return WF_ERR_ARG; }
-    }
+// The launches of n_steps steps of every run (validated by the caller). With
+// `strips` set, every run takes its Mur snapshot strips there instead of its
+// own pstrip: the runs' steps are stream-ordered (run b's k_pre, k_h, k_e
+// before run b+1's), so one strip buffer serves them all in turn.
+int launch_steps(const wf_run2d *runs, int n_runs, int n_steps, float *strips, hipStream_t stream, const char *what) {
     for (int s = 0; s < n_steps; ++s) {
         for (int b = 0; b < n_runs; ++b) {
-            const wf_run2d &r = runs[b];
+            wf_run2d r = runs[b];
+            if (strips) r.pstrip = strips;
             const int n = r.step + s;
             const int pre_threads = 1024;
             hipLaunchKernelGGL(k_pre, dim3(1), dim3(pre_threads), 0, stream, r, n);
@@ -267,10 +265,75 @@ extern "C" int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_st
             else
                 hipLaunchKernelGGL(k_e<false>, ge, dim3(kBX, kBY), 0, stream, r, n);
         }
-        const int rc = wf::check_launch("wf_fdtd2d_step");
+        const int rc = wf::check_launch(what);
         if (rc) return rc;
     }
     return WF_OK;
+}
+
+// Scratch of the allocation-free step: the Mur snapshot strips of the run being
+// stepped, 4*(nx+ny) floats, in a file-scope device array. Each GPU holds its
+// own copy until the process ends; in host code the array's name is not a
+// device address, so the address is taken once per device with
+// hipGetSymbolAddress and only that address is used.
+constexpr int64_t kNoAllocStripFloats = int64_t(1) << 18;  // nx + ny <= 65536
+__device__ __align__(256) float scratch[kNoAllocStripFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(float **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return wf::check_hip(e, "wf_fdtd2d_step_noalloc: hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) {
+        wf::set_error("wf_fdtd2d_step_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return WF_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return wf::check_hip(e, "wf_fdtd2d_step_noalloc: hipGetSymbolAddress");
+        g_scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return WF_OK;
+}
+
+}  // namespace
+
+extern "C" int wf_fdtd2d_step(const wf_run2d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream) {
+    if (!runs || n_runs < 1 || n_steps < 0) { wf::set_error("wf_fdtd2d_step: bad arguments"); return WF_ERR_ARG; }
+    for (int b = 0; b < n_runs; ++b) {
+        const int rc = validate_run(runs[b]);
+        if (rc) return rc;
+        if (!runs[b].pstrip) { wf::set_error("wf_fdtd2d_step: pstrip workspace is NULL"); return WF_ERR_ARG; }
+    }
+    return launch_steps(runs, n_runs, n_steps, nullptr, stream, "wf_fdtd2d_step");
+}
+
+extern "C" int wf_fdtd2d_step_noalloc(const wf_run2d *runs, int32_t n_runs, int32_t n_steps, hipStream_t stream) {
+    if (!runs || n_runs < 1 || n_steps < 0) {
+        wf::set_error("wf_fdtd2d_step_noalloc: bad arguments");
+        return WF_ERR_ARG;
+    }
+    for (int b = 0; b < n_runs; ++b) {
+        const int rc = validate_run(runs[b]);
+        if (rc) return rc;
+        if (4 * ((int64_t)runs[b].nx + runs[b].ny) > kNoAllocStripFloats) {
+            wf::set_error("wf_fdtd2d_step_noalloc: run %d (%dx%d) needs %lld strip floats, above the %lld of the "
+                          "library's scratch (nx + ny <= %lld)", b, runs[b].nx, runs[b].ny,
+                          (long long)(4 * ((int64_t)runs[b].nx + runs[b].ny)), (long long)kNoAllocStripFloats,
+                          (long long)(kNoAllocStripFloats / 4));
+            return WF_ERR_ARG;
+        }
+    }
+    float *strips = nullptr;
+    const int rc = scratch_for_current_device(&strips);
+    if (rc) return rc;
+    return launch_steps(runs, n_runs, n_steps, strips, stream, "wf_fdtd2d_step_noalloc");
 }
 
 extern "C" int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz, int64_t n, float *partial,
