@@ -92,6 +92,19 @@ def copy_probe() -> int:
     return 0
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (``lscpu``'s "Model name", from /proc/cpuinfo)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.lower().startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(wl, n_steps: int, slab=None, das=None) -> dict:
     """Time the oracle (torch CPU restatement of the reference step) on a sample
     of the headline run with every host core, and (``single_thread``) on a
@@ -108,6 +121,7 @@ def cpu_baseline(wl, n_steps: int, slab=None, das=None) -> dict:
     from oracle.fdtd2d_ref import OracleFDTD2D, OracleSource
 
     cores = os.cpu_count() or 1
+    model = cpu_model()
 
     def oracle_for(w):
         g = w.grid
@@ -134,6 +148,7 @@ def cpu_baseline(wl, n_steps: int, slab=None, das=None) -> dict:
         "value": value,
         "unit": "Mcell-updates/s",
         "cores": cores,
+        "cpu_model": model,
         "kind": "port",
         "sample": f"{g.Nx}x{g.Ny} blob medium, steps 3-{2 + n_steps} ({n_steps} timed steps) of the same run, "
                   f"oracle torch-CPU restatement, {cores} threads",
@@ -594,11 +609,26 @@ def main() -> int:
     # the timing events of the timed region are created and recorded once
     # before it (the first record of a fresh HIP event costs far more than a
     # reuse; hundreds of them stalled the first timed launch by tens of ms)
-    n_events = 2 * (args.steps // EVENT_STRIDE + 64)
+    # every launch of a short timed region is bracketed (a mean over a few
+    # launches is not one sample); a long one brackets every 4th
+    from waveforge_amd import autotune
+    from waveforge_amd.engine import schedule_launches
+
+    sim._tb_setup()
+    _, depth0, _ = sim.launch_shape
+    planned, _ = schedule_launches(args.warmup, args.steps, depth0, sim._n_check, odd_ok=True)
+    stride = 1 if planned is not None and len(planned) < 16 else EVENT_STRIDE
+    n_events = 2 * (args.steps // stride + 64)
     pool = [torch.cuda.Event(enable_timing=True) for _ in range(n_events)]
     for ev in pool:
         ev.record()
-    TBPlan.event_log, TBPlan.event_stride = [], EVENT_STRIDE
+    # the launch-shape tuner's table for this grid (reported below), timed on
+    # scratch fields before the warm-up: some 60 ms of GPU work that also brings
+    # the clock from idle to its working level, so a short timed region does not
+    # measure the clock ramp
+    print("[bench] autotuner table for this grid", file=sys.stderr, flush=True)
+    tuned = autotune.tune_runs(sim._build_runs(), fast_div=sim._fast_div)
+    TBPlan.event_log, TBPlan.event_stride = [], stride
     sim.run(args.warmup)
     torch.cuda.synchronize()
     TBPlan.event_pool = pool
@@ -632,23 +662,39 @@ def main() -> int:
         "traffic": None,
         "kernel": "k_tb (temporally blocked stepper)",
         "launches_timed": len(log),
-        "launch_sample": f"every {EVENT_STRIDE}th launch of the timed region",
+        "launch_sample": ("every launch of the timed region" if stride == 1
+                          else f"every {stride}th launch of the timed region"),
         "mean_launch_us": round(mean_ms * 1e3, 2),
         "first_launch_offset_ms": None if first_launch_ms is None else round(first_launch_ms, 3),
         "bytes_per_launch_algorithmic": mean_bytes,
         "depths": depths,
     }
+    if len(log) <= 16:
+        roofline["launch_us"] = [[d, round(ms * 1e3, 1)] for (d, _, _, _), ms in zip(log, launch_ms)]
     traffic_file = ROOT / "profiles" / "k_tb_blob4096_traffic.json"
-    tile_rows, depth = sim.launch_shape
-    from waveforge_amd import autotune
-
-    tuned = autotune.tune_runs(sim._build_runs(), fast_div=sim._fast_div)
+    valu_file = ROOT / "profiles" / "k_tb_blob4096_valu.json"
+    tile_rows, depth, cpl = sim.launch_shape
+    shape = (args.size, tile_rows, cpl)
+    depths_timed = [d for (d, _, _, _) in log]
+    dominant = max(set(depths_timed), key=depths_timed.count) if depths_timed else depth
+    roofline["dominant_depth"] = dominant
     if traffic_file.exists():
         tinfo = json.loads(traffic_file.read_text())
-        if (tinfo.get("grid"), tinfo.get("tile_rows"), tinfo.get("depth")) == (args.size, tile_rows, depth):
-            roofline["traffic"] = tinfo["hbm_bytes_per_launch"]
-            roofline["traffic_source"] = "profiles/" + tinfo["summary"] + " (rocprofv3 FETCH_SIZE+WRITE_SIZE, "
-            roofline["traffic_source"] += "copy-kernel calibrated, per depth-%d launch)" % depth
+        by_depth = {int(k): v for k, v in tinfo.get("hbm_bytes_by_depth", {}).items()}
+        if (tinfo.get("grid"), tinfo.get("tile_rows"), tinfo.get("cpl", 1)) == shape and dominant in by_depth:
+            roofline["traffic"] = by_depth[dominant]
+            roofline["traffic_source"] = (f"profiles/{tinfo['summary']} (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                          f"copy-kernel calibrated, per depth-{dominant} launch)")
+            timed = [(by_depth[d], ms) for (d, _, _, _), ms in zip(log, launch_ms) if d in by_depth]
+            # measured HBM bytes of the timed launches over their live launch time
+            gbs = sum(b for b, _ in timed) / (sum(ms for _, ms in timed) / 1e3) / 1e9
+            roofline["traffic_gb_per_s"] = round(gbs, 1)
+            roofline["traffic_frac"] = round(gbs / HBM_PEAK_GBS, 4)
+    if valu_file.exists():
+        vinfo = json.loads(valu_file.read_text())
+        by_depth = vinfo.get("by_depth", {})
+        if (vinfo.get("grid"), vinfo.get("tile_rows"), vinfo.get("cpl", 1)) == shape and str(dominant) in by_depth:
+            roofline["valu"] = dict(by_depth[str(dominant)], depth=dominant, source="profiles/" + vinfo["summary"])
     tr = wl.detectors.traces
     finite = bool(torch.isfinite(tr).all()) and bool(torch.isfinite(sim.fields.Hz).all())
     base = das_res = slab_res = None
@@ -693,9 +739,10 @@ def main() -> int:
             "steps": args.steps,
             "tile_rows": tile_rows,
             "depth": depth,
+            "cols_per_lane": cpl,
         },
         "roofline": roofline,
-        "autotune": {"choice": [tuned.tile_rows, tuned.depth], "used": [tile_rows, depth],
+        "autotune": {"choice": [tuned.tile_rows, tuned.depth, tuned.cpl], "used": [tile_rows, depth, cpl],
                      "rates_mcell_per_s": tuned.rates},
         "cpu_baseline": base,
         "fields_finite": finite,

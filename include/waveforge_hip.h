@@ -196,6 +196,16 @@ typedef struct wf_tb_args {
                                   the last one (materials, IEEE division, tile_i 32 or 48,
                                   no maxabs_out or stop); the adjoint's recompute of a
                                   segment's state history */
+    const int32_t *tile_lists; /* NULL, or device, 4 int32 per tile in the order of `tiles`:
+                                  [grp_lo, grp_hi) into grp_list and [det_lo, det_hi) into
+                                  det_list. grp_list[grp_lo..grp_hi) must be exactly the run's
+                                  source groups whose cell lies in the tile's region (the core
+                                  plus `depth` cells on every side), det_list[det_lo..det_hi)
+                                  exactly its detectors in the core. With them a tile reads its
+                                  own lists instead of testing every group and detector of its
+                                  run; NULL keeps that scan. */
+    const int32_t *grp_list;   /* group indices into the run's grp_start */
+    const int32_t *det_list;   /* detector indices into the run's det_cell */
 } wf_tb_args;
 
 int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream);
@@ -248,6 +258,12 @@ typedef struct wf_f64_args {
  * covering all runs (the per-step semantics of wf_fdtd2d_step). Replaces
  * FDTD2D.step() on a float64 grid (src/core/fdtd2d.py:187-266). */
 int wf_fdtd2d_step_f64(const wf_f64_args *args, hipStream_t stream);
+
+/* wf_fdtd2d_maxabs for float64 fields: max|ex|, max|ey|, max|hz| -> out3
+ * (device, 3 doubles); `partial` holds at least 3*n_partial doubles. The
+ * stability check of FDTD2D on a float64 grid (src/core/fdtd2d.py:280-285). */
+int wf_fdtd2d_maxabs_f64(const double *ex, const double *ey, const double *hz, int64_t n, double *partial,
+                         int32_t n_partial, double *out3, hipStream_t stream);
 
 /* One 3D simulation (FDTD3D). Fields are (nx, ny, nz) fp32, row-major with
  * k fastest: cell = (i*ny + j)*nz + k, the layout of a contiguous (Nx, Ny, Nz)

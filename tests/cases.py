@@ -92,7 +92,8 @@ def _source_cells(spec, nx, ny):
             i, j = torch.arange(start, stop), torch.full((n,), pos)
         else:
             i, j = torch.full((n,), pos), torch.arange(start, stop)
-        a = torch.ones(n) if amp is None else torch.tensor(amp, dtype=torch.float32)
+        # the profile as Python floats (doubles): each side rounds it to its grid dtype
+        a = torch.ones(n, dtype=torch.float64) if amp is None else torch.tensor(amp, dtype=torch.float64)
         return comp, i, j, a, wave, "cells"
     if kind == "plane":
         _, comp, plane, pos, wave = spec
@@ -122,7 +123,7 @@ def build_oracle(c, dtype=torch.float32):
     return o
 
 
-def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_div=True):
+def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_div=True, dtype=torch.float32):
     from waveforge_amd import core
     from waveforge_amd.core.sources import Waveform
 
@@ -136,7 +137,7 @@ def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_di
             raise NotImplementedError
 
         def build(self, N_steps, dt, device, dtype=torch.float32):
-            return waveform_table(self.kind, N_steps, dt).to(device)
+            return waveform_table(self.kind, N_steps, dt).to(device=device, dtype=dtype)
 
         peak_time = property(lambda self: 0.0)
         bandwidth = property(lambda self: 0.0)
@@ -145,7 +146,7 @@ def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_di
     gen = torch.Generator().manual_seed(c["seed"])
     ca, cb = _materials(c, gen)
     init = _init_fields(c, gen)
-    grid = core.YeeGrid(c["nx"], c["ny"], DX, DX, courant=c["courant"], device=device)
+    grid = core.YeeGrid(c["nx"], c["ny"], DX, DX, courant=c["courant"], device=device, dtype=dtype)
     fields = core.FieldSet(grid, batch_size=c["batch"])
     if init is not None:
         for k, v in init.items():

@@ -249,3 +249,53 @@ def test_grids_beyond_int32_cells_are_rejected():
     for nx, ny in ((46341, 46341), (4, 2 ** 29)):
         with _pytest.raises(ValueError, match="2\\^31-cell index range"):
             check_grid_cells(nx, ny)
+
+
+def test_tile_lists_match_brute_force():
+    """TBPlan.tile_lists: per tile, exactly the source groups whose cell lies
+    in the tile's region (core plus depth cells around it) and the detectors
+    in its core, for runs of odd sizes, both tile widths and several depths."""
+    import numpy as np
+    import torch
+
+    from waveforge_amd import _native
+    from types import SimpleNamespace
+
+    from waveforge_amd.engine import RunState, StepCoefficients, TBPlan
+
+    rng = np.random.default_rng(4)
+    cpu = torch.device("cpu")
+    coef = StepCoefficients.from_grid(2e-12, 1e-3, 1e-3)
+    runs = []
+    for nx, ny, n_src, n_det in ((97, 150, 7, 12), (40, 33, 0, 3), (130, 71, 20, 0), (64, 64, 3, 64)):
+        f = torch.zeros((3, nx, ny))
+        cells = sorted({int(rng.integers(nx)) * ny + int(rng.integers(ny)) for _ in range(n_src)})
+        # the group cells a SourceTable would hold (one Hz and one Ey group per cell)
+        gc = np.array(cells + cells, dtype=np.int32)
+        table = SimpleNamespace(n_grp=gc.size, n_inj=gc.size, group_cells=gc, inert=[], ptrs=lambda: (0, 0))
+        rs = RunState(f[0], f[1], f[2], None, None, nx, ny, coef, sources=table)
+        if n_det:
+            det = rng.choice(nx * ny, size=n_det, replace=False).astype(np.int32)
+            rs.det_cell = torch.from_numpy(det)
+            rs.traces = torch.zeros((n_det, 4))
+        runs.append(rs)
+    for tile_i, cpl in ((48, 1), (16, 1), (32, 2)):
+        plan = TBPlan(runs, tile_i, cpu, fast_div=False, cpl=cpl)
+        for depth in (1, 5, 8):
+            lists, grp, det = (x.numpy() for x in plan.tile_lists(depth))
+            tiles, n = plan.tiles(depth)
+            tiles = tiles.numpy()
+            TI, TJ = tile_i, _native.tile_cols(depth, cpl)
+            for t in range(n):
+                k, tile = tiles[t]
+                r = runs[k]
+                ntj = -(-r.ny // TJ)
+                I0, J0 = (tile // ntj) * TI, (tile % ntj) * TJ
+                gc = r.sources.group_cells.astype(np.int64)
+                want_g = sorted(g for g in range(gc.size)
+                                if I0 - depth <= gc[g] // r.ny < I0 + TI + depth
+                                and J0 - depth <= gc[g] % r.ny < J0 + TJ + depth)
+                assert sorted(grp[lists[t, 0]:lists[t, 1]].tolist()) == want_g, (tile_i, cpl, depth, t)
+                dc = [] if r.det_cell is None else r.det_cell.numpy().tolist()
+                want_d = sorted(d for d, c in enumerate(dc) if I0 <= c // r.ny < I0 + TI and J0 <= c % r.ny < J0 + TJ)
+                assert sorted(det[lists[t, 2]:lists[t, 3]].tolist()) == want_d, (tile_i, cpl, depth, t)

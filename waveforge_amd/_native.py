@@ -28,6 +28,7 @@ EXPORTED_SYMBOLS = (
     "wf_fdtd2d_step",
     "wf_fdtd2d_step_noalloc",
     "wf_fdtd2d_maxabs",
+    "wf_fdtd2d_maxabs_f64",
     "wf_fdtd2d_tb_advance",
     "wf_fdtd2d_judge",
     "wf_runs_add_step",
@@ -99,6 +100,7 @@ class WfTbArgs(ctypes.Structure):
         ("fast_div", _i32), ("pad", _i32),
         ("stop", _vp),
         ("hist_stride", ctypes.c_int64),
+        ("tile_lists", _vp), ("grp_list", _vp), ("det_list", _vp),
     ]
 
 
@@ -200,6 +202,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_fdtd2d_step_noalloc.restype = ctypes.c_int
         lib.wf_fdtd2d_maxabs.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp]
         lib.wf_fdtd2d_maxabs.restype = ctypes.c_int
+        lib.wf_fdtd2d_maxabs_f64.argtypes = [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp]
+        lib.wf_fdtd2d_maxabs_f64.restype = ctypes.c_int
         lib.wf_fdtd2d_tb_advance.argtypes = [ctypes.POINTER(WfTbArgs), _vp]
         lib.wf_fdtd2d_tb_advance.restype = ctypes.c_int
         lib.wf_tb_tile_shapes.argtypes = [ctypes.POINTER(_i32), _i32]

@@ -47,7 +47,9 @@ class DetectorSet:
         self.start_step = int(start_step)
         self.batch_size = int(batch_size)
         shape = (len(cells), self.n_steps) if batch_size == 1 else (batch_size, len(cells), self.n_steps)
-        self.traces = torch.zeros(shape, dtype=torch.float32, device=grid.device)
+        # recorded in the grid's dtype, as the reference's callers' buffers hold
+        # its fields (float64 on a float64 grid)
+        self.traces = torch.zeros(shape, dtype=getattr(grid, "dtype", torch.float32), device=grid.device)
         c3 = [c if len(c) == 3 else (c[0], c[1], 0) for c in cells]
         flat = [(i * grid.Ny + j) * Nz + k for i, j, k in c3]
         self.cell_index = torch.tensor(flat, dtype=torch.int32, device=grid.device)

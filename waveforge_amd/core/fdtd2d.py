@@ -19,21 +19,26 @@ step, replacing the callers' ``buf[:, step] = Hz[rx_i, rx_j, 0]`` loop.
 
 from __future__ import annotations
 
+import ctypes
 import time
 from typing import Optional
 
+import numpy as np
 import torch
 
 from .. import _native
+from .._native import WfF64Args, WfRun2dF64
 from ..engine import (
     RunState,
+    iter_launches,
     judge,
-    launch_parities,
-    schedule_launches,
+    launch_parity,
+    schedule_segments,
     SourceTable,
     StepCoefficients,
     TBPlan,
     maxabs,
+    maxabs64,
     per_step_advance,
     require_gpu_tensor,
     split_launches,
@@ -165,18 +170,69 @@ class FDTD2D:
         self._shape_key = None
         self._status: Optional[torch.Tensor] = None   # device word the kernels raise WF_STATUS_* bits in
         self._slots: Optional[torch.Tensor] = None    # check slots (+ a copy of the status word), reused
+        self._slots_dirty = 0                          # leading slot words the last call left nonzero
+        self._shadows: dict = {}     # contiguous stand-ins for strided field tensors
+        self._shadowed: list = []    # (FieldSet tensor, shadow) pairs of the current call
+        self._live = None            # (Ex, Ey, Hz) the kernels step
+        self._checked_key = None
+        self._f64_key = None
         self._stop: Optional[torch.Tensor] = None     # per-run stop flags, all zero between calls
 
     # ------------------------------------------------------------------
     # device state
     # ------------------------------------------------------------------
     def _device_check(self) -> None:
+        """Fields on the GPU, float32 (the fp32 kernels) or all float64 (the
+        fp64 kernels, as the reference runs a float64 grid); Ca/Cb float32, or
+        float32/float64 with float64 fields. Checked when the tensors change."""
+        f = self._fields
+        key = (id(f.Ex), id(f.Ey), id(f.Hz), id(self._Ca), id(self._Cb))
+        if key == self._checked_key:
+            return
+        f64 = f.Hz.dtype == torch.float64
+        kinds = (torch.float64,) if f64 else (torch.float32,)
         for name in ("Ex", "Ey", "Hz"):
-            require_gpu_tensor(getattr(self._fields, name), f"fields.{name}")
+            require_gpu_tensor(getattr(f, name), f"fields.{name}", kinds)
         if self._Ca is not None:
-            require_gpu_tensor(self._Ca, "Ca")
-            require_gpu_tensor(self._Cb, "Cb")
+            ckinds = (torch.float32, torch.float64) if f64 else (torch.float32,)
+            require_gpu_tensor(self._Ca, "Ca", ckinds)
+            require_gpu_tensor(self._Cb, "Cb", ckinds)
         _native.load()
+        self._checked_key = key
+
+    def _is_f64(self) -> bool:
+        return self._fields.Hz.dtype == torch.float64
+
+    def _live_fields(self) -> tuple:
+        """(Ex, Ey, Hz) the kernels step: the FieldSet tensors, or, for one not
+        laid out as the kernels read it (a strided view, as
+        ``FieldSet.from_tensor`` can give, ``src/core/fields.py:427-430``), a
+        contiguous stand-in that every call copies in before it steps and
+        back after, so the FieldSet tensors stay the live state."""
+        f = self._fields
+        out, shadowed = [], []
+        for name in ("Ex", "Ey", "Hz"):
+            t = getattr(f, name)
+            if t.is_contiguous() or (t.dim() == 4 and all(t[b].is_contiguous() for b in range(t.shape[0]))):
+                out.append(t)
+                continue
+            sh = self._shadows.get(name)
+            if sh is None or sh.shape != t.shape or sh.dtype != t.dtype or sh.device != t.device:
+                sh = torch.empty(t.shape, dtype=t.dtype, device=t.device)
+                self._shadows[name] = sh
+            shadowed.append((t, sh))
+            out.append(sh)
+        self._shadowed = shadowed
+        self._live = tuple(out)
+        return self._live
+
+    def _copy_in(self) -> None:
+        for t, sh in self._shadowed:
+            sh.copy_(t)
+
+    def _copy_out(self) -> None:
+        for t, sh in self._shadowed:
+            t.copy_(sh)
 
     def _stream(self) -> int:
         return _native.stream_handle(self._fields.Hz.device)
@@ -203,14 +259,7 @@ class FDTD2D:
         return [t.reshape(Nx, Ny)]
 
     def _build_runs(self) -> list[RunState]:
-        f = self._fields
-        tensors = (f.Ex, f.Ey, f.Hz)
-        for t in tensors:
-            if t.dim() == 4:
-                if not all(t[b].is_contiguous() for b in range(t.shape[0])):
-                    raise ValueError("waveforge_amd: each batch member of a field must be contiguous")
-            elif not t.is_contiguous():
-                raise ValueError("waveforge_amd: field tensors must be contiguous")
+        tensors = self._live_fields()
         srcs = self._source_table()
         det = self._detectors
         key = (
@@ -263,12 +312,16 @@ class FDTD2D:
                 src.step(d, n)
 
     def _wave_limit(self) -> Optional[int]:
-        t = self._source_table()
+        t = self._f64_table()[2] if self._is_f64() else self._source_table()
         return t.min_wave_len
 
     def _advance(self, n_steps: int) -> None:
         """Advance ``n_steps`` steps starting at ``steps_completed`` (no checks)."""
         if n_steps <= 0:
+            return
+        if self._is_f64():
+            self._f64_advance(n_steps)
+            self._apply_inert(self.steps_completed, n_steps)
             return
         setup = self._tb_setup()
         runs = self._build_runs()
@@ -276,15 +329,15 @@ class FDTD2D:
         if n_steps >= 2 and setup is not None:
             _, tile, depth = setup
             first = self.steps_completed
-            if split_launches(n_steps, depth) is None:
-                # odd step count at depth 1: one in-place step, then an even split
+            if split_launches(n_steps, depth, odd_ok=True) is None:
+                # no split into two or more launches: one in-place step first
                 for r in runs:
                     r.step = first
                 per_step_advance(runs, 1, stream)
                 first += 1
                 n_steps -= 1
             if n_steps > 0:
-                plan = self._tb_plan_for(runs, tile, third=len(split_launches(n_steps, depth, odd_ok=True)) % 2 == 1)
+                plan = self._tb_plan_for(runs, tile, depth)
                 plan.advance(n_steps, depth, stream, base_step=first)
         else:
             for r in runs:
@@ -292,19 +345,19 @@ class FDTD2D:
             per_step_advance(runs, n_steps, stream)
             self._apply_inert(self.steps_completed, n_steps)
 
-    def _tb_plan_for(self, runs, tile: tuple[int, int], third: bool = False) -> TBPlan:
-        """The blocked-kernel plan for these runs: a second state buffer always,
-        a third one (allocated on first need, then kept) for odd launch counts."""
+    def _tb_plan_for(self, runs, tile: tuple[int, int], depth: int) -> TBPlan:
+        """The blocked-kernel plan for these runs, with its tile tables and
+        launch arguments built for every depth up to ``depth``. It uses a
+        second and a third state buffer of the fields' shape, allocated
+        together on first use and kept, so launch counts of either parity end
+        in the live tensors."""
         shape = self._fields.Hz.shape
         dev = self._fields.Hz.device
         if self._second is None or self._second[0].shape != shape:
             self._second = tuple(torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3))
-            self._third = None
-            self._tb_key = None
-        if third and (self._third is None or self._third[0].shape != shape):
             self._third = tuple(torch.empty(shape, dtype=torch.float32, device=dev) for _ in range(3))
             self._tb_key = None
-        key = (self._cache_key, tile, self._fast_div, self._third is not None)
+        key = (self._cache_key, tile, self._fast_div, depth)
         if key != self._tb_key:
             ex2, ey2, hz2 = (self._members(t) for t in self._second)
             ex3 = ey3 = hz3 = None
@@ -315,6 +368,7 @@ class FDTD2D:
                 r.third = None if ex3 is None else (ex3[b], ey3[b], hz3[b])
                 r.step = 0  # launches pass the global step as step_offset
             self._tb_plan = TBPlan(runs, tile[0], dev, fast_div=self._fast_div, cpl=tile[1])
+            self._tb_plan.prepare(depth)
             self._tb_key = key
         return self._tb_plan
 
@@ -328,10 +382,22 @@ class FDTD2D:
                 "(build sources with a larger N_steps)"
             )
         self._device_check()
-        runs = self._build_runs()
-        for r in runs:
-            r.step = n
-        per_step_advance(runs, 1, self._stream())
+        if self._is_f64():
+            self._live_fields()
+            self._copy_in()
+            try:
+                self._f64_advance(1)
+            finally:
+                self._copy_out()
+        else:
+            runs = self._build_runs()
+            self._copy_in()
+            for r in runs:
+                r.step = n
+            try:
+                per_step_advance(runs, 1, self._stream())
+            finally:
+                self._copy_out()
         self._apply_inert(n, 1)
         self.steps_completed += 1
         if self.steps_completed % self._n_check == 0:
@@ -339,8 +405,88 @@ class FDTD2D:
 
     def _check_stability(self) -> None:
         """Max-abs check of Ex, Ey, Hz (``src/core/fdtd2d.py:268-295``)."""
-        f = self._fields
-        self._judge_values(maxabs(f.Ex, f.Ey, f.Hz, self._stream()).cpu().tolist())
+        ex, ey, hz = self._live if self._live is not None else self._live_fields()
+        reduce = maxabs64 if ex.dtype == torch.float64 else maxabs
+        self._judge_values(reduce(ex, ey, hz, self._stream()).cpu().tolist())
+
+    # ------------------------------------------------------------------
+    # float64 grids: the fp64 kernels, per step
+    # ------------------------------------------------------------------
+    def _f64_table(self):
+        """Device table of fp64 run descriptors over the live float64 fields
+        (one per batch member), rebuilt when tensors or sources change. The
+        update and Mur coefficients are the reference's Python doubles
+        (``src/core/fdtd2d.py:158-159``, ``boundaries.py:177-178``); Ca/Cb are
+        widened to float64 exactly, as torch promotes them."""
+        from ..fp64 import SourceTable64
+        from ..engine import EPS0 as _E0, MU0 as _M0, check_grid_cells
+
+        ex, ey, hz = self._live_fields()
+        srcs = self._source_list()
+        det = self._detectors
+        key = (ex.data_ptr(), ey.data_ptr(), hz.data_ptr(),
+               None if self._Ca is None else (self._Ca.data_ptr(), self._Cb.data_ptr()),
+               id(self._sources), tuple(id(q) for q in srcs), id(det))
+        if key == self._f64_key:
+            return self._f64_state
+        Nx, Ny = self._grid.Nx, self._grid.Ny
+        check_grid_cells(Nx, Ny)
+        dev = hz.device
+        exs, eys, hzs = (self._members(t) for t in (ex, ey, hz))
+        B = len(hzs)
+        keep = []
+        cas = cbs = [None] * B
+        if self._Ca is not None:
+            ca_m = self._members(self._Ca.to(torch.float64).contiguous())
+            cb_m = self._members(self._Cb.to(torch.float64).contiguous())
+            keep += ca_m + cb_m
+            cas = ca_m if len(ca_m) == B else ca_m * B
+            cbs = cb_m if len(cb_m) == B else cb_m * B
+        table = SourceTable64(srcs, Nx, Ny, dev)
+        if self._status is None or self._status.device != dev:
+            self._status = torch.zeros(1, dtype=torch.int32, device=dev)
+        structs = []
+        g = self._grid
+        for b in range(B):
+            r = WfRun2dF64()
+            r.ex, r.ey, r.hz = exs[b].data_ptr(), eys[b].data_ptr(), hzs[b].data_ptr()
+            if cas[b] is not None:
+                r.ca, r.cb = cas[b].data_ptr(), cbs[b].data_ptr()
+            strip = torch.empty(4 * (Nx + Ny), dtype=torch.float64, device=dev)
+            keep.append(strip)
+            r.pstrip = strip.data_ptr()
+            if table.n_grp:
+                r.inj, r.grp_start, r.n_grp, r.n_inj = (table.inj.data_ptr(), table.grp_start.data_ptr(),
+                                                         table.n_grp, table.n_inj)
+            if det is not None and len(det):
+                tr = det.traces if det.traces.dim() == 2 else det.traces[b]
+                if tr.dtype != torch.float64:
+                    raise ValueError("detectors of a float64 simulation record float64 traces "
+                                     "(build the DetectorSet with the float64 grid)")
+                r.det_cell, r.traces = det.cell_index.data_ptr(), tr.data_ptr()
+                r.trace_stride, r.n_det, r.trace_n0, r.trace_len = tr.stride(0), len(det), det.start_step, tr.shape[-1]
+            r.nx, r.ny, r.step = Nx, Ny, 0
+            r.dh, r.de, r.dx, r.dy = g.dt / _M0, g.dt / _E0, g.dx, g.dy
+            r.cmx, r.cmy = self._boundary.C_mur_x, self._boundary.C_mur_y
+            r.status = self._status.data_ptr()
+            structs.append(r)
+        arr = (WfRun2dF64 * B)(*structs)
+        dev_table = torch.from_numpy(np.frombuffer(bytes(arr), dtype=np.uint8).copy()).to(dev)
+        self._f64_state = (dev_table, B, table, keep)
+        self._f64_key = key
+        self._src_table = table   # its `inert` and `min_wave_len` serve the shared host logic
+        self._src_key = (id(self._sources), tuple(id(q) for q in srcs))
+        return self._f64_state
+
+    def _f64_advance(self, n_steps: int) -> None:
+        """``n_steps`` fp64 steps in place from ``steps_completed``."""
+        dev_table, B, _, _ = self._f64_table()
+        a = WfF64Args()
+        a.runs, a.n_runs = dev_table.data_ptr(), B
+        a.max_nx, a.max_ny = self._grid.Nx, self._grid.Ny
+        a.n_steps, a.step_offset = int(n_steps), self.steps_completed
+        lib = _native.load()
+        _native.check(lib.wf_fdtd2d_step_f64(ctypes.byref(a), ctypes.c_void_p(self._stream())), "wf_fdtd2d_step_f64")
 
     def _tb_setup(self):
         """(runs, tile, depth) if the temporally blocked kernel can run this
@@ -394,33 +540,38 @@ class FDTD2D:
         start = self.steps_completed
         limit = self._wave_limit()
         n_exec = n_steps if limit is None else max(0, min(n_steps, limit - start))
-        launches, checks = schedule_launches(start, n_exec, depth, self._n_check, odd_ok=True)
-        if n_exec and launches is None:
+        segs, checks = schedule_segments(start, n_exec, depth, self._n_check, odd_ok=True)
+        if n_exec and segs is None:
             return self._run_sync(n_steps)
         if n_exec:
             stream = self._stream()
             dev = self._fields.Hz.device
-            plan = self._tb_plan_for(runs, tile, third=len(launches) % 2 == 1)
+            plan = self._tb_plan_for(runs, tile, depth)
             n_runs = len(runs)
             if self._stop is None or self._stop.numel() != n_runs or self._stop.device != dev:
                 self._stop = torch.zeros(n_runs, dtype=torch.int32, device=dev)
             need = max(1, len(checks)) * n_runs * 3 + 1
             if self._slots is None or self._slots.numel() < need or self._slots.device != dev:
                 self._slots = torch.zeros(need, dtype=torch.int32, device=dev)
-            slots = self._slots[:len(checks) * n_runs * 3].view(len(checks), n_runs, 3) if checks else None
+                self._slots_dirty = 0
+            if checks and self._slots_dirty:
+                self._slots[:self._slots_dirty].zero_()  # what the previous call folded into
+                self._slots_dirty = 0
+            n_launch = sum(n for _, n, _ in segs)
             offset = start
-            for (d, slot), par in zip(launches, launch_parities(len(launches))):
-                plan.launch(d, par, offset, stream, maxabs_out=slots[slot] if slot >= 0 else None, stop=self._stop)
+            for k, (d, slot) in enumerate(iter_launches(segs)):
+                view = self._slots[slot * n_runs * 3:(slot + 1) * n_runs * 3] if slot >= 0 else None
+                plan.launch(d, launch_parity(k, n_launch), offset, stream, maxabs_out=view, stop=self._stop)
                 if slot >= 0:
-                    judge(slots[slot], self._threshold, self._stop, True, stream)
+                    judge(view, self._threshold, self._stop, True, stream)
                 offset += d
             if checks:
                 used = len(checks) * n_runs * 3
                 self._slots[used:used + 1].copy_(self._status)
-                host = self._slots[:used + 1].cpu()
-                self._slots[:used + 1].zero_()   # the next call's slots start at zero
+                host = self._slots[:used + 1].cpu().numpy()
+                self._slots_dirty = used + 1
                 status = int(host[used])
-                vals = host[:used].view(torch.float32).view(len(checks), n_runs, 3).amax(dim=1).tolist()
+                vals = host[:used].view(np.float32).reshape(len(checks), n_runs, 3).max(axis=1).tolist()
                 if status:
                     self._status.zero_()
                     raise RuntimeError(f"waveforge_amd: the stepping kernels reported status 0x{status:x} "
@@ -459,13 +610,18 @@ class FDTD2D:
             torch.cuda.synchronize()
         t_start = time.perf_counter()
         start = self.steps_completed
-        setup = self._tb_setup() if (n_steps >= 2 and not verbose) else None
+        self._device_check()
+        f64 = self._is_f64()
+        setup = self._tb_setup() if (n_steps >= 2 and not verbose and not f64) else None
+        self._live_fields()
+        self._copy_in()
         try:
             if setup is not None:
                 self._run_async(n_steps, setup)
             else:
                 self._run_sync(n_steps, verbose=verbose, t_start=t_start)
         finally:
+            self._copy_out()
             if is_cuda:
                 torch.cuda.synchronize()
             total = time.perf_counter() - t_start

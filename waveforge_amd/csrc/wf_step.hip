@@ -193,18 +193,27 @@ __global__ void __launch_bounds__(kBX * kBY) k_e64(const wf_run2d_f64 *runs, int
         e_body<false>(r, i, j);
 }
 
-__global__ void __launch_bounds__(256) k_maxabs_partial(const float *__restrict__ ex,
-                                                        const float *__restrict__ ey,
-                                                        const float *__restrict__ hz, int64_t n,
-                                                        unsigned int *__restrict__ partial) {
-    unsigned int m[3] = {0u, 0u, 0u};
+// |x| as an unsigned bit pattern: orders like |x|, and NaN sorts above inf,
+// so a NaN anywhere wins the maximum as torch's max() does.
+__device__ __forceinline__ unsigned int abs_key(float x) { return wf::abs_bits(x); }
+__device__ __forceinline__ unsigned long long abs_key(double x) {
+    return (unsigned long long)__double_as_longlong(x) & 0x7fffffffffffffffull;
+}
+__device__ __forceinline__ float from_key(unsigned int k) { return __uint_as_float(k); }
+__device__ __forceinline__ double from_key(unsigned long long k) { return __longlong_as_double((long long)k); }
+
+template <class T, class K>
+__global__ void __launch_bounds__(256) k_maxabs_partial(const T *__restrict__ ex, const T *__restrict__ ey,
+                                                        const T *__restrict__ hz, int64_t n,
+                                                        K *__restrict__ partial) {
+    K m[3] = {0, 0, 0};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        m[0] = max(m[0], wf::abs_bits(ex[k]));
-        m[1] = max(m[1], wf::abs_bits(ey[k]));
-        m[2] = max(m[2], wf::abs_bits(hz[k]));
+        m[0] = max(m[0], abs_key(ex[k]));
+        m[1] = max(m[1], abs_key(ey[k]));
+        m[2] = max(m[2], abs_key(hz[k]));
     }
-    __shared__ unsigned int sm[3][256];
+    __shared__ K sm[3][256];
     for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = m[c];
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -215,10 +224,10 @@ __global__ void __launch_bounds__(256) k_maxabs_partial(const float *__restrict_
     if (threadIdx.x < 3) partial[3 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
 }
 
-__global__ void __launch_bounds__(256) k_maxabs_final(const unsigned int *__restrict__ partial, int np,
-                                                      float *__restrict__ out3) {
-    __shared__ unsigned int sm[3][256];
-    unsigned int m[3] = {0u, 0u, 0u};
+template <class T, class K>
+__global__ void __launch_bounds__(256) k_maxabs_final(const K *__restrict__ partial, int np, T *__restrict__ out3) {
+    __shared__ K sm[3][256];
+    K m[3] = {0, 0, 0};
     for (int b = threadIdx.x; b < np; b += blockDim.x)
         for (int c = 0; c < 3; ++c) m[c] = max(m[c], partial[3 * b + c]);
     for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = m[c];
@@ -228,7 +237,7 @@ __global__ void __launch_bounds__(256) k_maxabs_final(const unsigned int *__rest
             for (int c = 0; c < 3; ++c) sm[c][threadIdx.x] = max(sm[c][threadIdx.x], sm[c][threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x < 3) out3[threadIdx.x] = __uint_as_float(sm[threadIdx.x][0]);
+    if (threadIdx.x < 3) out3[threadIdx.x] = from_key(sm[threadIdx.x][0]);
 }
 
 int validate_run(const wf_run2d &r) {
@@ -343,9 +352,24 @@ extern "C" int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *h
         return WF_ERR_ARG;
     }
     unsigned int *p = reinterpret_cast<unsigned int *>(partial);
-    hipLaunchKernelGGL(k_maxabs_partial, dim3(n_partial), dim3(256), 0, stream, ex, ey, hz, n, p);
-    hipLaunchKernelGGL(k_maxabs_final, dim3(1), dim3(256), 0, stream, p, n_partial, out3);
+    hipLaunchKernelGGL((k_maxabs_partial<float, unsigned int>), dim3(n_partial), dim3(256), 0, stream, ex, ey, hz,
+                       n, p);
+    hipLaunchKernelGGL((k_maxabs_final<float, unsigned int>), dim3(1), dim3(256), 0, stream, p, n_partial, out3);
     return wf::check_launch("wf_fdtd2d_maxabs");
+}
+
+extern "C" int wf_fdtd2d_maxabs_f64(const double *ex, const double *ey, const double *hz, int64_t n,
+                                    double *partial, int32_t n_partial, double *out3, hipStream_t stream) {
+    if (!ex || !ey || !hz || !partial || !out3 || n <= 0 || n_partial < 1 || n_partial > 4096) {
+        wf::set_error("wf_fdtd2d_maxabs_f64: bad arguments");
+        return WF_ERR_ARG;
+    }
+    unsigned long long *p = reinterpret_cast<unsigned long long *>(partial);
+    hipLaunchKernelGGL((k_maxabs_partial<double, unsigned long long>), dim3(n_partial), dim3(256), 0, stream, ex,
+                       ey, hz, n, p);
+    hipLaunchKernelGGL((k_maxabs_final<double, unsigned long long>), dim3(1), dim3(256), 0, stream, p, n_partial,
+                       out3);
+    return wf::check_launch("wf_fdtd2d_maxabs_f64");
 }
 
 extern "C" int wf_fdtd2d_step_f64(const wf_f64_args *args, hipStream_t stream) {

@@ -72,6 +72,10 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
                       "tile_i 32 or 48 with one column per lane and no maxabs_out or stop flags", (long long)a.hist_stride);
         return WF_ERR_UNSUPPORTED;
     }
+    if (a.tile_lists && (!a.grp_list || !a.det_list)) {
+        wf::set_error("wf_fdtd2d_tb_advance: tile_lists needs grp_list and det_list");
+        return WF_ERR_ARG;
+    }
     if (a.fast_div && (a.div_lo_x != 0u || a.div_lo_y != 0u || a.div_span_x > 0x7f800000u ||
                        a.div_span_y > 0x7f800000u)) {
         wf::set_error("wf_fdtd2d_tb_advance: fast_div needs div_lo = 0 and div_span <= bits(inf)");

@@ -1,7 +1,6 @@
 // Temporally blocked, multi-run 2D TM FDTD advance (the throughput path).
 //
-// Work decomposition. One workgroup (4 wavefronts for 16- and 32-row tiles of
-// one column per lane, 8 otherwise) owns one core tile of one run: TI rows x
+// Work decomposition. One workgroup of 8 wavefronts owns one core tile of one run: TI rows x
 // TJ = W - 2D columns, plus a halo of D cells per side. Every row of the
 // staged region is W = 64*CPL cells, one wavefront with CPL adjacent columns
 // per lane (lane l holds columns CPL*l .. CPL*l + CPL-1). Of NW wavefronts,
@@ -178,15 +177,16 @@ __device__ __forceinline__ void lds_put(float *p, const float (&v)[CPL]) {
 }
 
 // Wavefronts per workgroup, rows per lane and the occupancy the register
-// budget is sized for. One column per lane: tiles of 48 or more rows use 8
-// wavefronts (the halo rows are a smaller share of a taller region), shorter
-// tiles 4. Two columns per lane: always 8. Each lane keeps 5*CPL values per
-// row (Ex, Ey, Hz, Ca, Cb) plus 40-48 registers of temporaries; without the
-// cap the scheduler hoists every row's loads and runs at one or two waves per
-// SIMD. An 8-wavefront workgroup puts two waves on each SIMD, so its
-// occupancy is 4 or 6 waves per SIMD: 6 (80 registers) when the rows fit.
+// budget is sized for. Every shape runs 8 wavefronts: the more waves share a
+// region, the fewer rows each walks per step, which is the latency of a
+// launch on grids too small to fill the GPU (a 16-row tile holds 4 rows per
+// lane). Each lane keeps 5*CPL values per row (Ex, Ey, Hz, Ca, Cb) plus 40-48
+// registers of temporaries; without the cap the scheduler hoists every row's
+// loads and runs at one or two waves per SIMD. An 8-wavefront workgroup puts
+// two waves on each SIMD, so its occupancy is an even number of waves per
+// SIMD: 8 (64 registers), 6 (80) or 4 (128).
 template <int TI, int CPL>
-constexpr int kWavesOf = (CPL == 2 || TI >= 32) ? 8 : 4;
+constexpr int kWavesOf = 8;
 template <int TI, int CPL>
 constexpr int kThreadsOf = kWavesOf<TI, CPL> * kW;
 
@@ -314,7 +314,15 @@ k_tb(wf_tb_args a) {
         }
     }
     __syncthreads();
-    for (int g = tid; g < R.n_grp; g += kNT) {
+    // this tile's source groups and detectors: from the host's per-tile lists,
+    // or by testing every group and detector of the run
+    int g_lo = 0, g_hi = R.n_grp, d_lo = 0, d_hi = R.n_det;
+    if (a.tile_lists) {
+        const int4 tl = reinterpret_cast<const int4 *>(a.tile_lists)[t];
+        g_lo = tl.x, g_hi = tl.y, d_lo = tl.z, d_hi = tl.w;
+    }
+    for (int q = g_lo + tid; q < g_hi; q += kNT) {
+        const int g = a.tile_lists ? a.grp_list[q] : q;
         const wf_inj2d e0 = R.inj[R.grp_start[g]];
         const int li = e0.cell / ny - gi0, lj = e0.cell % ny - gj0;
         if (li >= 0 && li < RI && lj >= 0 && lj < W) {
@@ -329,7 +337,8 @@ k_tb(wf_tb_args a) {
             }
         }
     }
-    for (int d = tid; d < R.n_det; d += kNT) {
+    for (int q = d_lo + tid; q < d_hi; q += kNT) {
+        const int d = a.tile_lists ? a.det_list[q] : q;
         const int cell = R.det_cell[d];
         const int gi = cell / ny, gjd = cell % ny;
         if (gi >= I0 && gi < I0 + TI && gjd >= J0 && gjd < J0 + TJ) {

@@ -51,14 +51,15 @@ def f32(x: float) -> float:
     return float(np.float32(x))
 
 
-def require_gpu_tensor(t: torch.Tensor, what: str) -> None:
+def require_gpu_tensor(t: torch.Tensor, what: str, dtypes=(torch.float32,)) -> None:
     if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
         raise RuntimeError(
             f"waveforge_amd: {what} must live on the ROCm GPU (device 'cuda'); got "
             f"{getattr(t, 'device', type(t))}. The HIP kernels are the only compute path."
         )
-    if t.dtype != torch.float32:
-        raise RuntimeError(f"waveforge_amd: {what} must be float32, got {t.dtype}")
+    if t.dtype not in dtypes:
+        names = " or ".join(str(d).replace("torch.", "") for d in dtypes)
+        raise RuntimeError(f"waveforge_amd: {what} must be {names}, got {t.dtype}")
 
 
 @dataclass
@@ -281,6 +282,21 @@ def maxabs(ex: torch.Tensor, ey: torch.Tensor, hz: torch.Tensor, stream: int) ->
     return out
 
 
+def maxabs64(ex: torch.Tensor, ey: torch.Tensor, hz: torch.Tensor, stream: int) -> torch.Tensor:
+    """Device tensor ``[max|ex|, max|ey|, max|hz|]`` of float64 fields (``wf_fdtd2d_maxabs_f64``)."""
+    lib = _native.load()
+    n = ex.numel()
+    n_partial = int(max(1, min(1024, (n + 255) // 256)))
+    partial = torch.empty(3 * n_partial, dtype=torch.float64, device=ex.device)
+    out = torch.empty(3, dtype=torch.float64, device=ex.device)
+    _native.check(
+        lib.wf_fdtd2d_maxabs_f64(ex.data_ptr(), ey.data_ptr(), hz.data_ptr(), n, partial.data_ptr(), n_partial,
+                                 out.data_ptr(), ctypes.c_void_p(stream)),
+        "wf_fdtd2d_maxabs_f64",
+    )
+    return out
+
+
 def tile_count(nx: int, ny: int, tile_i: int, depth: int, cpl: int = 1) -> int:
     tj = _native.tile_cols(depth, cpl)
     return ((nx + tile_i - 1) // tile_i) * ((ny + tj - 1) // tj)
@@ -382,6 +398,13 @@ def split_launches(n_steps: int, depth: int, odd_ok: bool = False) -> Optional[l
     return [base + (1 if k < rem else 0) for k in range(n_launch)]
 
 
+def launch_parity(k: int, n_launch: int) -> int:
+    """State-buffer code of launch ``k`` of ``n_launch`` (``launch_parities``)."""
+    if n_launch % 2 == 0:
+        return k % 2
+    return (2, 3, 1)[k] if k < 3 else (k - 3) % 2
+
+
 def launch_parities(n_launch: int) -> list[int]:
     """State-buffer codes of ``n_launch`` consecutive launches that start and
     end in the live buffers A: 0 (A->B), 1 (B->A) alternating for an even
@@ -428,6 +451,7 @@ class TBPlan:
         raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
         self.run_table = torch.from_numpy(raw).to(device)
         self._tiles: dict[int, tuple[torch.Tensor, int]] = {}
+        self._lists: dict = {}
         self.div = None
         if fast_div:
             dxs = {r.coef.dx for r in runs}
@@ -439,6 +463,14 @@ class TBPlan:
                     self.div = (vx, vy)
         self.n_cells = sum(r.nx * r.ny for r in runs)
 
+    def prepare(self, max_depth: int) -> None:
+        """Upload the tile tables and build the launch arguments of every depth
+        up to ``max_depth`` and every state-buffer code now, so no later
+        launch waits on a table upload."""
+        for d in range(1, max_depth + 1):
+            for par in ((0, 1, 2, 3) if self.odd_ok else (0, 1)):
+                self._base_args(d, par)
+
     def tiles(self, depth: int) -> tuple[torch.Tensor, int]:
         if depth not in self._tiles:
             parts = []
@@ -448,6 +480,61 @@ class TBPlan:
             t = np.ascontiguousarray(np.concatenate(parts, axis=0))
             self._tiles[depth] = (torch.from_numpy(t).to(self.device), int(t.shape[0]))
         return self._tiles[depth]
+
+    def tile_lists(self, depth: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """Per-tile source-group and detector lists of a launch of ``depth``
+        steps (``wf_tb_args.tile_lists``, ``grp_list``, ``det_list``): the
+        groups whose cell lies in a tile's region (its core plus ``depth``
+        cells around it) and the detectors in its core, so a tile reads its
+        own few entries instead of testing every group and detector."""
+        if depth not in self._lists:
+            TI, TJ = self.tile_i, _native.tile_cols(depth, self.cpl)
+            rows, grp_all, det_all = [], [], []
+            g_off = d_off = 0
+            for r in self.runs:
+                nti, ntj = -(-r.nx // TI), -(-r.ny // TJ)
+                nt = nti * ntj
+                # source groups: every tile whose region holds the group's cell
+                if r.sources is not None and r.sources.n_grp:
+                    cells = r.sources.group_cells.astype(np.int64)
+                    gi, gj = cells // r.ny, cells % r.ny
+                    lo_i = np.maximum(0, -(-(gi - TI - depth + 1) // TI))
+                    hi_i = np.minimum(nti - 1, (gi + depth) // TI)
+                    lo_j = np.maximum(0, -(-(gj - TJ - depth + 1) // TJ))
+                    hi_j = np.minimum(ntj - 1, (gj + depth) // TJ)
+                    ni, nj = hi_i - lo_i + 1, hi_j - lo_j + 1
+                    cnt = ni * nj
+                    g_idx = np.repeat(np.arange(cells.size), cnt)
+                    within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+                    tile = (np.repeat(lo_i, cnt) + within // np.repeat(nj, cnt)) * ntj + \
+                        np.repeat(lo_j, cnt) + within % np.repeat(nj, cnt)
+                else:
+                    g_idx = tile = np.zeros(0, dtype=np.int64)
+                order = np.argsort(tile, kind="stable")
+                g_cnt = np.bincount(tile, minlength=nt)
+                # detectors: the tile whose core holds the cell
+                if r.det_cell is not None and r.det_cell.numel():
+                    dc = r.extra.get("det_cells_np")
+                    if dc is None:
+                        dc = r.det_cell.cpu().numpy().astype(np.int64)
+                        r.extra["det_cells_np"] = dc
+                    dtile = (dc // r.ny // TI) * ntj + (dc % r.ny) // TJ
+                else:
+                    dtile = np.zeros(0, dtype=np.int64)
+                d_order = np.argsort(dtile, kind="stable")
+                d_cnt = np.bincount(dtile, minlength=nt)
+                g_end = g_off + np.cumsum(g_cnt)
+                d_end = d_off + np.cumsum(d_cnt)
+                rows.append(np.stack([g_end - g_cnt, g_end, d_end - d_cnt, d_end], axis=1))
+                grp_all.append(g_idx[order])
+                det_all.append(d_order)
+                g_off += int(g_cnt.sum())
+                d_off += int(d_cnt.sum())
+            lists = np.ascontiguousarray(np.concatenate(rows).astype(np.int32))
+            grp = np.concatenate(grp_all + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+            det = np.concatenate(det_all + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+            self._lists[depth] = tuple(torch.from_numpy(x).to(self.device) for x in (lists, grp, det))
+        return self._lists[depth]
 
     def _base_args(self, depth: int, parity: int) -> WfTbArgs:
         """The launch arguments that do not change from launch to launch."""
@@ -465,6 +552,8 @@ class TBPlan:
             a.depth = int(depth)
             a.parity = int(parity)
             a.has_materials = self.has_materials
+            lists, grp, det = self.tile_lists(depth)
+            a.tile_lists, a.grp_list, a.det_list = lists.data_ptr(), grp.data_ptr(), det.data_ptr()
             if self.div is not None:
                 (a.rdx, a.div_lo_x, a.div_span_x), (a.rdy, a.div_lo_y, a.div_span_y) = self.div
                 a.fast_div = 1
@@ -520,6 +609,46 @@ def judge(maxabs: torch.Tensor, threshold: float, stop: torch.Tensor, all_runs: 
                                       stop.data_ptr(), ctypes.c_void_p(stream)), "wf_fdtd2d_judge")
 
 
+def schedule_segments(start: int, n_steps: int, depth: int, check_every: Optional[int], odd_ok: bool = False):
+    """The launch plan of ``schedule_launches`` in compact form: returns
+    ``(segments, checks)`` with ``segments`` a list of ``[steps, launches,
+    slot]`` (or ``None`` if no valid plan exists), cheap to build for long
+    runs; ``iter_launches`` expands it."""
+    end = start + n_steps
+    checks = []
+    if check_every:
+        m = (start // check_every + 1) * check_every
+        while m <= end:
+            checks.append(m)
+            m += check_every
+    segs = []
+    prev = start
+    for k, b in enumerate(checks + ([end] if not checks or checks[-1] != end else [])):
+        if b - prev > 0:
+            segs.append([b - prev, -(-(b - prev) // depth), k if k < len(checks) else -1])
+        prev = b
+    total = sum(n for _, n, _ in segs)
+    if total % 2 and not (odd_ok and total >= 3):
+        # one more launch in the segment where it costs least: the one whose
+        # launches are deepest (its split gets closest to equal depths)
+        cand = [k for k, (seg, n, _) in enumerate(segs) if n < seg]
+        if not cand:
+            return None, checks
+        k = max(cand, key=lambda i: (segs[i][0] / (segs[i][1] + 1), segs[i][0]))
+        segs[k][1] += 1
+    return segs, checks
+
+
+def iter_launches(segs):
+    """``(depth, slot)`` of every launch of a segment plan, in order: each
+    segment's launches as equal as possible, the last one feeding the
+    segment's check slot (or -1)."""
+    for seg, n_l, slot in segs:
+        base, rem = divmod(seg, n_l)
+        for k in range(n_l):
+            yield base + (1 if k < rem else 0), (slot if k == n_l - 1 else -1)
+
+
 def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optional[int], odd_ok: bool = False):
     """Launch plan for ``n_steps`` steps from step ``start``.
 
@@ -533,36 +662,10 @@ def schedule_launches(start: int, n_steps: int, depth: int, check_every: Optiona
     segment with the most launches is split into one launch more. ``None`` is
     returned if that is impossible (every launch of depth 1 and an odd
     count)."""
-    end = start + n_steps
-    checks = []
-    if check_every:
-        m = (start // check_every + 1) * check_every
-        while m <= end:
-            checks.append(m)
-            m += check_every
-    bounds = sorted(set(checks + [end]))
-    segs = []
-    prev = start
-    for b in bounds:
-        if b - prev > 0:
-            segs.append([b - prev, -(-(b - prev) // depth), checks.index(b) if b in checks else -1])
-        prev = b
-    total = sum(n for _, n, _ in segs)
-    if total % 2 and not (odd_ok and total >= 3):
-        # one more launch in the segment where it costs least: the one whose
-        # launches are deepest (its split gets closest to equal depths)
-        cand = [k for k, (seg, n, _) in enumerate(segs) if n < seg]
-        if not cand:
-            return None, checks
-        k = max(cand, key=lambda i: (segs[i][0] / (segs[i][1] + 1), segs[i][0]))
-        segs[k][1] += 1
-    launches = []
-    for seg, n_l, slot in segs:
-        base, rem = divmod(seg, n_l)
-        part = [[base + (1 if k < rem else 0), -1] for k in range(n_l)]
-        part[-1][1] = slot
-        launches += part
-    return [tuple(x) for x in launches], checks
+    segs, checks = schedule_segments(start, n_steps, depth, check_every, odd_ok)
+    if segs is None:
+        return None, checks
+    return list(iter_launches(segs)), checks
 
 
 def runs_add_step(run_table: torch.Tensor, n_runs: int, delta: int, stream: int) -> None:

@@ -139,6 +139,70 @@ class Fp64Runs:
         return self.trace_bufs[k]
 
 
+class SourceTable64:
+    """fp64 device records (``wf_inj2d_f64``) of a list of reference-style
+    source objects, grouped by (component, cell) with collection order kept
+    inside a group, as ``engine.SourceTable`` does for fp32.
+
+    Each contribution adds ``wave[n] * amp`` in double, which is the
+    reference's arithmetic on a float64 grid (tables and amplitudes in the
+    grid dtype, ``src/core/sources.py:826-833, 971-981``; ``values = w_now *
+    amp`` then ``target[idx] += values``, :882-887). A source whose table or
+    amplitudes are in another dtype gets a per-cell table of its own products
+    ``w[n] * amp`` formed in that dtype and widened, with amp 1, so the
+    rounding is the reference's too. Plane sources add ``w[n]`` (amp 1,
+    :715-736). Components outside Ex/Ey/Hz are kept in ``inert``."""
+
+    def __init__(self, sources, Nx: int, Ny: int, device) -> None:
+        from .core.sources import PlaneSource
+
+        self.inert, self._keep = [], []
+        self.min_wave_len = None
+        recs, keys = [], []
+        for src in sources:
+            comp = src.component
+            wave = src._waveform_tensor
+            self.min_wave_len = wave.numel() if self.min_wave_len is None else min(self.min_wave_len, wave.numel())
+            if comp not in _native.WF_COMP:
+                self.inert.append(src)
+                continue
+            if isinstance(src, PlaneSource):
+                i, j, _, _ = src.cells3d(Nx, Ny, 1)
+                amp = None
+            else:
+                i, j = src._i_idx.cpu(), src._j_idx.cpu()
+                amp = src._amp.detach()
+            w = wave.detach().to(device)
+            if amp is None or (w.dtype == torch.float64 and amp.dtype == torch.float64):
+                w64 = w.to(torch.float64).contiguous()
+                self._keep.append(w64)
+                a64 = [1.0] * i.numel() if amp is None else amp.cpu().tolist()
+                tables = [(w64, a) for a in a64]
+            else:  # products in the source's own dtype, then widened
+                amp_d = amp.to(device)
+                tables = []
+                for c in range(i.numel()):
+                    t = (w * amp_d[c:c + 1]).to(torch.float64).contiguous()
+                    self._keep.append(t)
+                    tables.append((t, 1.0))
+            for c in range(i.numel()):
+                cell = int(i[c]) * Ny + int(j[c])
+                t, a = tables[c]
+                recs.append((t.data_ptr(), t.numel(), _native.WF_COMP[comp], cell, 0, float(a)))
+                keys.append(_native.WF_COMP[comp] * Nx * Ny + cell)
+        self.n_grp = self.n_inj = 0
+        self.inj = self.grp_start = None
+        if recs:
+            rec = np.array(recs, dtype=INJ64_DTYPE)
+            key = np.array(keys, dtype=np.int64)
+            perm = np.argsort(key, kind="stable")
+            rec, key = rec[perm], key[perm]
+            starts = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+            self.grp_start = torch.from_numpy(np.r_[starts, len(rec)].astype(np.int32)).to(device)
+            self.inj = torch.from_numpy(rec.view(np.uint8).copy()).to(device)
+            self.n_grp, self.n_inj = int(starts.size), int(len(rec))
+
+
 class _TwinSpec:
     def __init__(self, nx, ny, ca, cb, sources, receivers, n_record, trace_n0):
         self.nx, self.ny, self.ca, self.cb = nx, ny, ca, cb

@@ -196,8 +196,7 @@ class BatchedRuns:
         launches, _ = schedule_launches(0, chunk, self.depth, None)
         if launches is None:
             raise ValueError("graph chunk needs an even launch split")
-        for d, _ in launches:
-            self.plan.tiles(d)  # upload tile tables before capture
+        self.plan.prepare(max(d for d, _ in launches))  # tile tables and lists uploaded before capture
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
