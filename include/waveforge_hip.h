@@ -94,6 +94,10 @@ typedef struct wf_run2d {
     int32_t step;
     int32_t flags;                /* reserved, 0 */
     uint32_t *status;             /* NULL or device word: WF_STATUS_* bits (see above) */
+    int32_t step_end;             /* 0, or the step index the run ends at: wf_fdtd2d_tb_advance takes no
+                                     step n >= step_end (a launch that reaches it stops the run there, a
+                                     launch that starts past it copies the run through) */
+    int32_t pad;
 } wf_run2d;
 
 int wf_abi_version(void);

@@ -15,6 +15,7 @@
  *
  * Layout: row-major (i, j), index i*Ny + j, one grid (no batch).
  */
+/* built with -fopenmp: wf_brute_run_f32 runs each phase's rows in parallel */
 #include <stdlib.h>
 #include <string.h>
 
@@ -89,3 +90,80 @@ void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,        
 
 DEFINE_STEP(wf_brute_step_f32, float)
 DEFINE_STEP(wf_brute_step_f64, double)
+
+/* n_steps of the same step on one grid, for long runs at full size: the
+ * snapshot and Faraday buffers are allocated once and every phase runs its
+ * rows in parallel (OpenMP). Each phase writes every cell once from values
+ * of the phase before, so the result does not depend on the thread count
+ * and equals n_steps calls of wf_brute_step_f32. Sources: n_src
+ * contributions applied in list order, contribution s adding
+ * src_vals[s*n_steps + n] to its cell at step n (the caller forms
+ * w[n]*amp as the reference does). Detectors: traces[d*n_steps + n] = Hz at
+ * det_cell[d] after step n. */
+void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const float *Ca, const float *Cb,
+                      float Dh, float De, float dx, float dy, float cx, float cy, int n_steps,
+                      int n_src, const int *src_comp, const int *src_cell, const float *src_vals,
+                      int n_det, const int *det_cell, float *traces)
+{
+    size_t n = (size_t)Nx * (size_t)Ny;
+    float *P = (float *)malloc(n * sizeof(float));
+    float *Hn = (float *)malloc(n * sizeof(float));
+    for (int st = 0; st < n_steps; ++st) {
+        #pragma omp parallel for schedule(static)
+        for (long c = 0; c < (long)n; ++c) P[c] = Hz[c];                  /* 1. snapshot */
+        for (int s = 0; s < n_src; ++s) {                                    /* 2. sources, in order */
+            float *f = src_comp[s] == 0 ? Ex : (src_comp[s] == 1 ? Ey : Hz);
+            f[src_cell[s]] = f[src_cell[s]] + src_vals[(size_t)s * n_steps + st];
+        }
+        #pragma omp parallel for schedule(static)
+        for (long c = 0; c < (long)n; ++c) Hn[c] = Hz[c];
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Nx - 1; ++i)                                     /* 3. Faraday */
+            for (int j = 0; j < Ny - 1; ++j) {
+                float a = (Ex[IDX(i, j + 1)] - Ex[IDX(i, j)]) / dy;
+                float b = (Ey[IDX(i + 1, j)] - Ey[IDX(i, j)]) / dx;
+                float c = a - b;
+                float d = Dh * c;
+                Hn[IDX(i, j)] = Hz[IDX(i, j)] + d;
+            }
+        for (int j = 0; j < Ny; ++j)                                         /* 4. Mur: rows, then columns */
+            Hn[IDX(0, j)] = P[IDX(1, j)] + cx * (Hn[IDX(1, j)] - P[IDX(0, j)]);
+        for (int j = 0; j < Ny; ++j)
+            Hn[IDX(Nx - 1, j)] = P[IDX(Nx - 2, j)] + cx * (Hn[IDX(Nx - 2, j)] - P[IDX(Nx - 1, j)]);
+        for (int i = 0; i < Nx; ++i)
+            Hn[IDX(i, 0)] = P[IDX(i, 1)] + cy * (Hn[IDX(i, 1)] - P[IDX(i, 0)]);
+        for (int i = 0; i < Nx; ++i)
+            Hn[IDX(i, Ny - 1)] = P[IDX(i, Ny - 2)] + cy * (Hn[IDX(i, Ny - 2)] - P[IDX(i, Ny - 1)]);
+        #pragma omp parallel for schedule(static)
+        for (long c = 0; c < (long)n; ++c) Hz[c] = Hn[c];
+        #pragma omp parallel for schedule(static)
+        for (int i = 0; i < Nx; ++i)                                         /* 5. Ampere */
+            for (int j = 1; j < Ny; ++j) {
+                float g = (Hz[IDX(i, j)] - Hz[IDX(i, j - 1)]) / dy;
+                if (Ca) {
+                    float t1 = Ca[IDX(i, j)] * Ex[IDX(i, j)];
+                    float t2 = Cb[IDX(i, j)] * g;
+                    Ex[IDX(i, j)] = t1 + t2;
+                } else {
+                    float t2 = De * g;
+                    Ex[IDX(i, j)] = Ex[IDX(i, j)] + t2;
+                }
+            }
+        #pragma omp parallel for schedule(static)
+        for (int i = 1; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j) {
+                float g = (Hz[IDX(i, j)] - Hz[IDX(i - 1, j)]) / dx;
+                if (Ca) {
+                    float t1 = Ca[IDX(i, j)] * Ey[IDX(i, j)];
+                    float t2 = Cb[IDX(i, j)] * g;
+                    Ey[IDX(i, j)] = t1 - t2;
+                } else {
+                    float t2 = De * g;
+                    Ey[IDX(i, j)] = Ey[IDX(i, j)] - t2;
+                }
+            }
+        for (int d = 0; d < n_det; ++d) traces[(size_t)d * n_steps + st] = Hz[det_cell[d]];
+    }
+    free(P);
+    free(Hn);
+}

@@ -65,56 +65,64 @@ def test_slab_256_1000_steps(device):
 
 
 def test_blob_4096_4000_steps(device):
+    """The full 4000 steps bit for bit against the oracle's C restatement
+    (all rows in parallel on the host), fields and the traces of the 64
+    detectors at r = 1800 plus an inner ring of 32 at r = 200 that the pulse
+    crosses; then the error against the fp64 run."""
+    import ctypes
+    from pathlib import Path
+
     from waveforge_amd import workloads
     from waveforge_amd.fp64 import fp64_twin
 
-    _log("blob 4096^2: building the medium")
-    wl = workloads.blob_medium(device)
+    steps = 4000
+    _log("blob 4096^2: building the medium (64 detectors at r = 1800, 32 at r = 200)")
+    wl = workloads.blob_medium(device, inner_rings=((32, 200.0),))
     sim, det = wl.sim, wl.detectors
-    # first steps bit for bit against the oracle at full size
-    sim.run(4)
+    sim.run(steps)
     torch.cuda.synchronize()
-    o = oracle_from_sim(sim, torch.float32)
-    o.run(4)
-    for k in ("Ex", "Ey", "Hz"):
-        assert_bitwise(getattr(sim.fields, k), o.fields[k], f"blob step 4 {k}")
+    assert sim.n_stability_checks == 40 and sim.launch_shape == (48, 8, 1)
+    lib = ctypes.CDLL(str(Path(__file__).resolve().parent.parent / "oracle" / "build" / "libwf_brute.so"))
+    P = ctypes.c_void_p
+    lib.wf_brute_run_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 5 + [ctypes.c_float] * 6 + \
+        [ctypes.c_int, ctypes.c_int] + [P] * 3 + [ctypes.c_int] + [P] * 2
+    o = oracle_from_sim(sim, torch.float32)   # the scalars and tables the oracle uses
+    n = 4096
+    ex, ey, hz = (np.zeros((n, n), dtype=np.float32) for _ in range(3))
+    ca = o.Ca.reshape(n, n).numpy().astype(np.float32).copy()
+    cb = o.Cb.reshape(n, n).numpy().astype(np.float32).copy()
+    src = o.sources[0]
+    vals = np.ascontiguousarray((src.wave[:steps] * src.amp).numpy().astype(np.float32)[None, :])
+    cells = np.array([int(src.i_idx[0]) * n + int(src.j_idx[0])], dtype=np.int32)
+    dcells = np.array([i * n + j for i, j in det.cells], dtype=np.int32)
+    tr = np.zeros((len(dcells), steps), dtype=np.float32)
+    ptr = lambda a: a.ctypes.data_as(P)
+    _log("blob: oracle (C restatement) for 4000 steps on the host")
+    lib.wf_brute_run_f32(n, n, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb), o.Dh, o.De, o.dx, o.dy, o.cx, o.cy,
+                         steps, 1, ptr(np.array([2], np.int32)), ptr(cells), ptr(vals), len(dcells), ptr(dcells),
+                         ptr(tr))
     del o
-    _log("blob: oracle 4 steps equal; temporally blocked vs per-step over 24 steps")
-    sim.run(20)
-    torch.cuda.synchronize()
-    tb = [getattr(sim.fields, k).clone() for k in ("Ex", "Ey", "Hz")]
-    tb_tr = det.traces[:, :24].clone()
-    sim.reset()
-    det.traces.zero_()
-    for _ in range(24):
-        sim.step()
-    torch.cuda.synchronize()
-    for k, ref in zip(("Ex", "Ey", "Hz"), tb):
-        assert torch.equal(getattr(sim.fields, k), ref), k
-    assert torch.equal(det.traces[:, :24], tb_tr)
-    _log("blob: full 4000 steps, fp32 and fp64")
-    sim.reset()
-    det.traces.zero_()
-    sim.run(4000)
+    for name, ref in (("Ex", ex), ("Ey", ey), ("Hz", hz)):
+        assert_bitwise(getattr(sim.fields, name), torch.from_numpy(ref), f"blob step 4000 {name}")
+    assert_bitwise(det.traces, torch.from_numpy(tr), "blob traces, both rings")
+    inner = det.traces[64:].abs().max().item()
+    _log(f"blob: 4000 steps equal; max |Hz| on the inner ring {inner:.3e}")
+    assert inner > 1e-3 * float(np.abs(hz).max())   # the pulse crossed the inner ring
+    _log("blob: fp64 run of the same simulation")
     twin = fp64_twin(sim)
-    twin.advance(4000)
+    twin.advance(steps)
     torch.cuda.synchronize()
-    assert bool(torch.isfinite(sim.fields.Hz).all()) and bool(torch.isfinite(det.traces).all())
-    assert sim.n_stability_checks == 40
-    e_tr = rel_l2(det.traces, twin.traces(0))
-    e_hz = rel_l2(sim.fields.Hz.reshape(4096, 4096), twin.fields(0)[2])
-    e_ex = rel_l2(sim.fields.Ex.reshape(4096, 4096), twin.fields(0)[0])
-    m32, m64 = float(det.traces.abs().max()), float(twin.traces(0).abs().max())
+    e_in = rel_l2(det.traces[64:], twin.traces(0)[64:])
+    e_hz = rel_l2(sim.fields.Hz.reshape(n, n), twin.fields(0)[2])
+    e_ex = rel_l2(sim.fields.Ex.reshape(n, n), twin.fields(0)[0])
+    m32, m64 = float(det.traces[:64].abs().max()), float(twin.traces(0)[:64].abs().max())
     hmax = float(twin.fields(0)[2].abs().max())
-    _log(f"blob: fp32 vs fp64 rel L2 traces {e_tr:.3e}, Hz {e_hz:.3e}, Ex {e_ex:.3e}; "
-         f"max|trace| fp32 {m32:.3e} fp64 {m64:.3e}, max|Hz| {hmax:.3e}")
-    assert e_hz < 1e-3 and e_ex < 1e-3
-    if m64 > 1e-6 * hmax:
-        assert e_tr < 1e-4
-    else:
-        # the pulse has not reached the detector circle: the fp64 traces hold
-        # only the scheme's precursor tail, far below fp32 resolution of the field
-        assert m32 <= 1e-6 * hmax
+    _log(f"blob: fp32 vs fp64 rel L2 inner-ring traces {e_in:.3e}, Hz {e_hz:.3e}, Ex {e_ex:.3e}; "
+         f"r = 1800 ring max|trace| fp32 {m32:.3e} fp64 {m64:.3e}, max|Hz| {hmax:.3e}")
+    assert e_in < 1e-4 and e_hz < 1e-3 and e_ex < 1e-3
+    # the pulse has not reached r = 1800 in 4000 steps: the fp64 traces there hold
+    # only the scheme's precursor tail, far below fp32 resolution of the field
+    assert m64 <= 1e-6 * hmax and m32 <= 1e-6 * hmax
 
 
 def test_batched_imaging_one_phantom_per_size(device):
@@ -161,6 +169,52 @@ def test_batched_imaging_one_phantom_per_size(device):
                                                                                        img64.shape)
         _log(f"imaging {n}^2: fp32 vs fp64 rel L2 traces {e_tr:.3e}, image {e_img:.3e}, peak equal {peak_same}")
         assert e_tr < 1e-4 and e_img < 1e-4 and peak_same
+
+
+def test_batched_imaging_32_phantoms_512_runs(device):
+    """The batched-imaging workload at its real size: 32 seeded head phantoms
+    (8 each of 150^2, 200^2, 256^2, 384^2), 16 TX x 16 RX, 2000 steps, all
+    512 runs in one batched launch sequence (HIP-graph replay), then 32 DAS
+    images. One seeded TX run per grid size equals a standalone FDTD2D run
+    bit for bit (fields and traces), and every image equals the oracle's DAS
+    of the same traces bit for bit."""
+    from oracle.das_ref import das_backprojection as das_oracle
+    from waveforge_amd import core, mimo, workloads
+
+    dt = workloads.imaging_dt()
+    steps = 2000
+    phantoms = workloads.phantom_set(32)
+    _log("imaging: 512 runs batched with graph replay")
+    images, traces, br = mimo.image_phantoms(phantoms, steps, dt=dt, device=device, use_graph=True, chunk=240)
+    torch.cuda.synchronize()
+    assert len(br.runs) == 512 and not any(br.status()["diverged"])
+    rng = np.random.default_rng(2024)
+    wave = core.RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5e-9)
+    for p in range(4):  # the first phantom of each size
+        eps, sig, ants = phantoms[p]
+        n = eps.shape[0]
+        tx = int(rng.integers(len(ants)))
+        g = core.YeeGrid(n, n, 1e-3, 1e-3, courant=0.99, device=device)
+        Ca, Cb = core.coefficients_from_maps(eps, sig, dt, device=device)
+        f = core.FieldSet(g)
+        det = core.DetectorSet(ants, steps, grid=g)
+        src = core.PointSource(wave, ants[tx][0], ants[tx][1], "Hz", grid=g, N_steps=steps)
+        sim = core.FDTD2D(g, f, core.MurABC(g, f.Hz), core.SourceCollection([src]), Ca=Ca, Cb=Cb, detectors=det)
+        sim.run(steps)
+        torch.cuda.synchronize()
+        k = 16 * p + tx
+        ex, ey, hz = br.fields(k)
+        for name, a, b in (("Ex", ex, f.Ex), ("Ey", ey, f.Ey), ("Hz", hz, f.Hz)):
+            assert torch.equal(a.reshape(n, n).view(torch.int32), b.reshape(n, n).view(torch.int32)), \
+                f"phantom {p} ({n}^2) TX {tx}: {name} differs from the standalone run"
+        assert torch.equal(traces[p][tx], det.traces), f"phantom {p} TX {tx}: traces differ"
+        _log(f"imaging: phantom {p} ({n}^2) TX {tx} equals its standalone FDTD2D run")
+    v = 299_792_458.0 / math.sqrt(40.0)
+    for p, (eps, sig, ants) in enumerate(phantoms):
+        n = eps.shape[0]
+        ref = das_oracle(traces[p].cpu().numpy(), ants, n, n, 1e-3, dt, steps, v_medium=v)
+        assert np.array_equal(images[p].cpu().numpy(), ref), f"phantom {p}: DAS differs from the oracle"
+    _log("imaging: 32 DAS images equal the oracle's")
 
 
 def test_adjoint_head_256_3000_steps(device):

@@ -109,3 +109,49 @@ def test_brute_detects_planted_sign_error(brute):
     o.Dh = -o.Dh  # plant the mistake in the oracle (CPU only)
     o.run(5)
     assert not np.array_equal(o.Hz.reshape(c["nx"], c["ny"]).numpy(), hz)
+
+
+def brute_run_many(lib, o, steps):
+    """The same run through wf_brute_run_f32 (all steps in one call, rows in
+    parallel): point/line sources only, as the full-size workloads use."""
+    lib.wf_brute_run_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5 + \
+        [ctypes.c_float] * 6 + [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_int] + \
+        [ctypes.c_void_p] * 2
+    nx, ny = o.Nx, o.Ny
+    ex, ey, hz = (getattr(o, k).reshape(nx, ny).numpy().astype(np.float32).copy() for k in ("Ex", "Ey", "Hz"))
+    ca = cb = None
+    if o.Ca is not None:
+        ca = o.Ca.reshape(nx, ny).numpy().astype(np.float32).copy()
+        cb = o.Cb.reshape(nx, ny).numpy().astype(np.float32).copy()
+    comp_code = {"Ex": 0, "Ey": 1, "Hz": 2}
+    comps, cells, vals = [], [], []
+    for s in o.sources:
+        assert s.kind == "cells"
+        for c, (i, j) in enumerate(zip(s.i_idx.tolist(), s.j_idx.tolist())):
+            comps.append(comp_code[s.component])
+            cells.append(i * ny + j)
+            vals.append((s.wave[:steps] * s.amp[c]).numpy().astype(np.float32))
+    vals_a = np.ascontiguousarray(np.stack(vals)) if vals else np.zeros((0, steps), np.float32)
+    det = np.array([i * ny + j for i, j in o.detectors], dtype=np.int32)
+    tr = np.zeros((len(det), steps), dtype=np.float32)
+    ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+    lib.wf_brute_run_f32(nx, ny, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb), o.Dh, o.De, o.dx, o.dy, o.cx, o.cy,
+                         steps, len(comps), ptr(np.array(comps, np.int32)), ptr(np.array(cells, np.int32)),
+                         ptr(vals_a), len(det), ptr(det), ptr(tr))
+    return ex, ey, hz, tr
+
+
+def test_multi_step_entry_equals_step_by_step(brute):
+    c = case(61, 47, steps=80, material=True, init=True, seed=9,
+             sources=[("point", "Hz", 30, 20, "dgauss"), ("point", "Ey", 0, 11, "gauss"),
+                      ("line", "Ex", "x", 0, 3, 40, "sin", [0.2 + 0.01 * k for k in range(37)]),
+                      ("point", "Hz", 30, 20, "sin")],
+             detectors=[(0, 0), (30, 20), (60, 46), (5, 40)])
+    o = build_oracle(c)
+    ex, ey, hz, tr = brute_run_many(brute, o, c["steps"])
+    ex1, ey1, hz1, tr1 = brute_run(brute, build_oracle(c), c["steps"], torch.float32)
+    for a, b in ((ex, ex1), (ey, ey1), (hz, hz1), (tr, tr1)):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    o.run(c["steps"])
+    assert np.array_equal(hz.view(np.int32), o.Hz.reshape(61, 47).numpy().view(np.int32))
+    assert np.array_equal(tr.view(np.int32), o.trace_array().numpy().view(np.int32))

@@ -81,6 +81,7 @@ class WfRun2d(ctypes.Structure):
         ("cmx", _f32), ("cmy", _f32),
         ("step", _i32), ("flags", _i32),
         ("status", _vp),
+        ("step_end", _i32), ("pad", _i32),
     ]
 
 

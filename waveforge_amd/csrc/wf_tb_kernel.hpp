@@ -255,20 +255,42 @@ k_tb(wf_tb_args a) {
     float *dst_ey = par == 1 ? R.ey : (par == 2 ? R.ey3 : R.ey2);
     float *dst_hz = par == 1 ? R.hz : (par == 2 ? R.hz3 : R.hz2);
 
-    if (a.stop != nullptr && a.stop[run] != 0) {
-        // The run failed a stability check earlier in this stream: carry its
-        // state through unchanged so it ends in the live buffers as it was at
-        // the failing check.
+    // the step index, read once: the kernel's own stores (traces) could alias the
+    // descriptor as far as the compiler knows, so a read inside the loop would
+    // be a global load and wait on every step
+    const int n0 = __builtin_amdgcn_readfirstlane(R.step + a.step_offset);
+    const int n_end = __builtin_amdgcn_readfirstlane(R.step_end > 0 ? R.step_end : 0x7fffffff);
+    if ((a.stop != nullptr && a.stop[run] != 0) || n0 >= n_end) {
+        // The run failed a stability check earlier in this stream, or has
+        // reached its last step: carry its state through unchanged (so it ends
+        // in the live buffers as it was), folding its maxima if asked.
+        unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
         for (int k = w; k < TI; k += NW) {
             const int gi = I0 + k;
             for (int c = lane; c < TJ; c += kW) {
                 const int cj = J0 + c;
                 if (gi < nx && cj < ny) {
                     const size_t g = (size_t)gi * ny + cj;
-                    dst_ex[g] = src_ex[g];
-                    dst_ey[g] = src_ey[g];
-                    dst_hz[g] = src_hz[g];
+                    const float vx = src_ex[g], vy = src_ey[g], vh = src_hz[g];
+                    dst_ex[g] = vx;
+                    dst_ey[g] = vy;
+                    dst_hz[g] = vh;
+                    m0 = max(m0, wf::abs_bits(vx));
+                    m1 = max(m1, wf::abs_bits(vy));
+                    m2 = max(m2, wf::abs_bits(vh));
                 }
+            }
+        }
+        if (a.maxabs_out) {
+            for (int off = 32; off > 0; off >>= 1) {
+                m0 = max(m0, (unsigned int)__shfl_xor((int)m0, off));
+                m1 = max(m1, (unsigned int)__shfl_xor((int)m1, off));
+                m2 = max(m2, (unsigned int)__shfl_xor((int)m2, off));
+            }
+            if (lane == 0) {
+                atomicMax(&a.maxabs_out[3 * run + 0], m0);
+                atomicMax(&a.maxabs_out[3 * run + 1], m1);
+                atomicMax(&a.maxabs_out[3 * run + 2], m2);
             }
         }
         return;
@@ -364,10 +386,6 @@ k_tb(wf_tb_args a) {
     bool lane_core[CPL];
 #pragma unroll
     for (int h = 0; h < CPL; ++h) lane_core[h] = c0 + h >= D && c0 + h < D + TJ && col_in[h];
-    // the step index, read once: the kernel's own stores (traces) could alias the
-    // descriptor as far as the compiler knows, so a read inside the loop would
-    // be a global load and wait on every step
-    const int n0 = __builtin_amdgcn_readfirstlane(R.step + a.step_offset);
 
     if (edge) {  // the first step's Mur snapshot (LDS holds the state before any source)
             {
@@ -394,6 +412,7 @@ k_tb(wf_tb_args a) {
 #pragma unroll 1
     for (int s = 1; s <= D; ++s) {
         const int n = n0 + s - 1;
+        if (n >= n_end) break;  // the run's last step was taken: its state stays as it is
         // row conditions below are formed from this per-step copy rather than
         // hoisted out of the step loop as one held mask per row
         int gib = gi0;

@@ -223,6 +223,7 @@ class RunState:
     second: Optional[tuple] = None   # (ex2, ey2, hz2)
     third: Optional[tuple] = None    # (ex3, ey3, hz3): lets a blocked run use an odd launch count
     status: Optional[torch.Tensor] = None  # device uint32 word: WF_STATUS_* bits the kernels raise
+    step_end: int = 0                      # 0, or the step the blocked kernel stops the run at
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self) -> None:
@@ -255,6 +256,7 @@ class RunState:
         r.step = self.step
         if self.status is not None:
             r.status = self.status.data_ptr()
+        r.step_end = int(self.step_end)
         return r
 
 

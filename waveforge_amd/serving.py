@@ -64,6 +64,7 @@ class ImagingRequest:
     traces: Optional[torch.Tensor] = None
     failed: bool = False
     _event: object = None
+    _failed_dev: object = None
     _runs: list = field(default_factory=list)
     _state: object = None
     _bytes: int = 0
@@ -78,11 +79,11 @@ class ImagingRequest:
         return n * self.n_steps * len(self.antennas)
 
 
-def request_footprint(nx: int, ny: int, n_ant: int, n_steps: int, quantum: int) -> int:
+def request_footprint(nx: int, ny: int, n_ant: int, n_steps: int, quantum: int = 0) -> int:
     """Device bytes a request holds while active: two field states per TX run,
     shared Ca/Cb, the traces, the waveform and the image."""
     cells = nx * ny
-    return (6 * cells * n_ant + 2 * cells) * 4 + n_ant * n_ant * n_steps * 4 + (n_steps + quantum) * 4 + cells * 8
+    return (6 * cells * n_ant + 2 * cells) * 4 + n_ant * n_ant * n_steps * 4 + n_steps * 4 + cells * 8
 
 
 class ImagingServer:
@@ -132,8 +133,8 @@ class ImagingServer:
         nx, ny = req.eps.shape
         ca, cb = coefficients_from_maps(req.eps, req.sigma, self.dt, device=dev)
         ca, cb = ca.reshape(nx, ny), cb.reshape(nx, ny)
-        wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(
-            req.n_steps + self.quantum, self.dt, dev, torch.float32)
+        wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(req.n_steps, self.dt, dev,
+                                                                              torch.float32)
         if any(not (0 <= i < nx and 0 <= j < ny) for i, j in req.antennas):
             raise ValueError(f"request {req.rid}: antenna cell outside the {nx}x{ny} grid")
         cells = torch.tensor([i * ny + j for i, j in req.antennas], dtype=torch.int32, device=dev)
@@ -146,10 +147,10 @@ class ImagingServer:
             table = SourceTable([CellSource("Hz", wave, i, j, 1.0)], nx, ny, dev, upload=False)
             tables.append(table)
             rs = RunState(f[0], f[1], f[2], ca, cb, nx, ny, self.coef, sources=table, det_cell=cells,
-                          traces=req.traces[t], trace_n0=0, step=0)
+                          traces=req.traces[t], trace_n0=0, step=0, step_end=req.n_steps)
             rs.second = (f[3], f[4], f[5])
             rs.extra["req"] = req
-            rs.extra["stop"] = 0
+            rs.extra["slot"] = -1  # index in the current plan's stop flags (-1: not yet in a plan)
             for d in range(1, self.depth + 1):
                 if not tb_limits_ok(rs, self.tile_rows, d):
                     raise ValueError(f"request {req.rid}: receivers too dense for the batched kernel")
@@ -160,25 +161,33 @@ class ImagingServer:
         req._bytes = self.footprint(req)
         req.admitted = now
         self.used += req._bytes
-        self._sync_flags()
         self.active.append(req)
         self._plan = None
 
     # ------------------------------------------------------------------
-    def _sync_flags(self) -> None:
-        """Copy the device stop flags of the current table back to the runs."""
-        if self._stop is not None and self._plan_runs:
-            for r, fl in zip(self._plan_runs, self._stop.cpu().tolist()):
-                r.extra["stop"] = fl
-
     def _build_plan(self) -> None:
-        self._plan_runs = [r for req in self.active for r in req._runs]
-        if not self._plan_runs:
+        """Plan over the active runs; their stop flags move to the new plan on
+        the device (a gather from the old flags), so no admission or
+        retirement waits for the GPU."""
+        runs = [r for req in self.active for r in req._runs]
+        if not runs:
             self._plan = self._stop = None
+            self._plan_runs = []
             return
-        self._plan = TBPlan(self._plan_runs, self.tile_rows, self.device)
-        self._stop = torch.tensor([r.extra["stop"] for r in self._plan_runs], dtype=torch.int32,
-                                  device=self.device)
+        old = self._stop
+        keep = [r.extra["slot"] for r in runs]
+        if old is not None and any(k >= 0 for k in keep):
+            idx = torch.tensor([max(k, 0) for k in keep], dtype=torch.long, device=self.device)
+            fresh = torch.tensor([k < 0 for k in keep], dtype=torch.bool, device=self.device)
+            stop = torch.where(fresh, torch.zeros_like(idx, dtype=torch.int32), old.index_select(0, idx))
+        else:
+            stop = torch.zeros(len(runs), dtype=torch.int32, device=self.device)
+        for k, r in enumerate(runs):
+            r.extra["slot"] = k
+        self._plan_runs = runs
+        self._plan = TBPlan(runs, self.tile_rows, self.device)
+        self._plan.prepare(self.depth)
+        self._stop = stop.contiguous()
 
     def advance_round(self) -> None:
         """Advance every active run by one quantum (asynchronous) and judge it."""
@@ -207,15 +216,15 @@ class ImagingServer:
 
     def retire_finished(self) -> None:
         """Requests whose runs reached their step count leave the active set;
-        their DAS image is enqueued and completion is tracked by an event."""
+        their DAS image is enqueued and completion is tracked by an event.
+        Each run stopped at its own last step inside the round (its step_end),
+        so a request is judged on its state at that step, not on steps past it."""
         finished = [req for req in self.active if req._runs[0].step >= req.n_steps]
         if not finished:
             return
-        self._sync_flags()
         for req in finished:
             self._finish(req)
         self.active = [req for req in self.active if req not in finished]
-        self._plan = None
         self._build_plan()
 
     def _finish(self, req: ImagingRequest) -> None:
@@ -226,7 +235,8 @@ class ImagingServer:
         req.image = torch.empty((nx, ny), dtype=torch.float64, device=dev)
         run_das(req.traces, ax, ay, req.image, pitch_x=self.dx, pitch_y=self.dx, v_medium=self.v_medium,
                 dt=self.dt, clip=True)
-        req.failed = any(r.extra.get("stop", 0) for r in req._runs)
+        lo, hi = req._runs[0].extra["slot"], req._runs[-1].extra["slot"] + 1
+        req._failed_dev = self._stop[lo:hi].clone()  # read once the image is ready (no wait here)
         ev = torch.cuda.Event()
         ev.record()
         req._event = ev
@@ -238,6 +248,8 @@ class ImagingServer:
         for req in self.retiring:
             if req._event.query():
                 req.finished = now_fn()
+                req.failed = bool(req._failed_dev.any())
+                req._failed_dev = None
                 self.used -= req._bytes
                 for r in req._runs:
                     r.ex = r.ey = r.hz = r.second = None
@@ -303,7 +315,8 @@ def _stats(done, total: float, rounds: int = 0) -> dict:
 
 
 def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e-3, device=None,
-                realtime: bool = True, v_medium: Optional[float] = None, n_check: int = 100) -> dict:
+                realtime: bool = True, v_medium: Optional[float] = None, n_check: int = 100,
+                stability_threshold: float = 1e10) -> dict:
     """The plain one-run-per-launch GPU path as a server, for comparison.
 
     Requests are taken one at a time in arrival order; every TX of a request
@@ -339,7 +352,7 @@ def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e
             det = DetectorSet(req.antennas, req.n_steps, grid=grid)
             src = PointSource(wave, i, j, "Hz", grid=grid, N_steps=req.n_steps)
             sim = FDTD2D(grid, f, MurABC(grid, f.Hz), SourceCollection([src]), Ca=ca, Cb=cb, detectors=det,
-                         n_check=n_check)
+                         n_check=n_check, stability_threshold=stability_threshold)
             try:
                 sim.run(req.n_steps)
             except SimulationDivergedError:
@@ -359,8 +372,6 @@ def poisson_requests(n: int, rate_hz: float, *, n_min: int = 128, n_max: int = 1
     """The serving stream: Poisson arrivals at ``rate_hz``; grid size
     log-uniform in ``[n_min, n_max]``, E from ``elements``, steps =
     ``steps_per_diag`` x grid diagonal in cells; head phantoms with a clot."""
-    from .workloads import head_phantom
-
     rng = np.random.default_rng(seed)
     t = 0.0
     out = []
@@ -368,7 +379,17 @@ def poisson_requests(n: int, rate_hz: float, *, n_min: int = 128, n_max: int = 1
         t += rng.exponential(1.0 / rate_hz)
         n_cells = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
         e = int(rng.choice(elements))
-        ph = head_phantom(n_cells, e, seed=seed * 100003 + rid, clot=True, dx=dx)
-        steps = int(math.floor(steps_per_diag * n_cells * math.sqrt(2.0)))
-        out.append(ImagingRequest(rid, t, ph.eps, ph.sigma, ph.antennas, steps))
+        out.append(make_request(rid, t, n_cells, e, seed=seed * 100003 + rid, steps_per_diag=steps_per_diag, dx=dx))
     return out
+
+
+def make_request(rid: int, arrival: float, n_cells: int, elements: int, *, seed: int,
+                 steps_per_diag: float = 8.0, dx: float = 1e-3) -> ImagingRequest:
+    """One request of the serving stream: a head phantom with a clot on an
+    ``n_cells``^2 grid, an ``elements``-element array, steps =
+    ``steps_per_diag`` x grid diagonal in cells."""
+    from .workloads import head_phantom
+
+    ph = head_phantom(n_cells, elements, seed=seed, clot=True, dx=dx)
+    steps = int(math.floor(steps_per_diag * n_cells * math.sqrt(2.0)))
+    return ImagingRequest(rid, arrival, ph.eps, ph.sigma, ph.antennas, steps)

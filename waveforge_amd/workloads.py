@@ -99,7 +99,7 @@ def slab_256(device="cuda", n: int = 256, n_steps: int = 1000, **sim_kw) -> Simu
 
 
 def blob_medium(device="cuda", n: int = 4096, n_steps: int = 4000, seed: int = 0, n_det: int = 64,
-                det_radius: float | None = None, **sim_kw) -> Simulation:
+                det_radius: float | None = None, inner_rings=(), **sim_kw) -> Simulation:
     dx = 1e-3
     grid = YeeGrid(n, n, dx, dx, device=device)
     eps = blob_eps_map(n, seed=seed)
@@ -112,6 +112,9 @@ def blob_medium(device="cuda", n: int = 4096, n_steps: int = 4000, seed: int = 0
     r = det_radius if det_radius is not None else 1800.0 * n / 4096.0
     cells = [(int(round(c + r * math.cos(2 * math.pi * k / n_det))), int(round(c + r * math.sin(2 * math.pi * k / n_det))))
              for k in range(n_det)]
+    for m, rr in inner_rings:  # further (count, radius) rings after the main one
+        cells += [(int(round(c + rr * math.cos(2 * math.pi * k / m))), int(round(c + rr * math.sin(2 * math.pi * k / m))))
+                  for k in range(m)]
     det = DetectorSet(cells, n_steps, grid=grid)
     sim = FDTD2D(grid, fields, MurABC(grid, fields.Hz), SourceCollection([src]), Ca=Ca, Cb=Cb, detectors=det,
                  **sim_kw)
