@@ -46,6 +46,21 @@ TB_MAX_DETS_PER_TILE = 64
 TB_MAX_GROUPS = 4096
 
 
+def to_device_async(a, device, dtype=None) -> torch.Tensor:
+    """A host array (NumPy or a list) as a device tensor, without waiting for
+    the device: the values are staged in a newly allocated pinned buffer and
+    copied asynchronously on the current stream (torch's pinned allocator
+    keeps the buffer until the copy has run), so a host that builds tables
+    while the GPU works is not held back by a synchronous copy."""
+    t = torch.as_tensor(np.asarray(a)) if dtype is None else torch.as_tensor(np.asarray(a), dtype=dtype)
+    device = torch.device(device)
+    if device.type != "cuda":
+        return t.to(device)
+    pinned = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    pinned.copy_(t)
+    return pinned.to(device, non_blocking=True)
+
+
 def f32(x: float) -> float:
     """Round a Python float to fp32, as torch does for a scalar operand."""
     return float(np.float32(x))
@@ -157,8 +172,8 @@ class SourceTable:
             self.group_cells = rec["cell"][starts].copy()
             self.host_inj, self.host_grp = rec, grp
             if upload:
-                self.inj = torch.from_numpy(rec.view(np.uint8).copy()).to(device)
-                self.grp_start = torch.from_numpy(grp).to(device)
+                self.inj = to_device_async(rec.view(np.uint8), device)
+                self.grp_start = to_device_async(grp, device)
             else:
                 self.inj = self.grp_start = None
         else:
@@ -176,8 +191,8 @@ class SourceTable:
         live = [t for t in tables if t.n_grp]
         if not live:
             return
-        inj = torch.from_numpy(np.concatenate([t.host_inj.view(np.uint8) for t in live])).to(device)
-        grp = torch.from_numpy(np.concatenate([t.host_grp for t in live])).to(device)
+        inj = to_device_async(np.concatenate([t.host_inj.view(np.uint8) for t in live]), device)
+        grp = to_device_async(np.concatenate([t.host_grp for t in live]), device)
         o_i = o_g = 0
         for t in live:
             n_b, n_g = t.host_inj.nbytes, t.host_grp.size
@@ -451,7 +466,7 @@ class TBPlan:
         self.has_materials = int(mats.pop())
         arr = (WfRun2d * len(runs))(*[r.struct() for r in runs])
         raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
-        self.run_table = torch.from_numpy(raw).to(device)
+        self.run_table = to_device_async(raw, device)
         self._tiles: dict[int, tuple[torch.Tensor, int]] = {}
         self._lists: dict = {}
         self.div = None
@@ -465,23 +480,70 @@ class TBPlan:
                     self.div = (vx, vy)
         self.n_cells = sum(r.nx * r.ny for r in runs)
 
-    def prepare(self, max_depth: int) -> None:
-        """Upload the tile tables and build the launch arguments of every depth
-        up to ``max_depth`` and every state-buffer code now, so no later
-        launch waits on a table upload."""
-        for d in range(1, max_depth + 1):
+    def prepare(self, max_depth: int, depths: Optional[Sequence[int]] = None) -> None:
+        """Upload the tile tables and lists and build the launch arguments of
+        ``depths`` (every depth up to ``max_depth`` by default) and every
+        state-buffer code now, so no later launch waits on a table upload."""
+        for d in (depths if depths is not None else range(1, max_depth + 1)):
             for par in ((0, 1, 2, 3) if self.odd_ok else (0, 1)):
                 self._base_args(d, par)
 
     def tiles(self, depth: int) -> tuple[torch.Tensor, int]:
         if depth not in self._tiles:
-            parts = []
-            for k, r in enumerate(self.runs):
-                nt = tile_count(r.nx, r.ny, self.tile_i, depth, self.cpl)
-                parts.append(np.stack([np.full(nt, k, dtype=np.int32), np.arange(nt, dtype=np.int32)], axis=1))
-            t = np.ascontiguousarray(np.concatenate(parts, axis=0))
-            self._tiles[depth] = (torch.from_numpy(t).to(self.device), int(t.shape[0]))
+            nts = np.array([tile_count(r.nx, r.ny, self.tile_i, depth, self.cpl) for r in self.runs], dtype=np.int64)
+            total = int(nts.sum())
+            starts = np.cumsum(nts) - nts
+            t = np.empty((total, 2), dtype=np.int32)
+            t[:, 0] = np.repeat(np.arange(len(self.runs)), nts)
+            t[:, 1] = np.arange(total) - np.repeat(starts, nts)
+            self._tiles[depth] = (to_device_async(t, self.device), total)
         return self._tiles[depth]
+
+    def _run_lists(self, r: RunState, depth: int):
+        """One run's per-tile lists for this tile shape and depth: ``(rows, grp,
+        det)`` with ``rows`` (n_tiles, 4) offsets into the run's own ``grp`` and
+        ``det`` index arrays. Cached on the run, so plans rebuilt over changing
+        run sets (the serving loop) only concatenate."""
+        key = (self.tile_i, self.cpl, depth)
+        cache = r.extra.setdefault("tile_lists", {})
+        if key in cache:
+            return cache[key]
+        TI, TJ = self.tile_i, _native.tile_cols(depth, self.cpl)
+        nti, ntj = -(-r.nx // TI), -(-r.ny // TJ)
+        nt = nti * ntj
+        # source groups: every tile whose region holds the group's cell
+        if r.sources is not None and r.sources.n_grp:
+            cells = r.sources.group_cells.astype(np.int64)
+            gi, gj = cells // r.ny, cells % r.ny
+            lo_i = np.maximum(0, -(-(gi - TI - depth + 1) // TI))
+            hi_i = np.minimum(nti - 1, (gi + depth) // TI)
+            lo_j = np.maximum(0, -(-(gj - TJ - depth + 1) // TJ))
+            hi_j = np.minimum(ntj - 1, (gj + depth) // TJ)
+            nj = hi_j - lo_j + 1
+            cnt = (hi_i - lo_i + 1) * nj
+            g_idx = np.repeat(np.arange(cells.size), cnt)
+            within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+            tile = (np.repeat(lo_i, cnt) + within // np.repeat(nj, cnt)) * ntj + \
+                np.repeat(lo_j, cnt) + within % np.repeat(nj, cnt)
+        else:
+            g_idx = tile = np.zeros(0, dtype=np.int64)
+        g_cnt = np.bincount(tile, minlength=nt)
+        grp = g_idx[np.argsort(tile, kind="stable")]
+        # detectors: the tile whose core holds the cell
+        if r.det_cell is not None and r.det_cell.numel():
+            dc = r.extra.get("det_cells_np")
+            if dc is None:
+                dc = r.det_cell.cpu().numpy().astype(np.int64)
+                r.extra["det_cells_np"] = dc
+            dtile = (dc // r.ny // TI) * ntj + (dc % r.ny) // TJ
+        else:
+            dtile = np.zeros(0, dtype=np.int64)
+        d_cnt = np.bincount(dtile, minlength=nt)
+        det = np.argsort(dtile, kind="stable")
+        g_end, d_end = np.cumsum(g_cnt), np.cumsum(d_cnt)
+        rows = np.stack([g_end - g_cnt, g_end, d_end - d_cnt, d_end], axis=1)
+        cache[key] = (rows, grp, det)
+        return cache[key]
 
     def tile_lists(self, depth: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
         """Per-tile source-group and detector lists of a launch of ``depth``
@@ -490,52 +552,17 @@ class TBPlan:
         cells around it) and the detectors in its core, so a tile reads its
         own few entries instead of testing every group and detector."""
         if depth not in self._lists:
-            TI, TJ = self.tile_i, _native.tile_cols(depth, self.cpl)
-            rows, grp_all, det_all = [], [], []
-            g_off = d_off = 0
-            for r in self.runs:
-                nti, ntj = -(-r.nx // TI), -(-r.ny // TJ)
-                nt = nti * ntj
-                # source groups: every tile whose region holds the group's cell
-                if r.sources is not None and r.sources.n_grp:
-                    cells = r.sources.group_cells.astype(np.int64)
-                    gi, gj = cells // r.ny, cells % r.ny
-                    lo_i = np.maximum(0, -(-(gi - TI - depth + 1) // TI))
-                    hi_i = np.minimum(nti - 1, (gi + depth) // TI)
-                    lo_j = np.maximum(0, -(-(gj - TJ - depth + 1) // TJ))
-                    hi_j = np.minimum(ntj - 1, (gj + depth) // TJ)
-                    ni, nj = hi_i - lo_i + 1, hi_j - lo_j + 1
-                    cnt = ni * nj
-                    g_idx = np.repeat(np.arange(cells.size), cnt)
-                    within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
-                    tile = (np.repeat(lo_i, cnt) + within // np.repeat(nj, cnt)) * ntj + \
-                        np.repeat(lo_j, cnt) + within % np.repeat(nj, cnt)
-                else:
-                    g_idx = tile = np.zeros(0, dtype=np.int64)
-                order = np.argsort(tile, kind="stable")
-                g_cnt = np.bincount(tile, minlength=nt)
-                # detectors: the tile whose core holds the cell
-                if r.det_cell is not None and r.det_cell.numel():
-                    dc = r.extra.get("det_cells_np")
-                    if dc is None:
-                        dc = r.det_cell.cpu().numpy().astype(np.int64)
-                        r.extra["det_cells_np"] = dc
-                    dtile = (dc // r.ny // TI) * ntj + (dc % r.ny) // TJ
-                else:
-                    dtile = np.zeros(0, dtype=np.int64)
-                d_order = np.argsort(dtile, kind="stable")
-                d_cnt = np.bincount(dtile, minlength=nt)
-                g_end = g_off + np.cumsum(g_cnt)
-                d_end = d_off + np.cumsum(d_cnt)
-                rows.append(np.stack([g_end - g_cnt, g_end, d_end - d_cnt, d_end], axis=1))
-                grp_all.append(g_idx[order])
-                det_all.append(d_order)
-                g_off += int(g_cnt.sum())
-                d_off += int(d_cnt.sum())
-            lists = np.ascontiguousarray(np.concatenate(rows).astype(np.int32))
-            grp = np.concatenate(grp_all + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
-            det = np.concatenate(det_all + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
-            self._lists[depth] = tuple(torch.from_numpy(x).to(self.device) for x in (lists, grp, det))
+            parts = [self._run_lists(r, depth) for r in self.runs]
+            g_n = np.array([p[1].size for p in parts], dtype=np.int64)
+            d_n = np.array([p[2].size for p in parts], dtype=np.int64)
+            t_n = np.array([p[0].shape[0] for p in parts], dtype=np.int64)
+            off = np.stack([np.cumsum(g_n) - g_n, np.cumsum(g_n) - g_n, np.cumsum(d_n) - d_n,
+                            np.cumsum(d_n) - d_n], axis=1)
+            rows = np.concatenate([p[0] for p in parts]) + np.repeat(off, t_n, axis=0)
+            lists = np.ascontiguousarray(rows.astype(np.int32))
+            grp = np.concatenate([p[1] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+            det = np.concatenate([p[2] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+            self._lists[depth] = tuple(to_device_async(x, self.device) for x in (lists, grp, det))
         return self._lists[depth]
 
     def _base_args(self, depth: int, parity: int) -> WfTbArgs:

@@ -42,6 +42,7 @@ from .engine import (
     judge,
     runs_add_step,
     tb_limits_ok,
+    to_device_async,
 )
 
 C0 = 299_792_458.0
@@ -131,13 +132,15 @@ class ImagingServer:
     def admit(self, req: ImagingRequest, now: float) -> None:
         dev = self.device
         nx, ny = req.eps.shape
-        ca, cb = coefficients_from_maps(req.eps, req.sigma, self.dt, device=dev)
-        ca, cb = ca.reshape(nx, ny), cb.reshape(nx, ny)
-        wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(req.n_steps, self.dt, dev,
-                                                                              torch.float32)
+        # tables built on the host and staged to the device without waiting on it
+        ca, cb = coefficients_from_maps(req.eps, req.sigma, self.dt)
+        ca = to_device_async(ca.reshape(nx, ny).numpy(), dev)
+        cb = to_device_async(cb.reshape(nx, ny).numpy(), dev)
+        wave = to_device_async(RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(
+            req.n_steps, self.dt, torch.device("cpu"), torch.float32).numpy(), dev)
         if any(not (0 <= i < nx and 0 <= j < ny) for i, j in req.antennas):
             raise ValueError(f"request {req.rid}: antenna cell outside the {nx}x{ny} grid")
-        cells = torch.tensor([i * ny + j for i, j in req.antennas], dtype=torch.int32, device=dev)
+        cells = to_device_async([i * ny + j for i, j in req.antennas], dev, torch.int32)
         n_ant = len(req.antennas)
         req.traces = torch.zeros((n_ant, n_ant, req.n_steps), dtype=torch.float32, device=dev)
         runs, tables = [], []
@@ -177,8 +180,8 @@ class ImagingServer:
         old = self._stop
         keep = [r.extra["slot"] for r in runs]
         if old is not None and any(k >= 0 for k in keep):
-            idx = torch.tensor([max(k, 0) for k in keep], dtype=torch.long, device=self.device)
-            fresh = torch.tensor([k < 0 for k in keep], dtype=torch.bool, device=self.device)
+            idx = to_device_async([max(k, 0) for k in keep], self.device, torch.long)
+            fresh = to_device_async([k < 0 for k in keep], self.device, torch.bool)
             stop = torch.where(fresh, torch.zeros_like(idx, dtype=torch.int32), old.index_select(0, idx))
         else:
             stop = torch.zeros(len(runs), dtype=torch.int32, device=self.device)
@@ -186,7 +189,6 @@ class ImagingServer:
             r.extra["slot"] = k
         self._plan_runs = runs
         self._plan = TBPlan(runs, self.tile_rows, self.device)
-        self._plan.prepare(self.depth)
         self._stop = stop.contiguous()
 
     def advance_round(self) -> None:
@@ -230,8 +232,8 @@ class ImagingServer:
     def _finish(self, req: ImagingRequest) -> None:
         dev = self.device
         nx, ny = req.eps.shape
-        ax = torch.tensor([a[0] * self.dx for a in req.antennas], dtype=torch.float64, device=dev)
-        ay = torch.tensor([a[1] * self.dx for a in req.antennas], dtype=torch.float64, device=dev)
+        ax = to_device_async([a[0] * self.dx for a in req.antennas], dev, torch.float64)
+        ay = to_device_async([a[1] * self.dx for a in req.antennas], dev, torch.float64)
         req.image = torch.empty((nx, ny), dtype=torch.float64, device=dev)
         run_das(req.traces, ax, ay, req.image, pitch_x=self.dx, pitch_y=self.dx, v_medium=self.v_medium,
                 dt=self.dt, clip=True)
