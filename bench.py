@@ -29,6 +29,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -253,6 +254,61 @@ def slab_suite(dev, slab) -> dict:
             "mcell_updates_per_s": round(slab.n_steps * slab.grid.Nx * slab.grid.Ny / el / 1e6, 1)}
 
 
+def das_batch_timing(phantoms, traces, images, dt, dev, reps: int = 10) -> dict:
+    """The 32 DAS images of the imaging workload formed from its traces two
+    ways: one ``wf_das2d`` launch per image, and ``wf_das2d_batch`` (32
+    images in one launch). Times each with HIP events on the launch stream
+    (mean of ``reps`` repetitions after one warm-up) and checks both equal
+    the images ``image_phantoms`` returned, bit for bit."""
+    import torch
+
+    from waveforge_amd.core.imaging import DasBatch, run_das
+
+    v = 299_792_458.0 / math.sqrt(40.0)
+    jobs = []
+    for p, (eps, _sig, ants) in enumerate(phantoms):
+        nx, ny = eps.shape
+        ax = torch.tensor([a[0] * 1e-3 for a in ants], dtype=torch.float64, device=dev)
+        ay = torch.tensor([a[1] * 1e-3 for a in ants], dtype=torch.float64, device=dev)
+        kw = dict(pitch_x=nx * 1e-3 / nx, pitch_y=ny * 1e-3 / ny, v_medium=v, dt=dt, gate_start=0, clip=True)
+        jobs.append((traces[p], ax, ay, kw))
+    outs = {m: [torch.empty_like(im) for im in images] for m in ("single", "batch")}
+
+    def single():
+        for (S, ax, ay, kw), img in zip(jobs, outs["single"]):
+            run_das(S, ax, ay, img, **kw)
+
+    def batch():
+        b = DasBatch(dev)
+        for (S, ax, ay, kw), img in zip(jobs, outs["batch"]):
+            b.add(S, ax, ay, img, **kw)
+        b.launch()
+
+    res = {}
+    for name, fn in (("single", single), ("batch", batch)):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / reps
+    same = all(torch.equal(a, c) and torch.equal(b, c)
+               for a, b, c in zip(outs["single"], outs["batch"], images))
+    n_img = len(images)
+    return {
+        "launches": {"single": n_img, "batch": -(-n_img // 32)},
+        "ms_all_single_launches": round(res["single"], 3),
+        "ms_all_batched": round(res["batch"], 3),
+        "ms_per_image_single": round(res["single"] / n_img, 4),
+        "ms_per_image_batched": round(res["batch"] / n_img, 4),
+        "speedup": round(res["single"] / res["batch"], 2),
+        "batched_equals_single_bitwise": bool(same),
+    }
+
+
 def imaging_suite(dev) -> dict:
     """Batched MIMO imaging: 32 seeded head phantoms (8 each of 150^2, 200^2,
     256^2, 384^2), 16 TX x 16 RX, 2000 steps, all 512 runs in one batched
@@ -312,6 +368,7 @@ def imaging_suite(dev) -> dict:
         return sum(sum(v) / len(v) for v in per_size.values()) * 8 * 16  # 8 phantoms per size, 16 TX each
     plain_run = projected(run_s)
     plain_step = projected(step_s)
+    das_t = das_batch_timing(phantoms, traces, images, dt, dev)
     peak = [tuple(int(x) for x in divmod(int(img.argmax()), img.shape[1])) for img in images[:4]]
     return {
         "workload": "32 head phantoms (8 x {150,200,256,384}^2), 16 TX x 16 RX, 2000 steps, 512 runs + 32 DAS images",
@@ -327,6 +384,7 @@ def imaging_suite(dev) -> dict:
         "launch_shape": [br.tile_rows, br.depth],
         "diverged_runs": int(sum(st["diverged"])),
         "alone_equals_in_batch": bool(same),
+        "das_32_images": das_t,
         "sampled_runs": len(run_s),
         "peak_pixels_first4": peak,
     }

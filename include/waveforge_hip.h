@@ -431,6 +431,15 @@ typedef struct wf_das2d_args {
 
 int wf_das2d(const wf_das2d_args *args, hipStream_t stream);
 
+/* wf_das2d for n_images images at once: `args` is a HOST array of n_images
+ * descriptors, each validated as wf_das2d validates it (images and trace sets
+ * may differ in size). Up to 32 images share one launch. Each image equals
+ * what wf_das2d gives for its descriptor, bit for bit. Replaces the
+ * reference's one das_backprojection call per phantom
+ * (examples/brain_mimo_imaging.py:211-266) when many images are formed at
+ * once (all phantoms of a batch, all requests a serving round retires). */
+int wf_das2d_batch(const wf_das2d_args *args, int32_t n_images, hipStream_t stream);
+
 /* Envelope delay-and-sum image with layered-medium travel times: replaces
  * AntennaRing.compute_das_image (src/datasets/brain/antenna.py:262-333), the
  * DAS the dataset generator runs on background-subtracted signals

@@ -127,6 +127,57 @@ def test_graph_replay_equals_eager(device):
     assert a.steps_completed == b.steps_completed == steps
 
 
+def test_graph_replay_judges_stability_at_chunk_ends(device):
+    """Three replays of a 20-step chunk with threshold 10 flag the same run
+    as the oracle checking every 20 steps, and that run keeps the fields of
+    its failing check step; the others equal the oracle."""
+    from waveforge_amd.mimo import BatchedRuns
+
+    steps, chunk = 60, 20
+    dt = stable_dt(DX, DX)
+    specs, oracles = _specs(device, steps, dt, sizes=((30, 30), (40, 35), (25, 50)), amps=(1e-3, 1e6, 1e-3))
+    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=5, tile_rows=32, n_check=10 ** 9,
+                     stability_threshold=10.0)
+    br.capture(chunk)
+    br.replay(steps // chunk)
+    st = br.status()
+    for k, o in enumerate(oracles):
+        o.n_check, o.threshold = chunk, 10.0
+        diverged = False
+        try:
+            o.run(steps)
+        except OracleDivergedError:
+            diverged = True
+        assert st["diverged"][k] == diverged, k
+        ex, ey, hz = br.fields(k)
+        assert_bitwise(hz, o.Hz.reshape(o.Nx, o.Ny), f"run {k} Hz")
+        assert_bitwise(ex, o.Ex.reshape(o.Nx, o.Ny), f"run {k} Ex")
+    assert st["diverged"] == [False, True, False] and st["checks"] == 3
+    assert st["field_max"][1] > 10.0 and st["field_max"][0] < 10.0
+
+
+@pytest.mark.parametrize("steps", [97, 49, 95])
+def test_image_phantoms_graph_equals_eager(device, steps):
+    """image_phantoms with graph replay of 48-step chunks equals the eager
+    path bit for bit (traces and images), also when one step is left over
+    (97 = 2 x 48 + 1, 49 = 48 + 1) or a remainder of 47."""
+    from waveforge_amd import mimo, workloads
+
+    dt = workloads.imaging_dt()
+    phantoms = [workloads.head_phantom(n, 8, seed=40 + n, clot=True) for n in (48, 64)]
+    ph = [(p.eps, p.sigma, p.antennas) for p in phantoms]
+    print(f"[image-phantoms] steps={steps} chunk=48", flush=True)
+    img_g, tr_g, br_g = mimo.image_phantoms(ph, steps, dt=dt, device=device, use_graph=True, chunk=48, depth=6)
+    img_e, tr_e, br_e = mimo.image_phantoms(ph, steps, dt=dt, device=device, use_graph=False, depth=6)
+    torch.cuda.synchronize()
+    assert br_g.steps_completed == br_e.steps_completed == steps
+    assert not any(br_g.status()["diverged"])
+    for p in range(2):
+        assert torch.equal(tr_g[p], tr_e[p]), f"phantom {p} traces"
+        assert torch.equal(img_g[p], img_e[p]), f"phantom {p} image"
+        assert float(img_g[p].abs().max()) > 0
+
+
 def test_run_mimo_matches_oracle_per_tx(device):
     from waveforge_amd import core
     from waveforge_amd.mimo import run_mimo

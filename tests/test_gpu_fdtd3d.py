@@ -340,3 +340,29 @@ def test_fused_without_snapshot_equals_per_stage(device, shape, material, batch)
     o.run(14)
     for c in COMPS:
         assert torch.equal(runs[1][0][c].cpu(), o.fields[c]), c
+
+
+def test_waveform_exhaustion_steps_to_the_table_end(device):
+    """run(200) with a 150-sample table and n_check=100 steps to 150, then
+    raises (the reference's per-step loop fails at step 150,
+    src/core/fdtd3d.py:457-486); the fields equal 150 steps run directly."""
+    from waveforge_amd import core
+
+    def sim_of():
+        g = _grid(core, 16, 16, 12, device)
+        f = core.FieldSet(g)
+        src = core.PointSource(core.GaussianPulse(1.0, sigma=8 * g.dt), 8, 8, "Ez", k=6, grid=g, N_steps=150)
+        return core.FDTD3D(g, f, core.MurABC3D(g, f.Hx, f.Hy, f.Hz), core.SourceCollection([src]),
+                           n_check=100), f
+
+    sim, f = sim_of()
+    with pytest.raises(RuntimeError, match="150"):
+        sim.run(200)
+    assert sim.steps_completed == 150 and sim.n_stability_checks == 1
+    ref, fr = sim_of()
+    ref.run(150)
+    for c in COMPS:
+        assert torch.equal(getattr(f, c), getattr(fr, c)), c
+    with pytest.raises(RuntimeError):
+        sim.step()
+    assert sim.steps_completed == 150

@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "wf_tb_tile_shapes",
     "wf_div_verify",
     "wf_das2d",
+    "wf_das2d_batch",
     "wf_adjoint_step",
     "wf_fdtd2d_step_f64",
     "wf_das_envelope",
@@ -229,6 +230,8 @@ def load(path: Path | None = None) -> ctypes.CDLL:
         lib.wf_das_envelope.restype = ctypes.c_int
         lib.wf_das2d.argtypes = [ctypes.POINTER(WfDas2dArgs), _vp]
         lib.wf_das2d.restype = ctypes.c_int
+        lib.wf_das2d_batch.argtypes = [_vp, _i32, _vp]
+        lib.wf_das2d_batch.restype = ctypes.c_int
         ver = lib.wf_abi_version()
         if ver != ABI_VERSION:
             raise RuntimeError(f"waveforge_amd: ABI version {ver} != expected {ABI_VERSION}")

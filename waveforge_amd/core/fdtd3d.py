@@ -276,6 +276,14 @@ class FDTD3D:
                 chunk = min(n_steps - done, self._n_check - self.steps_completed % self._n_check)
                 if verbose:
                     chunk = min(chunk, report - done % report)
+                self._build()  # the source table (cached per set-up)
+                limit = self._src_table.min_wave_len
+                if limit is not None and self.steps_completed + chunk > limit:
+                    # step up to the table's last sample, then raise (as the
+                    # reference's per-step loop does at the first step past it)
+                    chunk = limit - self.steps_completed
+                    if chunk <= 0:
+                        self._advance(1)  # raises with the step named
                 self._advance(chunk)
                 self.steps_completed += chunk
                 done += chunk

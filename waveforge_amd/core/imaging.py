@@ -94,6 +94,16 @@ def run_das(S: torch.Tensor, ax: torch.Tensor, ay: torch.Tensor, image: torch.Te
             pitch_y: float, v_medium: float, dt: float, gate_start: int = 0, clip: bool = True) -> None:
     """Launch ``wf_das2d`` on device tensors (traces fp32 ``(n_tx, n_rx, n_t)``,
     antenna coordinates fp64, image fp64) on the current stream."""
+    a = das_args(S, ax, ay, image, pitch_x=pitch_x, pitch_y=pitch_y, v_medium=v_medium, dt=dt,
+                 gate_start=gate_start, clip=clip)
+    lib = _native.load()
+    stream = _native.stream_handle(S.device)
+    _native.check(lib.wf_das2d(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_das2d")
+
+
+def das_args(S: torch.Tensor, ax: torch.Tensor, ay: torch.Tensor, image: torch.Tensor, *, pitch_x: float,
+             pitch_y: float, v_medium: float, dt: float, gate_start: int = 0, clip: bool = True):
+    """The ``wf_das2d_args`` of one image (device tensors as ``run_das`` takes)."""
     n_tx, n_rx, n_t = S.shape
     a = _native.WfDas2dArgs()
     a.traces, a.ant_x, a.ant_y, a.image = S.data_ptr(), ax.data_ptr(), ay.data_ptr(), image.data_ptr()
@@ -103,6 +113,35 @@ def run_das(S: torch.Tensor, ax: torch.Tensor, ay: torch.Tensor, image: torch.Te
     a.clip = 1 if clip else 0
     a.pitch_x, a.pitch_y = float(pitch_x), float(pitch_y)
     a.v_medium, a.dt = float(v_medium), float(dt)
-    lib = _native.load()
-    stream = _native.stream_handle(S.device)
-    _native.check(lib.wf_das2d(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_das2d")
+    return a
+
+
+class DasBatch:
+    """Several DAS images formed together (``wf_das2d_batch``, up to 32 images
+    a launch): ``add`` takes the arguments of ``run_das``, ``launch`` forms
+    every image added on the current stream. Each image equals what
+    ``run_das`` gives for the same arguments, bit for bit. The batch holds
+    the added tensors until ``launch`` (the descriptors carry only their
+    addresses), so a caller may drop its own references in between."""
+
+    def __init__(self, device) -> None:
+        self.device = torch.device(device)
+        self._args = []
+        self._keep = []
+
+    def add(self, S, ax, ay, image, **kw) -> None:
+        self._args.append(das_args(S, ax, ay, image, **kw))
+        self._keep.append((S, ax, ay, image))
+
+    def __len__(self) -> int:
+        return len(self._args)
+
+    def launch(self) -> None:
+        if not self._args:
+            return
+        arr = (_native.WfDas2dArgs * len(self._args))(*self._args)
+        lib = _native.load()
+        stream = _native.stream_handle(self.device)
+        _native.check(lib.wf_das2d_batch(arr, len(self._args), ctypes.c_void_p(stream)), "wf_das2d_batch")
+        # launched on the current stream: memory freed from here on is reused only by later work on it
+        self._args, self._keep = [], []

@@ -16,7 +16,9 @@ continue (failure isolation), and ``status()`` reports it.
 
 ``use_graph=True`` captures ``chunk`` steps of launches (ending with a
 kernel that advances the runs' step index) in a HIP graph through
-``torch.cuda.CUDAGraph`` and replays it.
+``torch.cuda.CUDAGraph`` and replays it. A replayed chunk judges stability
+once, at its end (a step that is a multiple of ``chunk``, not of
+``n_check``), on the same device flags.
 """
 
 from __future__ import annotations
@@ -144,6 +146,7 @@ class BatchedRuns:
         self.stop = torch.zeros(len(self.runs), dtype=torch.int32, device=self.device)
         self.check_log: list[tuple[int, torch.Tensor]] = []
         self._graph = None
+        self._graph_checks = 0
 
     # ------------------------------------------------------------------
     @property
@@ -192,11 +195,16 @@ class BatchedRuns:
 
     # ------------------------------------------------------------------
     def capture(self, chunk: int) -> None:
-        """Capture ``chunk`` steps (no stability checks inside) in a HIP graph."""
+        """Capture ``chunk`` steps in a HIP graph. The chunk's last launch
+        folds every run's max |Ex|, |Ey|, |Hz| into a slot that keeps the
+        running maximum over all replays (zeroed once, here, outside the
+        graph), and the judge flags a run whose maximum passed the threshold,
+        so a run diverged at any replayed chunk end is stopped and reported."""
         launches, _ = schedule_launches(0, chunk, self.depth, None)
         if launches is None:
             raise ValueError("graph chunk needs an even launch split")
         self.plan.prepare(max(d for d, _ in launches))  # tile tables and lists uploaded before capture
+        self._graph_slot = torch.zeros((len(self.runs), 3), dtype=torch.int32, device=self.device)
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))
         g = torch.cuda.CUDAGraph()
@@ -207,8 +215,11 @@ class BatchedRuns:
                 stream = _native.stream_handle(self.device)
                 off = 0
                 for k, (d, _) in enumerate(launches):
-                    self.plan.launch(d, k % 2, off, stream, stop=self.stop)
+                    last = k == len(launches) - 1
+                    self.plan.launch(d, k % 2, off, stream, maxabs_out=self._graph_slot if last else None,
+                                     stop=self.stop)
                     off += d
+                judge(self._graph_slot, self.threshold, self.stop, False, stream)
                 runs_add_step(self.plan.run_table, len(self.runs), chunk, stream)
         torch.cuda.current_stream(self.device).wait_stream(s)
         self._graph = (g, chunk)
@@ -225,6 +236,9 @@ class BatchedRuns:
             self._dev_step = self.steps_completed
         for _ in range(n_chunks):
             g.replay()
+        if n_chunks > 0 and not self._graph_checks:
+            self.check_log.append((-1, self._graph_slot))  # running maximum over replayed chunk ends
+        self._graph_checks += n_chunks
         self.steps_completed += n_chunks * chunk
         self._dev_step += n_chunks * chunk
 
@@ -238,7 +252,8 @@ class BatchedRuns:
         for _, slot in self.check_log:
             v = slot.view(torch.float32).cpu().numpy().max(axis=1)
             fmax = np.maximum(fmax, np.where(np.isnan(v), -1.0, v))
-        return {"diverged": stop.tolist(), "checks": len(self.check_log), "field_max": fmax.tolist()}
+        n_checks = len(self.check_log) - (1 if self._graph_checks else 0) + self._graph_checks
+        return {"diverged": stop.tolist(), "checks": n_checks, "field_max": fmax.tolist()}
 
 
 # ---------------------------------------------------------------------------
@@ -287,16 +302,24 @@ def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, devic
                    tile_rows=None, n_check: int = 500, v_medium: float | None = None, gate_start: int = 0,
                    use_graph: bool = False, chunk: int = 240):
     """Batched MIMO imaging of many phantoms: all TX runs of all phantoms in
-    one batched launch sequence, then one DAS image per phantom (raw traces).
+    one batched launch sequence, then the DAS images of all phantoms together
+    (``wf_das2d_batch``; raw traces).
 
     ``phantoms`` is a list of ``(eps_r, sigma, antennas)`` (NumPy maps in
     cells, antenna cells). Returns ``(images, traces, batched_runs)`` with
     ``images[p]`` a device fp64 ``(n, n)`` image and ``traces[p]`` the
     ``(N_tx, N_rx, n_steps)`` device signals.
+
+    With ``use_graph`` the first ``q`` whole chunks of ``chunk`` steps are
+    replayed from a HIP graph and the rest runs eagerly; a remainder of one
+    step (which no even launch split covers) joins the last chunk, run
+    eagerly. Replayed chunks judge stability at their ends (multiples of
+    ``chunk``), eager steps at multiples of ``n_check``.
     """
-    from .core.imaging import run_das
+    from .core.imaging import DasBatch
     from .core.materials import coefficients_from_maps
     from .core.sources import RickerWavelet
+    from .engine import to_device_async
 
     wave = RickerWavelet(amplitude=1.0, peak_freq=1e9, t0=1.5 / 1e9).build(n_steps, dt, device, torch.float32)
     specs, owners = [], []
@@ -307,26 +330,43 @@ def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, devic
         owners += [p] * len(sp)
     br = BatchedRuns(specs, dx=dx, dy=dx, dt=dt, device=device, tile_rows=tile_rows, depth=depth,
                      n_check=n_check)
-    if use_graph and n_steps >= chunk:
+    n_chunks, rest = divmod(n_steps, chunk) if use_graph else (0, n_steps)
+    if n_chunks and rest == 1:
+        n_chunks, rest = n_chunks - 1, rest + chunk
+    if n_chunks:
         br.capture(chunk)
-        br.replay(n_steps // chunk)
-        br.advance(n_steps - (n_steps // chunk) * chunk)
-    else:
-        br.advance(n_steps)
-    images, traces = [], []
-    k = 0
+        br.replay(n_chunks)
+    br.advance(rest)
     if v_medium is None:
         v_medium = 299_792_458.0 / math.sqrt(40.0)
+    # antenna coordinates of every phantom in one upload
+    xy = to_device_async(np.array([[a[0] * dx, a[1] * dx] for _, _, ants in phantoms for a in ants],
+                                  dtype=np.float64).T.copy(), device)
+    das = DasBatch(device)
+    images, traces = [], []
+    k = 0
     for p, (eps, sig, ants) in enumerate(phantoms):
         n_ant = len(ants)
-        S = torch.stack([br.traces(k + t) for t in range(n_ant)])
-        k += n_ant
+        S = _trace_block(br, k, n_ant)
         nx, ny = eps.shape
-        ax = torch.tensor([a[0] * dx for a in ants], dtype=torch.float64, device=device)
-        ay = torch.tensor([a[1] * dx for a in ants], dtype=torch.float64, device=device)
         img = torch.empty((nx, ny), dtype=torch.float64, device=device)
-        run_das(S, ax, ay, img, pitch_x=nx * dx / nx, pitch_y=ny * dx / ny, v_medium=v_medium, dt=dt,
-                gate_start=gate_start, clip=True)
+        das.add(S, xy[0, k:k + n_ant], xy[1, k:k + n_ant], img, pitch_x=nx * dx / nx, pitch_y=ny * dx / ny,
+                v_medium=v_medium, dt=dt, gate_start=gate_start, clip=True)
+        k += n_ant
         images.append(img)
         traces.append(S)
+    das.launch()
     return images, traces, br
+
+
+def _trace_block(br: BatchedRuns, k0: int, n: int) -> torch.Tensor:
+    """The ``(n, n_rx, n_record)`` traces of runs ``k0 .. k0+n-1``: a view of
+    the batch's trace buffer when those runs' traces lie back to back in it
+    (as ``BatchedRuns`` lays them out), else a stacked copy."""
+    ts = [br.traces(k0 + t) for t in range(n)]
+    t0 = ts[0]
+    step = t0.numel() * t0.element_size()
+    if all(t.shape == t0.shape and t.is_contiguous() and t.data_ptr() == t0.data_ptr() + i * step
+           for i, t in enumerate(ts)):
+        return t0.as_strided((n,) + tuple(t0.shape), (t0.numel(),) + tuple(t0.stride()), t0.storage_offset())
+    return torch.stack(ts)

@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from . import _native
-from .core.imaging import run_das
+from .core.imaging import DasBatch, run_das
 from .core.materials import coefficients_from_maps
 from .core.sources import RickerWavelet
 from .engine import (
@@ -224,25 +224,35 @@ class ImagingServer:
         finished = [req for req in self.active if req._runs[0].step >= req.n_steps]
         if not finished:
             return
-        for req in finished:
-            self._finish(req)
+        self._finish(finished)
         self.active = [req for req in self.active if req not in finished]
         self._build_plan()
 
-    def _finish(self, req: ImagingRequest) -> None:
+    def _finish(self, reqs: list) -> None:
+        """Enqueue the DAS images of every request in ``reqs`` together (one
+        ``wf_das2d_batch`` call, antenna coordinates in one upload) and the
+        copies of their stop flags, then one event they all complete on."""
         dev = self.device
-        nx, ny = req.eps.shape
-        ax = to_device_async([a[0] * self.dx for a in req.antennas], dev, torch.float64)
-        ay = to_device_async([a[1] * self.dx for a in req.antennas], dev, torch.float64)
-        req.image = torch.empty((nx, ny), dtype=torch.float64, device=dev)
-        run_das(req.traces, ax, ay, req.image, pitch_x=self.dx, pitch_y=self.dx, v_medium=self.v_medium,
-                dt=self.dt, clip=True)
-        lo, hi = req._runs[0].extra["slot"], req._runs[-1].extra["slot"] + 1
-        req._failed_dev = self._stop[lo:hi].clone()  # read once the image is ready (no wait here)
+        xy = to_device_async(np.array([[a[0] * self.dx, a[1] * self.dx] for req in reqs for a in req.antennas],
+                                      dtype=np.float64).T.copy(), dev)
+        das = DasBatch(dev)
+        k = 0
+        for req in reqs:
+            nx, ny = req.eps.shape
+            n = len(req.antennas)
+            req.image = torch.empty((nx, ny), dtype=torch.float64, device=dev)
+            das.add(req.traces, xy[0, k:k + n], xy[1, k:k + n], req.image, pitch_x=self.dx, pitch_y=self.dx,
+                    v_medium=self.v_medium, dt=self.dt, clip=True)
+            k += n
+        das.launch()
+        for req in reqs:
+            lo, hi = req._runs[0].extra["slot"], req._runs[-1].extra["slot"] + 1
+            req._failed_dev = self._stop[lo:hi].clone()  # read once the image is ready (no wait here)
         ev = torch.cuda.Event()
         ev.record()
-        req._event = ev
-        self.retiring.append(req)
+        for req in reqs:
+            req._event = ev
+            self.retiring.append(req)
 
     def poll(self, now_fn) -> None:
         """Complete retiring requests whose image is ready; free their memory."""
