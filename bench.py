@@ -537,14 +537,35 @@ def adjoint_suite(dev, n_fd: int) -> dict:
     ca64, cb64 = coefficients(eps, sig, dt)
     ca, cb = ca64.float().contiguous(), cb64.float().contiguous()
     torch.cuda.synchronize()
+    prob.history = "checkpoint"  # the plain forward: traces and a checkpoint every 100 steps
     t0 = time.perf_counter()
     d32, _ = prob.forward(ca, cb)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
+    prob.history = "auto"
+    full = prob.full_history()
     t0 = time.perf_counter()
     loss, ge, gs = misfit_gradient(eps, sig, prob, d_obs)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+
+    def grad_with(**kw):  # the same gradient by another path (warm-up, then timed)
+        old = {k: getattr(prob, k) for k in kw}
+        for k, v in kw.items():
+            setattr(prob, k, v)
+        misfit_gradient(eps, sig, prob, d_obs)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _, g_e, g_s = misfit_gradient(eps, sig, prob, d_obs)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t1
+        for k, v in old.items():
+            setattr(prob, k, v)
+        return el, bool(torch.equal(ge, g_e) and torch.equal(gs, g_s))
+
+    # checkpoints + recompute with two-step adjoint launches, and with one step a launch
+    t_grad_ck, same_ck = grad_with(history="checkpoint")
+    t_grad_one, same_one = grad_with(history="checkpoint", one_step=True)
     t0 = time.perf_counter()
     loss64, ge64, gs64 = misfit_gradient(eps, sig, prob, d_obs, fp64_residual=True)
     torch.cuda.synchronize()
@@ -582,6 +603,11 @@ def adjoint_suite(dev, n_fd: int) -> dict:
     out = {"workload": "256^2 head phantom (clot-free model, clot data), 16 TX x 16 RX, 3000 steps, "
                        "checkpoint every 100, loss 0.5||d_sim - d_obs||^2",
            "loss": loss, "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+           "gradient_path": ("full state history (%.1f GB), two adjoint steps per launch" % (prob.history_bytes() / 1e9)
+                             if full else "checkpoints every 100 steps + recompute, two adjoint steps per launch"),
+           "gradient_checkpoint_recompute_s": round(t_grad_ck, 4),
+           "gradient_checkpoint_one_step_per_launch_s": round(t_grad_one, 4),
+           "gradients_equal_bitwise_across_paths": bool(same_ck and same_one),
            "forward_mcell_updates_per_s": round(cu / t_fwd / 1e6, 1),
            "gradient_cell_updates": 3 * cu, "gradient_mcell_updates_per_s": round(3 * cu / t_grad / 1e6, 1),
            "traces_rel_l2_vs_fp64": fp64_err, "residual_rel_l2": resid_rel,

@@ -372,14 +372,19 @@ int wf_div_verify(float d, float r, uint32_t lo, uint32_t span, uint32_t *bad, h
  * fields before step n, hz_next the Hz after it. Sources are affine and drop
  * out. With n_steps > 1 the call runs steps step, step-1, ..., step-n_steps+1
  * in turn, the state pointers moving back by hist_stride floats per step (a
- * segment of a state history laid out slot after slot). Each step is one
- * launch that reads the adjoint state from one set and writes the other
- * (lex2/ley2/lhz2 are the second set, overwritten); the state is back in
- * (lex, ley, lhz) when the call returns. No reference code counterpart: the
+ * segment of a state history laid out slot after slot). Two consecutive
+ * steps share one launch when the receivers' records can be folded in (no
+ * receivers, or rx_first/rx_order given) unless flags has WF_ADJ_ONE_STEP;
+ * otherwise each step is one launch. Every launch reads the adjoint state
+ * from one set and writes the other (lex2/ley2/lhz2 are the second set,
+ * overwritten); the state is back in (lex, ley, lhz) when the call returns.
+ * All three launch forms give the same bits. No reference code counterpart: the
  * reference's differentiable-FDTD plan is design prose
  * (docs/architecture/DIFFERENTIABLE_PHYSICS.md:33); the forward it
  * differentiates is FDTD2D.step (src/core/fdtd2d.py:187-266). */
 #define WF_ADJ_TWO_PASS 1
+#define WF_ADJ_ONE_STEP 2  /* one launch per step even where two steps could share one */
+#define WF_ADJ_FAST_DIV 4  /* two-step launches divide by dx, dy with the verified fast quotient */
 
 typedef struct wf_adj_args {
     const float *ca, *cb;            /* (nx, ny), shared by all runs */
@@ -402,8 +407,13 @@ typedef struct wf_adj_args {
                                         are added in that order and all steps of a call after the
                                         first record inside the previous step's last kernel */
     int32_t flags;                   /* WF_ADJ_TWO_PASS: two launches per step (k_adj_e, k_adj_h)
-                                        instead of one; the results are the same bits */
+                                        instead of one; WF_ADJ_ONE_STEP: one launch per step;
+                                        WF_ADJ_FAST_DIV: the fields below hold verified divisions.
+                                        Every form gives the same bits */
     int32_t pad2;
+    float div_rx, div_ry;            /* with WF_ADJ_FAST_DIV: RN(1/dx), RN(1/dy) */
+    uint32_t div_span_x, div_span_y; /* and the |x| bit ranges [0, span) on which wf_div_verify
+                                        found the fast quotient by dx, dy equal to IEEE x/d */
 } wf_adj_args;
 
 int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream);

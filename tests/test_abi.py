@@ -92,7 +92,7 @@ int main(void) {
   printf("wf_adj_args size %zu\n", sizeof(wf_adj_args));
   O(wf_adj_args, ca) O(wf_adj_args, hz_next) O(wf_adj_args, lp) O(wf_adj_args, gcb) O(wf_adj_args, dl_dd)
   O(wf_adj_args, rx_cell) O(wf_adj_args, run_stride) O(wf_adj_args, dd_rx_stride) O(wf_adj_args, nx)
-  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, n_steps) O(wf_adj_args, hist_stride) O(wf_adj_args, rx_first) O(wf_adj_args, rx_order) O(wf_adj_args, flags)
+  O(wf_adj_args, step) O(wf_adj_args, dh) O(wf_adj_args, cmy) O(wf_adj_args, n_steps) O(wf_adj_args, hist_stride) O(wf_adj_args, rx_first) O(wf_adj_args, rx_order) O(wf_adj_args, flags) O(wf_adj_args, div_rx) O(wf_adj_args, div_span_y)
   printf("wf_inj2d_f64 size %zu\nwf_run2d_f64 size %zu\nwf_f64_args size %zu\n",
          sizeof(wf_inj2d_f64), sizeof(wf_run2d_f64), sizeof(wf_f64_args));
   O(wf_inj2d_f64, wave) O(wf_inj2d_f64, cell) O(wf_inj2d_f64, amp)

@@ -135,6 +135,7 @@ def test_recompute_history_equals_oracle_states(device, tile, depth):
     eps, sig = _media(prob.nx, prob.ny, 2)
     ca64, cb64 = coefficients(eps.double(), sig.double(), dt)
     ca, cb = ca64.float().contiguous(), cb64.float().contiguous()
+    prob.history = "checkpoint"
     _, ckpts = prob.forward(ca.to(device), cb.to(device))
     s0, L = 30, 30
     H = prob._recompute(prob._history(ca.to(device), cb.to(device)), ckpts[1], s0, L)
@@ -150,19 +151,120 @@ def test_recompute_history_equals_oracle_states(device, tile, depth):
                 assert_bitwise(H[k, r, ci], o.fields[name], f"run {r} step {s0 + k} {name}")
 
 
-def test_one_launch_step_equals_two_launch_step(device):
-    """The one-launch adjoint step (stages through LDS, state alternating
-    between two sets) gives the same gradient bits as the two-launch step,
-    also for checkpoint segments of odd length and grids that are not
-    multiples of its tile."""
+@pytest.mark.parametrize("nx,ny,steps,ck,rx", [
+    (75, 131, 121, 45, ((3, 20), (74, 4), (14, 130), (0, 0), (40, 66))),
+    (17, 57, 40, 13, ((0, 56), (16, 0), (8, 28), (8, 28))),
+    (5, 6, 30, 7, ((4, 5), (1, 1))),
+    (40, 40, 24, 24, None),
+])
+def test_adjoint_launch_forms_give_the_same_bits(device, nx, ny, steps, ck, rx):
+    """The adjoint step's launch forms give the same gradient bits: two steps
+    per launch with the verified fast division by dx, dy (the default; the
+    last step of an odd segment alone) and with IEEE division, one step per
+    launch (stages through LDS), and two launches per step;
+    also for segments of odd length, grids that are not multiples of any of
+    their tiles or smaller than one, receivers on the faces and corners or
+    sharing a cell, and no receivers (rx None: the adjoint call gets no
+    residuals and starts from a seeded adjoint state instead, the gradient
+    of <lam0, u_n>)."""
     from waveforge_amd.adjoint import misfit_gradient
 
-    prob, wave, dt = _setup(device, nx=75, ny=131, steps=121, ck=45, tx=((6, 5), (60, 100), (37, 64)),
-                            rx=((3, 20), (74, 4), (14, 130), (0, 0), (40, 66)))
+    tx = ((nx // 2, ny // 3), (nx - 2, 1))
+    prob, wave, dt = _setup(device, nx=nx, ny=ny, steps=steps, ck=ck, tx=tx, rx=rx or ((1, 1),))
     eps, sig = _media(prob.nx, prob.ny, 5)
     g = torch.Generator().manual_seed(3)
+    if rx:
+        d_obs = 1e-3 * torch.randn((len(prob.tx), len(prob.rx), prob.n_steps), generator=g, dtype=torch.float64)
+        assert prob._divisions() is not None  # the default form takes the fast division
+        runs = [misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device))]
+        for attr, val in (("fast_div", False), ("one_step", True), ("two_pass", True)):
+            old = getattr(prob, attr)
+            setattr(prob, attr, val)
+            runs.append(misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device)))
+            setattr(prob, attr, old)
+        grads = [(ge, gs) for _, ge, gs in runs]
+    else:
+        grads = [_seeded_adjoint(prob, eps, sig, dt, device, flags) for flags in (4, 0, 2, 1)]
+    print(f"[adjoint] {nx}x{ny} steps={steps} ck={ck} rx={rx}", flush=True)
+    assert bool(grads[0][0].abs().max() > 0) and bool(grads[0][1].abs().max() > 0)
+    for k, (ge, gs) in enumerate(grads[1:], 1):
+        assert torch.equal(grads[0][0], ge) and torch.equal(grads[0][1], gs), f"launch form {k}"
+
+
+def _seeded_adjoint(prob, eps, sig, dt, device, flags):
+    """dL/dCa, dL/dCb through one wf_adjoint_step call over all of the
+    problem's steps (one segment) with no receivers: the adjoint state starts
+    from a seeded field instead of trace residuals."""
+    import ctypes
+
+    from waveforge_amd import _native
+    from waveforge_amd.adjoint import WfAdjArgs, coefficients
+
+    ca64, cb64 = coefficients(eps.double(), sig.double(), dt)
+    ca, cb = ca64.float().contiguous().to(device), cb64.float().contiguous().to(device)
+    prob.history = "checkpoint"
+    _, ckpts = prob.forward(ca, cb)
+    n_tx, nx, ny = len(prob.tx), prob.nx, prob.ny
+    gen = torch.Generator().manual_seed(11)
+    lam = [torch.randn((n_tx, nx, ny), generator=gen).to(device) for _ in range(3)]
+    z = lambda dt=torch.float32: torch.zeros((n_tx, nx, ny), dtype=dt, device=device)  # noqa: E731
+    lex2, ley2, lhz2, lp = z(), z(), z(), z()
+    gca, gcb = z(torch.float64), z(torch.float64)
+    hist = prob._history(ca, cb)
+    L = prob.n_steps
+    prob._recompute(hist, ckpts[0], 0, L)
+    H = hist[0]
+    a = WfAdjArgs()
+    a.ca, a.cb = ca.data_ptr(), cb.data_ptr()
+    a.lex, a.ley, a.lhz = (t.data_ptr() for t in lam)
+    a.lex2, a.ley2, a.lhz2, a.lp = lex2.data_ptr(), ley2.data_ptr(), lhz2.data_ptr(), lp.data_ptr()
+    a.gca, a.gcb = gca.data_ptr(), gcb.data_ptr()
+    a.flags = flags
+    if flags & _native.WF_ADJ_FAST_DIV:
+        (a.div_rx, _, a.div_span_x), (a.div_ry, _, a.div_span_y) = prob._divisions()
+    a.run_stride, a.state_stride = nx * ny, 3 * nx * ny
+    a.nx, a.ny, a.n_runs, a.n_rx = nx, ny, n_tx, 0
+    c = prob.coef
+    a.dh, a.dx, a.dy, a.cmx, a.cmy = c.dh, c.dx, c.dy, c.cmx, c.cmy
+    step_bytes = n_tx * 3 * nx * ny * 4
+    k = L - 1
+    a.step, a.n_steps, a.hist_stride = k, L, step_bytes // 4
+    a.ex_prev = H.data_ptr() + k * step_bytes
+    a.ey_prev = H.data_ptr() + k * step_bytes + nx * ny * 4
+    a.hz_next = H.data_ptr() + (k + 1) * step_bytes + 2 * nx * ny * 4
+    lib = _native.load()
+    _native.check(lib.wf_adjoint_step(ctypes.byref(a), ctypes.c_void_p(_native.stream_handle(device))),
+                  "wf_adjoint_step")
+    torch.cuda.synchronize()
+    return gca.sum(0), gcb.sum(0)
+
+@pytest.mark.parametrize("nx,ny,steps,ck", [(75, 131, 121, 45), (40, 52, 64, 16), (20, 20, 9, 4)])
+def test_full_history_equals_checkpoint_recompute(device, nx, ny, steps, ck):
+    """The full-history path (one forward writing every step's state, the
+    adjoint straight through it) gives the checkpointed path's traces and
+    gradients bit for bit, and auto mode picks it within its budget and the
+    checkpoints beyond."""
+    from waveforge_amd.adjoint import FullHistory, coefficients, misfit_gradient
+
+    prob, wave, dt = _setup(device, nx=nx, ny=ny, steps=steps, ck=ck, tx=((nx // 2, ny // 3), (nx - 3, 2)),
+                            rx=((3, 5), (nx - 1, ny // 2), (nx // 2, ny // 3), (0, 0)))
+    eps, sig = _media(prob.nx, prob.ny, 7)
+    ca64, cb64 = coefficients(eps.double(), sig.double(), dt)
+    ca, cb = ca64.float().contiguous().to(device), cb64.float().contiguous().to(device)
+    g = torch.Generator().manual_seed(4)
     d_obs = 1e-3 * torch.randn((len(prob.tx), len(prob.rx), prob.n_steps), generator=g, dtype=torch.float64)
-    _, ge1, gs1 = misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device))
-    prob.two_pass = True
-    _, ge0, gs0 = misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device))
-    assert torch.equal(ge1, ge0) and torch.equal(gs1, gs0)
+    out = {}
+    for mode in ("full", "checkpoint"):
+        prob.history = mode
+        tr, saved = prob.forward(ca, cb)
+        assert isinstance(saved, FullHistory) == (mode == "full")
+        out[mode] = (tr,) + tuple(misfit_gradient(eps.to(device), sig.to(device), prob, d_obs.to(device))[1:])
+    print(f"[adjoint-history] {nx}x{ny} steps={steps} checkpoint={ck}", flush=True)
+    for k, name in enumerate(("traces", "dL/deps", "dL/dsigma")):
+        assert torch.equal(out["full"][k], out["checkpoint"][k]), name
+    assert bool(out["full"][1].abs().max() > 0)
+    prob.history = "auto"
+    prob.history_budget = prob.history_bytes()
+    assert prob.full_history()
+    prob.history_budget = prob.history_bytes() - 1
+    assert not prob.full_history()

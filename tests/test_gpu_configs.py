@@ -231,7 +231,17 @@ def test_adjoint_head_256_3000_steps(device):
     e_tr = rel_l2(d32, d64)
     _log(f"adjoint: forward traces fp32 vs fp64 rel L2 {e_tr:.3e}")
     assert e_tr < 1e-5
-    _, ge32, _ = misfit_gradient(eps, sig, prob, d_obs)
+    _, ge32, gs32 = misfit_gradient(eps, sig, prob, d_obs)
+    # every adjoint path gives the same bits: the full state history (the
+    # default here), checkpoints + recompute, and one adjoint step per launch
+    assert prob.full_history()
+    for kw in ({"history": "checkpoint"}, {"history": "checkpoint", "one_step": True}):
+        for k, v in kw.items():
+            setattr(prob, k, v)
+        _, ge_p, gs_p = misfit_gradient(eps, sig, prob, d_obs)
+        prob.history, prob.one_step = "auto", False
+        assert torch.equal(ge_p, ge32) and torch.equal(gs_p, gs32), f"adjoint path {kw}"
+    _log("adjoint: full history == checkpoint/recompute == one step per launch, bit for bit")
     _, ge, gs = misfit_gradient(eps, sig, prob, d_obs, fp64_residual=True)
     ge32, ge, gs = ge32.double().cpu(), ge.double().cpu(), gs.double().cpu()
     d_obs64 = d_obs.double()

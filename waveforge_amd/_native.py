@@ -17,7 +17,9 @@ ABI_VERSION = 2
 WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 WF_COMP3 = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
 WF_RUN3D_SNAP_FROM_STATE = 1
-WF_ADJ_TWO_PASS = 1  # wf_run3d.flags: the Mur snapshot equals the read state
+WF_ADJ_TWO_PASS = 1  # wf_adj_args.flags: two launches per adjoint step
+WF_ADJ_ONE_STEP = 2  # wf_adj_args.flags: one launch per step (no two-step launches)
+WF_ADJ_FAST_DIV = 4  # wf_adj_args.flags: verified fast division in two-step launches
 TB_MAX_DEPTH = 8
 
 # Every symbol the header declares; tests check the built library exports them.
@@ -127,7 +129,8 @@ class WfAdjArgs(ctypes.Structure):
         [(n, _i64) for n in ("run_stride", "state_stride", "dd_run_stride", "dd_rx_stride")] + \
         [(n, _i32) for n in ("nx", "ny", "n_runs", "n_rx", "step")] + \
         [(n, _f32) for n in ("dh", "dx", "dy", "cmx", "cmy")] + [("n_steps", _i32), ("hist_stride", _i64)] + \
-        [("rx_first", ctypes.c_void_p), ("rx_order", ctypes.c_void_p), ("flags", _i32), ("pad2", _i32)]
+        [("rx_first", ctypes.c_void_p), ("rx_order", ctypes.c_void_p), ("flags", _i32), ("pad2", _i32)] + \
+        [("div_rx", _f32), ("div_ry", _f32), ("div_span_x", ctypes.c_uint32), ("div_span_y", ctypes.c_uint32)]
 
 
 class WfInj2dF64(ctypes.Structure):

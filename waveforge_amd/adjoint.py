@@ -38,6 +38,13 @@ def coefficients(eps: torch.Tensor, sigma: torch.Tensor, dt: float) -> tuple[tor
 
 
 @dataclass
+class FullHistory:
+    """Every step's state of a forward: ``states[k]`` is u_k, ``(n_tx, 3, nx, ny)``."""
+
+    states: torch.Tensor
+
+
+@dataclass
 class TraceProblem:
     nx: int
     ny: int
@@ -48,9 +55,16 @@ class TraceProblem:
     wave: torch.Tensor          # fp32 on the device, >= n_steps samples
     n_steps: int
     checkpoint: int = 100
-    depth: int = 6
+    depth: object = 6           # forward launch depth (or "auto": the autotuner's)
+    tile_rows: object = None    # forward tile rows (None: FDTD2D's default; "auto": the autotuner's)
     receiver_map: bool = True   # False: one record launch per adjoint step (atomic adds)
     two_pass: bool = False      # True: two launches per adjoint step (same results)
+    one_step: bool = False      # True: one launch per adjoint step, never two steps a launch (same results)
+    fast_div: bool = True       # two-step launches divide by dx, dy with the device-verified fast quotient
+    history: str = "auto"       # "full": the forward keeps every step's state and backward runs straight
+                                # through it; "checkpoint": every `checkpoint`-th state, each segment
+                                # recomputed in backward; "auto": full when it fits in history_budget
+    history_budget: Optional[float] = None  # bytes for a full history (None: half the free device memory)
 
     def __post_init__(self):
         self.device = self.wave.device
@@ -70,13 +84,35 @@ class TraceProblem:
         self.rx_first = first.to(self.device)
 
     # ------------------------------------------------------------------
+    def history_bytes(self) -> int:
+        """Device bytes of the full state history (every step's Ex, Ey, Hz of every TX)."""
+        return (self.n_steps + 1) * len(self.tx) * 3 * self.nx * self.ny * 4
+
+    def full_history(self) -> bool:
+        """Whether ``forward`` keeps the full state history (``history`` mode)."""
+        if self.history in ("full", "checkpoint"):
+            return self.history == "full"
+        if self.history != "auto":
+            raise ValueError(f"history must be 'auto', 'full' or 'checkpoint', got {self.history!r}")
+        budget = self.history_budget
+        if budget is None:
+            budget = 0.5 * torch.cuda.mem_get_info(self.device)[0]
+        return self.history_bytes() <= budget
+
+    def _specs(self, ca, cb):
+        return [RunSpec(self.nx, self.ny, ca, cb, sources=[("Hz", i, j, self.wave, 1.0)], receivers=list(self.rx),
+                        n_record=self.n_steps) for (i, j) in self.tx]
+
     def forward(self, ca: torch.Tensor, cb: torch.Tensor):
-        """Traces ``(n_tx, n_rx, n_steps)`` and the list of checkpoints
+        """Traces ``(n_tx, n_rx, n_steps)`` and what backward needs: with the
+        full history, ``FullHistory`` (every step's state, written by the same
+        launches that record the traces); otherwise the list of checkpoints
         ``(n_tx, 3, nx, ny)`` taken at steps 0, K, 2K, ..."""
-        specs = [RunSpec(self.nx, self.ny, ca, cb, sources=[("Hz", i, j, self.wave, 1.0)], receivers=list(self.rx),
-                         n_record=self.n_steps) for (i, j) in self.tx]
+        if self.full_history():
+            return self._forward_full(ca, cb)
+        specs = self._specs(ca, cb)
         br = BatchedRuns(specs, dx=self.dx, dy=self.dx, dt=self.dt, device=self.device, depth=self.depth,
-                         n_check=10 ** 9)
+                         tile_rows=self.tile_rows, n_check=10 ** 9)
         ckpts = []
         done = 0
         K = self.checkpoint
@@ -99,6 +135,21 @@ class TraceProblem:
         self._tables = [r.sources for r in br.runs]
         return traces, ckpts
 
+    def _forward_full(self, ca: torch.Tensor, cb: torch.Tensor):
+        """The forward as one pass of state-history launches from the zero
+        state over all n_steps: slot k of the history holds u_k, and the
+        launches record the receivers (the stepper's arithmetic, so traces and
+        states equal the checkpointed forward's bit for bit)."""
+        specs = self._specs(ca, cb)
+        # the runs' source tables, receiver cells and trace buffer (their state is not stepped)
+        br = BatchedRuns(specs, dx=self.dx, dy=self.dx, dt=self.dt, device=self.device, depth=1, tile_rows=48,
+                         n_check=10 ** 9)
+        self._tables = [r.sources for r in br.runs]
+        hist = self._history(ca, cb, self.n_steps, recorders=br.runs)
+        self._recompute(hist, None, 0, self.n_steps)
+        traces = torch.stack([br.traces(k) for k in range(len(specs))])
+        return traces, FullHistory(hist[0])
+
     def forward_fp64(self, ca64: torch.Tensor, cb64: torch.Tensor) -> torch.Tensor:
         """fp64 traces ``(n_tx, n_rx, n_steps)`` for fp64 coefficients (the
         double-precision step, no checkpoints): the forward of central-difference
@@ -111,19 +162,33 @@ class TraceProblem:
         runs.advance(self.n_steps)
         return torch.stack([runs.traces(k) for k in range(len(specs))])
 
+    def _divisions(self):
+        """The verified fast divisions by dx and dy (``engine.verified_division``),
+        or None when either divisor has none."""
+        from .engine import verified_division
+
+        vx = verified_division(self.coef.dx, self.device)
+        vy = verified_division(self.coef.dy, self.device)
+        return None if vx is None or vy is None else (vx, vy)
+
     # ------------------------------------------------------------------
-    def _history(self, ca, cb):
-        """State history buffer ``H[k] = u_{s0+k}`` (k = 0..K) for one segment
-        and the run table whose entry k makes a launch read H[k] and write
-        H[k+1], H[k+2], ... one slot per step (descriptor step = k; the launch's
-        step_offset adds s0). Built once per backward: descriptors differ only
-        in their state pointers."""
-        n_tx, nx, ny, K = len(self.tx), self.nx, self.ny, self.checkpoint
+    def _history(self, ca, cb, K: Optional[int] = None, recorders=None):
+        """State history buffer ``H[k] = u_{s0+k}`` (k = 0..K, K the checkpoint
+        interval by default) and the run table whose entry k makes a launch
+        read H[k] and write H[k+1], H[k+2], ... one slot per step (descriptor
+        step = k; the launch's step_offset adds s0). With ``recorders`` (the
+        forward's RunStates) the descriptors also record their receivers.
+        Descriptors differ only in their state pointers and step."""
+        n_tx, nx, ny = len(self.tx), self.nx, self.ny
+        K = self.checkpoint if K is None else K
         H = torch.empty((K + 1, n_tx, 3, nx, ny), dtype=torch.float32, device=self.device)
         base = []
         for r in range(n_tx):
+            rec = recorders[r] if recorders is not None else None
             rs = RunState(H[0, r, 0], H[0, r, 1], H[0, r, 2], ca, cb, nx, ny, self.coef,
-                          sources=self._tables[r], step=0)
+                          sources=self._tables[r], step=0,
+                          det_cell=None if rec is None else rec.det_cell,
+                          traces=None if rec is None else rec.traces, trace_n0=0)
             rs.second = (H[1, r, 0], H[1, r, 1], H[1, r, 2])
             base.append(rs.struct())
         sz = ctypes.sizeof(WfRun2d)
@@ -161,7 +226,10 @@ class TraceProblem:
         writing the state after every one of its steps into its slot."""
         H, table = hist
         n_tx, nx, ny = len(self.tx), self.nx, self.ny
-        H[0].copy_(start)
+        if start is None:
+            H[0].zero_()
+        else:
+            H[0].copy_(start)
         lib = _native.load()
         stream = _native.stream_handle(self.device)
         sz = ctypes.sizeof(WfRun2d)
@@ -199,28 +267,40 @@ class TraceProblem:
         a.dl_dd, a.rx_cell = dl_dd.data_ptr(), self.rx_cells.data_ptr()
         if len(self.rx) and self.receiver_map:
             a.rx_first, a.rx_order = self.rx_first.data_ptr(), self.rx_order.data_ptr()
-        a.flags = _native.WF_ADJ_TWO_PASS if self.two_pass else 0
+        a.flags = (_native.WF_ADJ_TWO_PASS if self.two_pass else 0) | (_native.WF_ADJ_ONE_STEP if self.one_step else 0)
+        div = self._divisions() if self.fast_div else None
+        if div is not None:
+            (a.div_rx, _, a.div_span_x), (a.div_ry, _, a.div_span_y) = div
+            a.flags |= _native.WF_ADJ_FAST_DIV
         a.run_stride = nx * ny
         a.state_stride = 3 * nx * ny
         a.dd_run_stride, a.dd_rx_stride = dl_dd.stride(0), dl_dd.stride(1)
         a.nx, a.ny, a.n_runs, a.n_rx = nx, ny, n_tx, len(self.rx)
         c = self.coef
         a.dh, a.dx, a.dy, a.cmx, a.cmy = c.dh, c.dx, c.dy, c.cmx, c.cmy
-        hist = self._history(ca, cb)
-        H = hist[0]
         step_bytes = n_tx * 3 * nx * ny * 4
-        h0 = H.data_ptr()
-        for seg in range(len(ckpts) - 1, -1, -1):
-            s0 = seg * K
-            L = min(K, self.n_steps - s0)
-            self._recompute(hist, ckpts[seg], s0, L)
-            # the segment's L adjoint steps in one call, from step s0+L-1 back to s0
+        a.hist_stride = step_bytes // 4
+
+        def adjoint_segment(h0: int, s0: int, L: int) -> None:
+            # L adjoint steps in one call, from step s0+L-1 back to s0, over a
+            # history whose slot 0 (at address h0) holds u_{s0}
             k = L - 1
-            a.step, a.n_steps, a.hist_stride = s0 + k, L, step_bytes // 4
+            a.step, a.n_steps = s0 + k, L
             a.ex_prev = h0 + k * step_bytes
             a.ey_prev = h0 + k * step_bytes + nx * ny * 4
             a.hz_next = h0 + (k + 1) * step_bytes + 2 * nx * ny * 4
             _native.check(lib.wf_adjoint_step(ctypes.byref(a), ctypes.c_void_p(stream)), "wf_adjoint_step")
+
+        if isinstance(ckpts, FullHistory):
+            adjoint_segment(ckpts.states.data_ptr(), 0, self.n_steps)
+            return gca.sum(0), gcb.sum(0)
+        hist = self._history(ca, cb)
+        H = hist[0]
+        for seg in range(len(ckpts) - 1, -1, -1):
+            s0 = seg * K
+            L = min(K, self.n_steps - s0)
+            self._recompute(hist, ckpts[seg], s0, L)
+            adjoint_segment(H.data_ptr(), s0, L)
         del hist, H
         return gca.sum(0), gcb.sum(0)
 

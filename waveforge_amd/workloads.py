@@ -221,7 +221,9 @@ def adjoint_case(device="cuda", n: int = 256, n_steps: int = 3000, seed: int = 0
                  depth: int = 6, n_elements: int = 16) -> AdjointCase:
     """256^2 head phantom, 16 TX x 16 RX, 3000 steps: d_obs from the clot
     phantom, the model is its clot-free twin (same seed, same head), loss
-    0.5 ||d_sim - d_obs||^2, checkpoints every ``checkpoint`` steps."""
+    0.5 ||d_sim - d_obs||^2; the gradient keeps the full state history when
+    it fits (``TraceProblem.history``), else checkpoints every ``checkpoint``
+    steps."""
     from .adjoint import TraceProblem, coefficients
 
     dev = torch.device(device)
@@ -234,5 +236,7 @@ def adjoint_case(device="cuda", n: int = 256, n_steps: int = 3000, seed: int = 0
     e_c = torch.from_numpy(ph.eps).to(dev)
     s_c = torch.from_numpy(ph.sigma).to(dev)
     ca, cb = coefficients(e_c, s_c, dt)
+    prob.history = "checkpoint"  # the data need only the traces
     d_obs, _ = prob.forward(ca.float().contiguous(), cb.float().contiguous())
+    prob.history = "auto"
     return AdjointCase(prob, torch.from_numpy(free.eps).to(dev), torch.from_numpy(free.sigma).to(dev), d_obs, ph)
