@@ -59,6 +59,8 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="skip the serving section")
     p.add_argument("--serving-requests", type=int, default=200)
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient section")
+    p.add_argument("--adjoint-history", default="auto", choices=("auto", "full", "checkpoint"),
+                   help="the adjoint section's state history (TraceProblem.history) for its main gradient")
     p.add_argument("--no-3d", action="store_true", help="skip the 3D section")
     p.add_argument("--no-das", action="store_true", help="skip the DAS and reference-CPU-case sections")
     p.add_argument("--size3d", type=int, default=256)
@@ -514,7 +516,7 @@ def fdtd3d_suite(dev, n: int, steps: int = 200, warmup: int = 20) -> dict:
     return out
 
 
-def adjoint_suite(dev, n_fd: int) -> dict:
+def adjoint_suite(dev, n_fd: int, history: str = "auto") -> dict:
     """Trace-misfit gradient on the 256^2 head phantom, 16 TX x 16 RX, 3000
     steps, checkpoints every 100 steps: d_obs from the clot phantom, the model
     is the clot-free twin. Times the fp32 forward and forward + adjoint
@@ -542,12 +544,14 @@ def adjoint_suite(dev, n_fd: int) -> dict:
     d32, _ = prob.forward(ca, cb)
     torch.cuda.synchronize()
     t_fwd = time.perf_counter() - t0
-    prob.history = "auto"
+    prob.history = history
     full = prob.full_history()
     t0 = time.perf_counter()
     loss, ge, gs = misfit_gradient(eps, sig, prob, d_obs)
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
+    print(f"[bench] adjoint: gradient ({'full history' if full else 'checkpoints'}) {t_grad * 1e3:.1f} ms",
+          file=sys.stderr, flush=True)
 
     def grad_with(**kw):  # the same gradient by another path (warm-up, then timed)
         old = {k: getattr(prob, k) for k in kw}
@@ -563,9 +567,27 @@ def adjoint_suite(dev, n_fd: int) -> dict:
             setattr(prob, k, v)
         return el, bool(torch.equal(ge, g_e) and torch.equal(gs, g_s))
 
+
     # checkpoints + recompute with two-step adjoint launches, and with one step a launch
-    t_grad_ck, same_ck = grad_with(history="checkpoint")
-    t_grad_one, same_one = grad_with(history="checkpoint", one_step=True)
+    if history == "checkpoint":  # the main gradient is already the checkpointed one
+        t_grad_ck, same_ck = t_grad, True
+        t_grad_one, same_one = grad_with(one_step=True)
+    else:
+        t_grad_ck, same_ck = grad_with(history="checkpoint")
+        t_grad_one, same_one = grad_with(history="checkpoint", one_step=True)
+    print("[bench] adjoint: checkpointed and one-step gradients done", file=sys.stderr, flush=True)
+    out = {"workload": "256^2 head phantom (clot-free model, clot data), 16 TX x 16 RX, 3000 steps, "
+                       "loss 0.5||d_sim - d_obs||^2",
+           "loss": loss, "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
+           "gradient_path": ("full state history (%.1f GB), two adjoint steps per launch" % (prob.history_bytes() / 1e9)
+                             if full else "checkpoints every 100 steps + recompute, two adjoint steps per launch"),
+           "gradient_checkpoint_recompute_s": round(t_grad_ck, 4),
+           "gradient_checkpoint_one_step_per_launch_s": round(t_grad_one, 4),
+           "gradients_equal_bitwise_across_paths": bool(same_ck and same_one),
+           "forward_mcell_updates_per_s": round(cu / t_fwd / 1e6, 1),
+           "gradient_cell_updates": 3 * cu, "gradient_mcell_updates_per_s": round(3 * cu / t_grad / 1e6, 1)}
+    if n_fd <= 0:  # the gradient paths only (a profiling form): no fp64 runs, no central differences
+        return out
     t0 = time.perf_counter()
     loss64, ge64, gs64 = misfit_gradient(eps, sig, prob, d_obs, fp64_residual=True)
     torch.cuda.synchronize()
@@ -588,6 +610,7 @@ def adjoint_suite(dev, n_fd: int) -> dict:
     cells = [(int(rng.integers(2, n - 2)), int(rng.integers(2, n - 2))) for _ in range(n_fd)]
     checks = []
     t0 = time.perf_counter()
+    print(f"[bench] adjoint: fp64 runs done; central differences at {n_fd} cells", file=sys.stderr, flush=True)
     for k, (i, j) in enumerate(cells):
         for name, base in (("eps", eps), ("sigma", sig)):
             if name == "sigma" and k >= 8:
@@ -600,19 +623,9 @@ def adjoint_suite(dev, n_fd: int) -> dict:
             lm = fd_loss(m, sig) if name == "eps" else fd_loss(eps, m)
             checks.append((name, i, j, (lp - lm) / (2 * h)))
     t_fd = time.perf_counter() - t0
-    out = {"workload": "256^2 head phantom (clot-free model, clot data), 16 TX x 16 RX, 3000 steps, "
-                       "checkpoint every 100, loss 0.5||d_sim - d_obs||^2",
-           "loss": loss, "forward_s": round(t_fwd, 4), "gradient_s": round(t_grad, 4),
-           "gradient_path": ("full state history (%.1f GB), two adjoint steps per launch" % (prob.history_bytes() / 1e9)
-                             if full else "checkpoints every 100 steps + recompute, two adjoint steps per launch"),
-           "gradient_checkpoint_recompute_s": round(t_grad_ck, 4),
-           "gradient_checkpoint_one_step_per_launch_s": round(t_grad_one, 4),
-           "gradients_equal_bitwise_across_paths": bool(same_ck and same_one),
-           "forward_mcell_updates_per_s": round(cu / t_fwd / 1e6, 1),
-           "gradient_cell_updates": 3 * cu, "gradient_mcell_updates_per_s": round(3 * cu / t_grad / 1e6, 1),
-           "traces_rel_l2_vs_fp64": fp64_err, "residual_rel_l2": resid_rel,
-           "gradient_fp64_residual_s": round(t_grad64, 4), "loss_fp64_residual": loss64,
-           "fp64_forward_s": round(t_f64, 3), "fd_s": round(t_fd, 2)}
+    out.update({"traces_rel_l2_vs_fp64": fp64_err, "residual_rel_l2": resid_rel,
+                "gradient_fp64_residual_s": round(t_grad64, 4), "loss_fp64_residual": loss64,
+                "fp64_forward_s": round(t_f64, 3), "fd_s": round(t_fd, 2)})
     for tag, grads in (("fp32", (ge, gs)), ("fp64_residual", (ge64, gs64))):
         for name, grad in zip(("eps", "sigma"), grads):
             rows = [(fd, float(grad[i, j])) for (nm, i, j, fd) in checks if nm == name]
@@ -823,7 +836,7 @@ def main() -> int:
         serving_res = serving_suite(dev, args.serving_requests)
     if not args.no_adjoint:
         print("[bench] adjoint section", file=sys.stderr, flush=True)
-        adjoint = adjoint_suite(dev, args.fd_cells)
+        adjoint = adjoint_suite(dev, args.fd_cells, args.adjoint_history)
     if not args.no_3d:
         print("[bench] 3D section", file=sys.stderr, flush=True)
         fd3 = fdtd3d_suite(dev, args.size3d, steps=args.steps3d)
