@@ -252,7 +252,7 @@ def slab_suite(dev, slab) -> dict:
     slab.sim.run(slab.n_steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
-    return {"workload": slab.description, "seconds": round(el, 4),
+    return {"workload": slab.description, "seconds": round(el, 5),
             "mcell_updates_per_s": round(slab.n_steps * slab.grid.Nx * slab.grid.Ny / el / 1e6, 1)}
 
 

@@ -162,7 +162,7 @@ int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz,
  * (device, 2 int32 per tile: run index, tile index within the run); a tile is
  * tile_i rows x tile_j columns, numbered row-major over ceil(nx/tile_i) x
  * ceil(ny/tile_j). tile_j = 64 - 2*depth stages one column per lane of each
- * 64-lane row (tile_i 16, 32, 48 or 64); tile_j = 128 - 2*depth stages two
+ * 64-lane row (tile_i 8, 16, 32, 48 or 64); tile_j = 128 - 2*depth stages two
  * (tile_i 16, 32 or 48), which halves the share of recomputed halo columns.
  * Source, Mur and detector handling is exactly the per-step semantics above;
  * step indices are run->step + step_offset + level. A tile region may hold at
@@ -348,9 +348,10 @@ int wf_runs_add_step(wf_run2d *runs, int32_t n_runs, int32_t delta, hipStream_t 
 
 /* Supported tile shapes of wf_tb_args, written into out as (tile_i, cpl)
  * pairs (host, 2*count int32; cpl = columns per lane, so tile_j =
- * 64*cpl - 2*depth); returns count (<= cap). One column per lane: 16 and 32
- * rows run 4-wavefront workgroups, 48 and 64 rows 8-wavefront ones; two
- * columns per lane: 16, 32 and 48 rows, 8-wavefront workgroups. */
+ * 64*cpl - 2*depth); returns count (<= cap). One column per lane: 8, 16,
+ * 32, 48 and 64 rows; two columns per lane: 16, 32 and 48 rows. Every shape
+ * runs 8-wavefront workgroups (the tile's rows plus 2*depth halo rows dealt
+ * round-robin over the wavefronts). */
 int wf_tb_tile_shapes(int32_t *out, int32_t cap);
 
 /* Exhaustive device check that the fast quotient equals IEEE x/d bit for bit

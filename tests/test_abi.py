@@ -42,7 +42,7 @@ def test_header_constants_match_binding():
 
 def test_tile_shapes_listed():
     shapes = _native.tile_shapes()
-    assert (32, 1) in shapes and (48, 2) in shapes and all(r > 0 and c in (1, 2) for r, c in shapes)
+    assert (32, 1) in shapes and (48, 2) in shapes and (8, 1) in shapes and all(r > 0 and c in (1, 2) for r, c in shapes)
     assert _native.tile_rows() == [r for r, c in shapes if c == 1]
     assert [_native.tile_cols(d) for d in (1, 4, 8)] == [62, 56, 48]
     assert [_native.tile_cols(d, 2) for d in (1, 4, 8)] == [126, 120, 112]

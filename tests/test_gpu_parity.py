@@ -82,7 +82,7 @@ def test_per_step_path_bitwise(device, name):
 
 
 @pytest.mark.parametrize("fast_div", [True, False])
-@pytest.mark.parametrize("tile", [32, 16, 48, 64, (16, 2), (32, 2), (48, 2)])
+@pytest.mark.parametrize("tile", [32, 16, 48, 64, 8, (16, 2), (32, 2), (48, 2)])
 @pytest.mark.parametrize("depth", [1, 2, 3, 4, 5, 8])
 @pytest.mark.parametrize("name", ["tiny5x7_mat", "mat_all_sources", "edge_sources", "face_cells", "odd_tiles_mat",
                                   "batch3", "free_point_hz", "extreme_mat", "extreme_free"])

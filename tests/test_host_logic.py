@@ -211,8 +211,9 @@ def test_snapshot_free_fused_step_condition():
 
 
 def test_default_launch_shape_by_workload():
-    """48-row tiles 8 steps deep, 16-row tiles for sets of fewer than 64 such
-    tiles, and smaller tiles where a run's detectors crowd one tile."""
+    """48-row tiles 8 steps deep; for sets of fewer than 64 such tiles, 8-row
+    tiles while there are at most 288 of them, else 16-row tiles; smaller
+    tiles where a run's detectors crowd one tile."""
     from types import SimpleNamespace
 
     import numpy as np
@@ -227,8 +228,10 @@ def test_default_launch_shape_by_workload():
     assert default_launch_shape(runs((4096, 4096))) == (48, 8, 1)   # 86 x 86 tiles
     assert default_launch_shape(runs((1024, 1024))) == (48, 8, 1)   # 22 x 22 tiles
     assert default_launch_shape(runs(*[(256, 256)] * 16)) == (48, 8, 1)   # 36 tiles each
-    assert default_launch_shape(runs((256, 256))) == (16, 8, 1)
-    assert default_launch_shape(runs((128, 128), (200, 120))) == (16, 8, 1)   # 9 + 15 tiles
+    assert default_launch_shape(runs((256, 256))) == (8, 8, 1)   # 32 x 6 = 192 tiles of 8 rows
+    assert default_launch_shape(runs((128, 128), (200, 120))) == (8, 8, 1)   # 48 + 75 such tiles
+    assert default_launch_shape(runs((300, 380))) == (16, 8, 1)   # 38 x 8 = 304 > 288; 7 x 8 = 56 < 64
+    assert default_launch_shape(runs((384, 384))) == (48, 8, 1)   # 8 x 8 = 64 tiles of 48 rows
     # a 9 x 9 block of detectors (81 > 64 per tile) in one 48-row tile, split
     # 4 + 5 rows by the 32-row tile boundary at row 160
     side = int(np.sqrt(TB_MAX_DETS_PER_TILE)) + 1

@@ -14,6 +14,7 @@ the 2D TM stepper records Hz only.
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 
@@ -53,6 +54,7 @@ class DetectorSet:
         c3 = [c if len(c) == 3 else (c[0], c[1], 0) for c in cells]
         flat = [(i * grid.Ny + j) * Nz + k for i, j, k in c3]
         self.cell_index = torch.tensor(flat, dtype=torch.int32, device=grid.device)
+        self._cell_index_host = np.array(flat, dtype=np.int64)  # the same cells, for host-side tables
         self.component = component
         self.component_index = torch.full((len(cells),), COMPONENT_CODES[component], dtype=torch.int32,
                                           device=grid.device)

@@ -57,6 +57,8 @@ DEFAULT_DEPTH = 8
 DEFAULT_CPL = 1
 SMALL_SET_TILE_ROWS = 16
 SMALL_SET_MAX_TILES = 64
+TINY_SET_TILE_ROWS = 8
+TINY_SET_MAX_TILES = 288
 FALLBACK_SHAPES = ((32, 6, 1), (16, 6, 1))
 
 
@@ -78,13 +80,21 @@ def default_launch_shape(runs) -> tuple[int, int, int]:
     from 128^2 to 4096^2, 16 to 64 runs of 128^2 to 512^2) within 10 % of the
     autotuner's best shape and 10-36 % ahead of 32-row tiles 6 deep. Sets of
     fewer than 64 such tiles (one grid up to about 300^2) fill the GPU better
-    with 16-row tiles, also 8 deep (about 10 % ahead there). If a run has more
-    sources or detectors in one tile than the kernel stages, smaller tiles
-    are tried before the caller falls back to the per-step path."""
+    with smaller tiles, also 8 deep: 8-row tiles while there are at most 288
+    of them (about one per CU: one grid up to about 300^2; 5-10 % ahead of
+    16-row tiles for 1000 steps of 128^2 to 300^2), else 16-row tiles (ahead
+    from 384^2). If a run has more sources or detectors in one tile than the
+    kernel stages, smaller tiles are tried before the caller falls back to
+    the per-step path."""
     tj = 64 * DEFAULT_CPL - 2 * DEFAULT_DEPTH
     n = sum(-(-r.nx // DEFAULT_TILE_ROWS) * -(-r.ny // tj) for r in runs)
-    first = ((SMALL_SET_TILE_ROWS, DEFAULT_DEPTH, 1) if n < SMALL_SET_MAX_TILES
-             else (DEFAULT_TILE_ROWS, DEFAULT_DEPTH, DEFAULT_CPL))
+    n_tiny = sum(-(-r.nx // TINY_SET_TILE_ROWS) * -(-r.ny // tj) for r in runs)
+    if n_tiny <= TINY_SET_MAX_TILES:
+        first = (TINY_SET_TILE_ROWS, DEFAULT_DEPTH, 1)
+    elif n < SMALL_SET_MAX_TILES:
+        first = (SMALL_SET_TILE_ROWS, DEFAULT_DEPTH, 1)
+    else:
+        first = (DEFAULT_TILE_ROWS, DEFAULT_DEPTH, DEFAULT_CPL)
     for tile, depth, cpl in (first,) + FALLBACK_SHAPES:
         if all(tb_limits_ok(r, tile, d, cpl) for r in runs for d in range(1, depth + 1)):
             return tile, depth, cpl
@@ -291,6 +301,9 @@ class FDTD2D:
                 rs.det_cell = det.cell_index
                 rs.traces = det.traces if det.traces.dim() == 2 else det.traces[b]
                 rs.trace_n0 = det.start_step
+                host = getattr(det, "_cell_index_host", None)
+                if host is not None and host.size == det.cell_index.numel():
+                    rs.extra["det_cells_np"] = host  # the tile lists need no read-back
             runs.append(rs)
         self._runs = runs
         self._cache_key = key
@@ -337,17 +350,19 @@ class FDTD2D:
                 first += 1
                 n_steps -= 1
             if n_steps > 0:
-                plan = self._tb_plan_for(runs, tile, depth)
+                plan = self._tb_plan_for(runs, tile, depth, set(split_launches(n_steps, depth, odd_ok=True)))
                 plan.advance(n_steps, depth, stream, base_step=first)
+                plan.prepare(depth)  # the other depths, while the GPU runs these launches
         else:
             for r in runs:
                 r.step = self.steps_completed
             per_step_advance(runs, n_steps, stream)
             self._apply_inert(self.steps_completed, n_steps)
 
-    def _tb_plan_for(self, runs, tile: tuple[int, int], depth: int) -> TBPlan:
+    def _tb_plan_for(self, runs, tile: tuple[int, int], depth: int, depths=None) -> TBPlan:
         """The blocked-kernel plan for these runs, with its tile tables and
-        launch arguments built for every depth up to ``depth``. It uses a
+        launch arguments built for ``depths`` (the depths the caller will
+        launch; every depth up to ``depth`` if None), in one upload. It uses a
         second and a third state buffer of the fields' shape, allocated
         together on first use and kept, so launch counts of either parity end
         in the live tensors."""
@@ -368,8 +383,8 @@ class FDTD2D:
                 r.third = None if ex3 is None else (ex3[b], ey3[b], hz3[b])
                 r.step = 0  # launches pass the global step as step_offset
             self._tb_plan = TBPlan(runs, tile[0], dev, fast_div=self._fast_div, cpl=tile[1])
-            self._tb_plan.prepare(depth)
             self._tb_key = key
+        self._tb_plan.prepare(depth, depths)
         return self._tb_plan
 
     def step(self) -> None:
@@ -546,7 +561,9 @@ class FDTD2D:
         if n_exec:
             stream = self._stream()
             dev = self._fields.Hz.device
-            plan = self._tb_plan_for(runs, tile, depth)
+            # the depths this call launches: each segment's launches are as equal as possible
+            used = {q for seg, n_l, _ in segs for q in (seg // n_l, -(-seg // n_l))}
+            plan = self._tb_plan_for(runs, tile, depth, used)
             n_runs = len(runs)
             if self._stop is None or self._stop.numel() != n_runs or self._stop.device != dev:
                 self._stop = torch.zeros(n_runs, dtype=torch.int32, device=dev)
@@ -565,6 +582,9 @@ class FDTD2D:
                 if slot >= 0:
                     judge(view, self._threshold, self._stop, True, stream)
                 offset += d
+            # the other depths' tables, built while the GPU runs the launches above,
+            # so a later call with another step count finds them ready
+            plan.prepare(depth)
             if checks:
                 used = len(checks) * n_runs * 3
                 self._slots[used:used + 1].copy_(self._status)

@@ -29,6 +29,7 @@ import warnings
 from abc import ABC, abstractmethod
 from typing import Union
 
+import numpy as np
 import torch
 
 VALID_COMPONENTS: tuple[str, ...] = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
@@ -379,6 +380,32 @@ class PlaneSource:
     position = property(lambda self: self._position)
 
 
+def remember_host_cells(src, i, j, k, amp) -> None:
+    """Keep host copies of a source's cell indices and amplitudes with the
+    device tensors they equal (``_i_idx``, ``_j_idx``, ``_k_idx``, ``_amp``)
+    and those tensors' version counters: the kernels' source tables are built
+    from these copies, with no read-back, while the tensors are the same
+    objects and have not been written in place since."""
+    ts = (src._i_idx, src._j_idx, src._k_idx, src._amp)
+    src._host_cells = (ts, tuple(t._version for t in ts),
+                       tuple(np.asarray(v) for v in (i, j, k, amp)))
+
+
+def host_cells(src):
+    """``(i, j, k, amp)`` of a source as NumPy arrays: the remembered host
+    copies when still valid, else read from the device tensors (and kept)."""
+    ts = (src._i_idx, src._j_idx, getattr(src, "_k_idx", None), src._amp)
+    rec = getattr(src, "_host_cells", None)
+    if rec is not None and all(a is b for a, b in zip(rec[0], ts)) and \
+            tuple(t._version for t in ts) == rec[1]:
+        return rec[2]
+    i, j, a = ts[0].cpu().numpy(), ts[1].cpu().numpy(), ts[3].detach().cpu().numpy()
+    k = np.zeros_like(i) if ts[2] is None else ts[2].cpu().numpy()
+    if ts[2] is not None:
+        src._host_cells = (ts, tuple(t._version for t in ts), (i, j, k, a))
+    return i, j, k, a
+
+
 class PointSource:
     """Single-cell soft source (``src/core/sources.py:774-887``)."""
 
@@ -399,6 +426,7 @@ class PointSource:
         self._j_idx = torch.tensor([j], dtype=torch.long, device=dev)
         self._k_idx = torch.tensor([k], dtype=torch.long, device=dev)
         self._amp = torch.tensor([1.0], dtype=grid.dtype, device=dev)
+        remember_host_cells(self, np.array([i]), np.array([j]), np.array([k]), np.array([1.0]))
 
     n_cells = property(lambda self: 1)
     waveform_memory_bytes = property(lambda self: self._N_steps * self._waveform_tensor.element_size())
@@ -456,6 +484,15 @@ class LineSource:
             if amp.shape != (n_cells,):
                 raise ValueError(f"amplitude_profile must have length {n_cells}, got shape {tuple(amp.shape)}")
             self._amp = amp
+        span = np.arange(start, stop)
+        fixed = np.full(n_cells, position)
+        hi, hj = (span, fixed) if axis == "x" else (fixed, span)
+        if amplitude_profile is None:
+            remember_host_cells(self, hi, hj, np.zeros(n_cells, dtype=np.int64), np.ones(n_cells))
+        elif not isinstance(amplitude_profile, torch.Tensor) or amplitude_profile.device.type == "cpu":
+            # the profile's values as the amplitude tensor holds them (rounded to the grid dtype)
+            a_host = torch.as_tensor(amplitude_profile, dtype=grid.dtype).numpy()
+            remember_host_cells(self, hi, hj, np.zeros(n_cells, dtype=np.int64), a_host)
 
     n_cells = property(lambda self: self._n_cells)
     waveform_memory_bytes = property(lambda self: self._N_steps * self._waveform_tensor.element_size())

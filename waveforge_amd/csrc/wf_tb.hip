@@ -6,7 +6,7 @@
 namespace {
 
 // (tile_i, columns per lane) of every compiled shape
-constexpr int kTileShapes[][2] = {{32, 1}, {16, 1}, {48, 1}, {64, 1}, {16, 2}, {32, 2}, {48, 2}};
+constexpr int kTileShapes[][2] = {{32, 1}, {16, 1}, {48, 1}, {64, 1}, {16, 2}, {32, 2}, {48, 2}, {8, 1}};
 
 // Exhaustive check of the fast quotient (wf_tb::divide<true>) against IEEE
 // division for every x with lo <= |x bits| < lo + span (both signs). Each
@@ -62,7 +62,7 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: parity must be 0..3 (2 and 3 not with a state history)");
         return WF_ERR_ARG;
     }
-    if (a.tile_i != 16 && a.tile_i != 32 && a.tile_i != 48 && (a.tile_i != 64 || cpl != 1)) {
+    if (a.tile_i != 16 && a.tile_i != 32 && a.tile_i != 48 && ((a.tile_i != 64 && a.tile_i != 8) || cpl != 1)) {
         wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d with %d column(s) per lane", a.tile_i, cpl);
         return WF_ERR_UNSUPPORTED;
     }

@@ -685,6 +685,9 @@ namespace wf_tb {
 template <bool MAT, bool FAST, int CPL>
 hipError_t launch_tile(const wf_tb_args &a, hipStream_t st) {
     switch (a.tile_i) {
+        case 8:
+            if constexpr (CPL == 1) return launch_depth<8, MAT, FAST, false, CPL>(a, st);
+            return hipErrorInvalidValue;
         case 16: return launch_depth<16, MAT, FAST, false, CPL>(a, st);
         case 32: return launch_depth<32, MAT, FAST, false, CPL>(a, st);
         case 48: return launch_depth<48, MAT, FAST, false, CPL>(a, st);

@@ -61,6 +61,35 @@ def to_device_async(a, device, dtype=None) -> torch.Tensor:
     return pinned.to(device, non_blocking=True)
 
 
+_TORCH_OF = {np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64, np.dtype(np.float32): torch.float32,
+             np.dtype(np.float64): torch.float64, np.dtype(np.uint8): torch.uint8}
+
+
+def upload_packed(arrays: Sequence[np.ndarray], device) -> list[torch.Tensor]:
+    """Several host arrays in one asynchronous copy (``to_device_async`` of
+    their bytes, each at a 16-byte boundary) and a device view of each, of its
+    shape and dtype (structured records come back as bytes). One copy instead
+    of one per array: each copy costs a queue round trip before the kernels
+    that need it can start."""
+    parts, spans, off = [], [], 0
+    for x in arrays:
+        x = np.ascontiguousarray(x)
+        b = x.view(np.uint8).reshape(-1)
+        spans.append((off, b.size, x))
+        parts.append(b)
+        pad = (-b.size) % 16
+        if pad:
+            parts.append(np.zeros(pad, dtype=np.uint8))
+        off += b.size + pad
+    buf = to_device_async(np.concatenate(parts) if parts else np.zeros(16, np.uint8), device)
+    out = []
+    for o, n, x in spans:
+        v = buf[o:o + n]
+        td = _TORCH_OF.get(x.dtype)
+        out.append(v if td is None or td == torch.uint8 else v.view(td).view(x.shape))
+    return out
+
+
 def f32(x: float) -> float:
     """Round a Python float to fp32, as torch does for a scalar operand."""
     return float(np.float32(x))
@@ -142,9 +171,9 @@ class SourceTable:
             if isinstance(src, PlaneSource):
                 i, j, k, a = src.cells3d(Nx, Ny, Nz)
             else:
-                i, j, a = src._i_idx.cpu(), src._j_idx.cpu(), src._amp.detach().float().cpu()
-                kk = getattr(src, "_k_idx", None)
-                k = torch.zeros_like(i) if kk is None else kk.cpu()
+                from .core.sources import host_cells
+
+                i, j, k, a = (torch.as_tensor(np.asarray(v)) for v in host_cells(src))
             n = i.numel()
             comps.append(np.full(n, cmap[comp], dtype=np.int32))
             flat = (i.numpy().astype(np.int64) * Ny + j.numpy().astype(np.int64)) * Nz + k.numpy().astype(np.int64)
@@ -172,8 +201,7 @@ class SourceTable:
             self.group_cells = rec["cell"][starts].copy()
             self.host_inj, self.host_grp = rec, grp
             if upload:
-                self.inj = to_device_async(rec.view(np.uint8), device)
-                self.grp_start = to_device_async(grp, device)
+                self.inj, self.grp_start = upload_packed([rec.view(np.uint8), grp], device)
             else:
                 self.inj = self.grp_start = None
         else:
@@ -191,8 +219,8 @@ class SourceTable:
         live = [t for t in tables if t.n_grp]
         if not live:
             return
-        inj = to_device_async(np.concatenate([t.host_inj.view(np.uint8) for t in live]), device)
-        grp = to_device_async(np.concatenate([t.host_grp for t in live]), device)
+        inj, grp = upload_packed([np.concatenate([t.host_inj.view(np.uint8) for t in live]),
+                                  np.concatenate([t.host_grp for t in live])], device)
         o_i = o_g = 0
         for t in live:
             n_b, n_g = t.host_inj.nbytes, t.host_grp.size
@@ -465,8 +493,8 @@ class TBPlan:
             raise ValueError("temporally blocked launch needs all runs with or all without materials")
         self.has_materials = int(mats.pop())
         arr = (WfRun2d * len(runs))(*[r.struct() for r in runs])
-        raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
-        self.run_table = to_device_async(raw, device)
+        self._run_raw = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
+        self._run_table: Optional[torch.Tensor] = None  # uploaded with the first tables (prepare) or on use
         self._tiles: dict[int, tuple[torch.Tensor, int]] = {}
         self._lists: dict = {}
         self.div = None
@@ -480,23 +508,50 @@ class TBPlan:
                     self.div = (vx, vy)
         self.n_cells = sum(r.nx * r.ny for r in runs)
 
+    @property
+    def run_table(self) -> torch.Tensor:
+        """The runs' descriptors on the device (``wf_run2d`` array)."""
+        if self._run_table is None:
+            self._run_table = to_device_async(self._run_raw, self.device)
+        return self._run_table
+
     def prepare(self, max_depth: int, depths: Optional[Sequence[int]] = None) -> None:
         """Upload the tile tables and lists and build the launch arguments of
         ``depths`` (every depth up to ``max_depth`` by default) and every
-        state-buffer code now, so no later launch waits on a table upload."""
-        for d in (depths if depths is not None else range(1, max_depth + 1)):
+        state-buffer code now, so no later launch waits on a table upload.
+        The tables and lists of all those depths go up in one copy (one
+        int32 buffer, every array at a 16-byte boundary) instead of four
+        small copies per depth, each of which costs a queue round trip."""
+        ds = list(depths if depths is not None else range(1, max_depth + 1))
+        todo = [d for d in ds if d not in self._tiles or d not in self._lists]
+        if todo:
+            host = [a for d in todo for a in ((self._tiles_host(d),) + self._lists_host(d))]
+            with_runs = self._run_table is None
+            views = upload_packed(([self._run_raw] if with_runs else []) + host, self.device)
+            if with_runs:
+                self._run_table = views.pop(0)
+            for k, d in enumerate(todo):
+                t, lists, grp, det = views[4 * k:4 * k + 4]
+                self._tiles[d] = (t, int(t.shape[0]))
+                self._lists[d] = (lists, grp, det)
+        for d in ds:
             for par in ((0, 1, 2, 3) if self.odd_ok else (0, 1)):
                 self._base_args(d, par)
 
+    def _tiles_host(self, depth: int) -> np.ndarray:
+        """The (run, tile) pairs of a launch of ``depth`` steps, on the host."""
+        nts = np.array([tile_count(r.nx, r.ny, self.tile_i, depth, self.cpl) for r in self.runs], dtype=np.int64)
+        total = int(nts.sum())
+        starts = np.cumsum(nts) - nts
+        t = np.empty((total, 2), dtype=np.int32)
+        t[:, 0] = np.repeat(np.arange(len(self.runs)), nts)
+        t[:, 1] = np.arange(total) - np.repeat(starts, nts)
+        return t
+
     def tiles(self, depth: int) -> tuple[torch.Tensor, int]:
         if depth not in self._tiles:
-            nts = np.array([tile_count(r.nx, r.ny, self.tile_i, depth, self.cpl) for r in self.runs], dtype=np.int64)
-            total = int(nts.sum())
-            starts = np.cumsum(nts) - nts
-            t = np.empty((total, 2), dtype=np.int32)
-            t[:, 0] = np.repeat(np.arange(len(self.runs)), nts)
-            t[:, 1] = np.arange(total) - np.repeat(starts, nts)
-            self._tiles[depth] = (to_device_async(t, self.device), total)
+            t = self._tiles_host(depth)
+            self._tiles[depth] = (to_device_async(t, self.device), int(t.shape[0]))
         return self._tiles[depth]
 
     def _run_lists(self, r: RunState, depth: int):
@@ -552,18 +607,22 @@ class TBPlan:
         cells around it) and the detectors in its core, so a tile reads its
         own few entries instead of testing every group and detector."""
         if depth not in self._lists:
-            parts = [self._run_lists(r, depth) for r in self.runs]
-            g_n = np.array([p[1].size for p in parts], dtype=np.int64)
-            d_n = np.array([p[2].size for p in parts], dtype=np.int64)
-            t_n = np.array([p[0].shape[0] for p in parts], dtype=np.int64)
-            off = np.stack([np.cumsum(g_n) - g_n, np.cumsum(g_n) - g_n, np.cumsum(d_n) - d_n,
-                            np.cumsum(d_n) - d_n], axis=1)
-            rows = np.concatenate([p[0] for p in parts]) + np.repeat(off, t_n, axis=0)
-            lists = np.ascontiguousarray(rows.astype(np.int32))
-            grp = np.concatenate([p[1] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
-            det = np.concatenate([p[2] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
-            self._lists[depth] = tuple(to_device_async(x, self.device) for x in (lists, grp, det))
+            self._lists[depth] = tuple(to_device_async(x, self.device) for x in self._lists_host(depth))
         return self._lists[depth]
+
+    def _lists_host(self, depth: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """``tile_lists``' three arrays on the host (int32)."""
+        parts = [self._run_lists(r, depth) for r in self.runs]
+        g_n = np.array([p[1].size for p in parts], dtype=np.int64)
+        d_n = np.array([p[2].size for p in parts], dtype=np.int64)
+        t_n = np.array([p[0].shape[0] for p in parts], dtype=np.int64)
+        off = np.stack([np.cumsum(g_n) - g_n, np.cumsum(g_n) - g_n, np.cumsum(d_n) - d_n,
+                        np.cumsum(d_n) - d_n], axis=1)
+        rows = np.concatenate([p[0] for p in parts]) + np.repeat(off, t_n, axis=0)
+        lists = np.ascontiguousarray(rows.astype(np.int32))
+        grp = np.concatenate([p[1] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+        det = np.concatenate([p[2] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+        return lists, grp, det
 
     def _base_args(self, depth: int, parity: int) -> WfTbArgs:
         """The launch arguments that do not change from launch to launch."""
