@@ -29,7 +29,7 @@ from typing import Optional, Sequence
 import torch
 
 from . import _native
-from .engine import RunState, TBPlan, tb_limits_ok
+from .engine import RunState, TBPlan, tb_fits_any_tile, tb_limits_ok
 
 DEPTHS = tuple(range(1, _native.TB_MAX_DEPTH + 1))
 
@@ -57,9 +57,10 @@ _CACHE: dict = {}
 def _admissible_depth(runs: Sequence[RunState], ti: int, cpl: int, max_depth: int) -> int:
     """Largest depth d <= max_depth such that the runs meet the per-tile
     source and detector limits at every depth up to d (0 if none)."""
+    hard = [r for r in runs if not tb_fits_any_tile(r)]  # the others fit every shape
     ok = 0
     for d in range(1, max_depth + 1):
-        if not all(tb_limits_ok(r, ti, d, cpl) for r in runs):
+        if not all(tb_limits_ok(r, ti, d, cpl) for r in hard):
             break
         ok = d
     return ok

@@ -347,14 +347,21 @@ def tile_count(nx: int, ny: int, tile_i: int, depth: int, cpl: int = 1) -> int:
     return ((nx + tile_i - 1) // tile_i) * ((ny + tj - 1) // tj)
 
 
+def tb_fits_any_tile(run: RunState) -> bool:
+    """True if ``run`` meets the blocked kernel's per-tile limits at every tile
+    shape and depth: it has no more source groups and detectors in all than
+    one tile may hold (and no sources the kernel does not apply)."""
+    n_grp = run.sources.n_grp if run.sources is not None else 0
+    n_det = run.det_cell.numel() if run.det_cell is not None else 0
+    return n_grp <= TB_MAX_GROUPS_PER_TILE and n_det <= TB_MAX_DETS_PER_TILE and \
+        not (run.sources is not None and run.sources.inert)
+
+
 def tb_limits_ok(run: RunState, tile_i: int, depth: int, cpl: int = 1) -> bool:
     """True if every tile region of ``run`` holds at most the per-tile source
     groups and core detectors the temporally blocked kernel can stage."""
     TI, TJ = tile_i, _native.tile_cols(depth, cpl)
-    n_grp = run.sources.n_grp if run.sources is not None else 0
-    n_det = run.det_cell.numel() if run.det_cell is not None else 0
-    if n_grp <= TB_MAX_GROUPS_PER_TILE and n_det <= TB_MAX_DETS_PER_TILE and \
-            not (run.sources is not None and run.sources.inert):
+    if tb_fits_any_tile(run):
         return True  # no tile can hold more than the whole run has
     nti = (run.nx + TI - 1) // TI
     ntj = (run.ny + TJ - 1) // TJ
@@ -554,51 +561,14 @@ class TBPlan:
             self._tiles[depth] = (to_device_async(t, self.device), int(t.shape[0]))
         return self._tiles[depth]
 
-    def _run_lists(self, r: RunState, depth: int):
-        """One run's per-tile lists for this tile shape and depth: ``(rows, grp,
-        det)`` with ``rows`` (n_tiles, 4) offsets into the run's own ``grp`` and
-        ``det`` index arrays. Cached on the run, so plans rebuilt over changing
-        run sets (the serving loop) only concatenate."""
-        key = (self.tile_i, self.cpl, depth)
-        cache = r.extra.setdefault("tile_lists", {})
-        if key in cache:
-            return cache[key]
-        TI, TJ = self.tile_i, _native.tile_cols(depth, self.cpl)
-        nti, ntj = -(-r.nx // TI), -(-r.ny // TJ)
-        nt = nti * ntj
-        # source groups: every tile whose region holds the group's cell
-        if r.sources is not None and r.sources.n_grp:
-            cells = r.sources.group_cells.astype(np.int64)
-            gi, gj = cells // r.ny, cells % r.ny
-            lo_i = np.maximum(0, -(-(gi - TI - depth + 1) // TI))
-            hi_i = np.minimum(nti - 1, (gi + depth) // TI)
-            lo_j = np.maximum(0, -(-(gj - TJ - depth + 1) // TJ))
-            hi_j = np.minimum(ntj - 1, (gj + depth) // TJ)
-            nj = hi_j - lo_j + 1
-            cnt = (hi_i - lo_i + 1) * nj
-            g_idx = np.repeat(np.arange(cells.size), cnt)
-            within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
-            tile = (np.repeat(lo_i, cnt) + within // np.repeat(nj, cnt)) * ntj + \
-                np.repeat(lo_j, cnt) + within % np.repeat(nj, cnt)
-        else:
-            g_idx = tile = np.zeros(0, dtype=np.int64)
-        g_cnt = np.bincount(tile, minlength=nt)
-        grp = g_idx[np.argsort(tile, kind="stable")]
-        # detectors: the tile whose core holds the cell
-        if r.det_cell is not None and r.det_cell.numel():
-            dc = r.extra.get("det_cells_np")
-            if dc is None:
-                dc = r.det_cell.cpu().numpy().astype(np.int64)
-                r.extra["det_cells_np"] = dc
-            dtile = (dc // r.ny // TI) * ntj + (dc % r.ny) // TJ
-        else:
-            dtile = np.zeros(0, dtype=np.int64)
-        d_cnt = np.bincount(dtile, minlength=nt)
-        det = np.argsort(dtile, kind="stable")
-        g_end, d_end = np.cumsum(g_cnt), np.cumsum(d_cnt)
-        rows = np.stack([g_end - g_cnt, g_end, d_end - d_cnt, d_end], axis=1)
-        cache[key] = (rows, grp, det)
-        return cache[key]
+    @staticmethod
+    def _det_cells_host(r: RunState) -> np.ndarray:
+        """A run's detector cells on the host (kept with the run; read back once if absent)."""
+        dc = r.extra.get("det_cells_np")
+        if dc is None:
+            dc = r.det_cell.cpu().numpy().astype(np.int64)
+            r.extra["det_cells_np"] = dc
+        return dc
 
     def tile_lists(self, depth: int) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
         """Per-tile source-group and detector lists of a launch of ``depth``
@@ -611,17 +581,64 @@ class TBPlan:
         return self._lists[depth]
 
     def _lists_host(self, depth: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
-        """``tile_lists``' three arrays on the host (int32)."""
-        parts = [self._run_lists(r, depth) for r in self.runs]
-        g_n = np.array([p[1].size for p in parts], dtype=np.int64)
-        d_n = np.array([p[2].size for p in parts], dtype=np.int64)
-        t_n = np.array([p[0].shape[0] for p in parts], dtype=np.int64)
-        off = np.stack([np.cumsum(g_n) - g_n, np.cumsum(g_n) - g_n, np.cumsum(d_n) - d_n,
-                        np.cumsum(d_n) - d_n], axis=1)
-        rows = np.concatenate([p[0] for p in parts]) + np.repeat(off, t_n, axis=0)
-        lists = np.ascontiguousarray(rows.astype(np.int32))
-        grp = np.concatenate([p[1] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
-        det = np.concatenate([p[2] for p in parts] + [np.zeros(1, dtype=np.int64)]).astype(np.int32)
+        """``tile_lists``' three arrays on the host (int32), for every run of
+        the plan in one vectorised pass: a source group enters every tile
+        whose region (core plus ``depth`` cells around it) holds its cell, a
+        detector the tile whose core holds it; within a tile both keep their
+        run's order. Tiles are numbered as in ``tiles`` (run after run, each
+        run's tiles row-major), and list entries are indices into the run's
+        own group and detector arrays."""
+        TI, TJ = self.tile_i, _native.tile_cols(depth, self.cpl)
+        runs = self.runs
+        nx = np.array([r.nx for r in runs], dtype=np.int64)
+        ny = np.array([r.ny for r in runs], dtype=np.int64)
+        nti, ntj = -(-nx // TI), -(-ny // TJ)
+        nt = nti * ntj
+        toff = np.cumsum(nt) - nt
+        total = int(nt.sum())
+        # source groups: (run, cell, index in the run) of every group
+        g_parts = [(k, r.sources.group_cells) for k, r in enumerate(runs)
+                   if r.sources is not None and r.sources.n_grp]
+        if g_parts:
+            g_n = np.array([c.size for _, c in g_parts], dtype=np.int64)
+            g_run = np.repeat(np.array([k for k, _ in g_parts], dtype=np.int64), g_n)
+            cells = np.concatenate([c for _, c in g_parts]).astype(np.int64)
+            g_loc = np.arange(cells.size) - np.repeat(np.cumsum(g_n) - g_n, g_n)
+            gi, gj = cells // ny[g_run], cells % ny[g_run]
+            lo_i = np.maximum(0, -(-(gi - TI - depth + 1) // TI))
+            hi_i = np.minimum(nti[g_run] - 1, (gi + depth) // TI)
+            lo_j = np.maximum(0, -(-(gj - TJ - depth + 1) // TJ))
+            hi_j = np.minimum(ntj[g_run] - 1, (gj + depth) // TJ)
+            nj = hi_j - lo_j + 1
+            cnt = (hi_i - lo_i + 1) * nj
+            within = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+            tile = (np.repeat(toff[g_run], cnt)
+                    + (np.repeat(lo_i, cnt) + within // np.repeat(nj, cnt)) * np.repeat(ntj[g_run], cnt)
+                    + np.repeat(lo_j, cnt) + within % np.repeat(nj, cnt))
+            order = np.argsort(tile, kind="stable")
+            grp = np.repeat(g_loc, cnt)[order]
+            g_cnt = np.bincount(tile, minlength=total)
+        else:
+            grp = np.zeros(0, dtype=np.int64)
+            g_cnt = np.zeros(total, dtype=np.int64)
+        # detectors: the tile whose core holds the cell
+        d_parts = [(k, self._det_cells_host(r)) for k, r in enumerate(runs)
+                   if r.det_cell is not None and r.det_cell.numel()]
+        if d_parts:
+            d_n = np.array([c.size for _, c in d_parts], dtype=np.int64)
+            d_run = np.repeat(np.array([k for k, _ in d_parts], dtype=np.int64), d_n)
+            dc = np.concatenate([c for _, c in d_parts]).astype(np.int64)
+            d_loc = np.arange(dc.size) - np.repeat(np.cumsum(d_n) - d_n, d_n)
+            dtile = toff[d_run] + (dc // ny[d_run] // TI) * ntj[d_run] + (dc % ny[d_run]) // TJ
+            det = d_loc[np.argsort(dtile, kind="stable")]
+            d_cnt = np.bincount(dtile, minlength=total)
+        else:
+            det = np.zeros(0, dtype=np.int64)
+            d_cnt = np.zeros(total, dtype=np.int64)
+        g_end, d_end = np.cumsum(g_cnt), np.cumsum(d_cnt)
+        lists = np.ascontiguousarray(np.stack([g_end - g_cnt, g_end, d_end - d_cnt, d_end], axis=1).astype(np.int32))
+        grp = np.concatenate([grp, np.zeros(1, dtype=np.int64)]).astype(np.int32)
+        det = np.concatenate([det, np.zeros(1, dtype=np.int64)]).astype(np.int32)
         return lists, grp, det
 
     def _base_args(self, depth: int, parity: int) -> WfTbArgs:

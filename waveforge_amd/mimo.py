@@ -40,6 +40,7 @@ from .engine import (
     judge,
     runs_add_step,
     schedule_launches,
+    tb_fits_any_tile,
     tb_limits_ok,
 )
 
@@ -137,6 +138,8 @@ class BatchedRuns:
             tile_rows, depth, cpl = self.tuned.tile_rows, self.tuned.depth, self.tuned.cpl
         self.tile_rows, self.depth, self.cpl = tile_rows, depth, cpl
         for k, rs in enumerate(self.runs):
+            if tb_fits_any_tile(rs):
+                continue
             for d in range(1, depth + 1):
                 if not tb_limits_ok(rs, tile_rows, d, cpl):
                     raise ValueError(f"run {k}: too many sources or receivers per tile for the batched kernel")
@@ -203,7 +206,8 @@ class BatchedRuns:
         launches, _ = schedule_launches(0, chunk, self.depth, None)
         if launches is None:
             raise ValueError("graph chunk needs an even launch split")
-        self.plan.prepare(max(d for d, _ in launches))  # tile tables and lists uploaded before capture
+        # the tile tables and lists of the chunk's depths, uploaded before capture
+        self.plan.prepare(max(d for d, _ in launches), {d for d, _ in launches})
         self._graph_slot = torch.zeros((len(self.runs), 3), dtype=torch.int32, device=self.device)
         s = torch.cuda.Stream(device=self.device)
         s.wait_stream(torch.cuda.current_stream(self.device))

@@ -41,6 +41,7 @@ from .engine import (
     TBPlan,
     judge,
     runs_add_step,
+    tb_fits_any_tile,
     tb_limits_ok,
     to_device_async,
 )
@@ -140,7 +141,8 @@ class ImagingServer:
             req.n_steps, self.dt, torch.device("cpu"), torch.float32).numpy(), dev)
         if any(not (0 <= i < nx and 0 <= j < ny) for i, j in req.antennas):
             raise ValueError(f"request {req.rid}: antenna cell outside the {nx}x{ny} grid")
-        cells = to_device_async([i * ny + j for i, j in req.antennas], dev, torch.int32)
+        cells_host = np.array([i * ny + j for i, j in req.antennas], dtype=np.int64)
+        cells = to_device_async(cells_host.astype(np.int32), dev)
         n_ant = len(req.antennas)
         req.traces = torch.zeros((n_ant, n_ant, req.n_steps), dtype=torch.float32, device=dev)
         runs, tables = [], []
@@ -153,9 +155,10 @@ class ImagingServer:
                           traces=req.traces[t], trace_n0=0, step=0, step_end=req.n_steps)
             rs.second = (f[3], f[4], f[5])
             rs.extra["req"] = req
+            rs.extra["det_cells_np"] = cells_host  # the receivers' cells for host-side tables (no read-back)
             rs.extra["slot"] = -1  # index in the current plan's stop flags (-1: not yet in a plan)
             for d in range(1, self.depth + 1):
-                if not tb_limits_ok(rs, self.tile_rows, d):
+                if not tb_fits_any_tile(rs) and not tb_limits_ok(rs, self.tile_rows, d):
                     raise ValueError(f"request {req.rid}: receivers too dense for the batched kernel")
             runs.append(rs)
         SourceTable.upload_many(tables, dev)
