@@ -424,7 +424,7 @@ def serving_suite(dev, n_requests: int) -> dict:
         return r
 
     def server(max_requests=None):
-        return serving.ImagingServer(dt=dt, device=dev, hbm_budget=128e9, quantum=96, max_requests=max_requests)
+        return serving.ImagingServer(dt=dt, device=dev, hbm_budget=128e9, quantum=192, max_requests=max_requests)
 
     def log(msg):
         print(f"[bench] serving: {msg}", file=sys.stderr, flush=True)
@@ -658,7 +658,7 @@ def serving_full() -> int:
         print(f"[serving-full] {msg}", file=sys.stderr, flush=True)
 
     def server():
-        return serving.ImagingServer(dt=dt, device=dev, hbm_budget=192e9, quantum=96)
+        return serving.ImagingServer(dt=dt, device=dev, hbm_budget=192e9, quantum=192)
 
     warm = serving.poisson_requests(2, 100.0, n_min=64, n_max=96, seed=1)
     server().serve(warm, realtime=False)
