@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define WF_ABI_VERSION 2
+#define WF_ABI_VERSION 3
 
 #define WF_OK 0
 #define WF_ERR_ARG (-1)
@@ -73,6 +73,15 @@ typedef struct wf_inj2d {
     float amp;
 } wf_inj2d;
 
+/* How a run divides by its cell sizes (wf_run2d.flags). The reference
+ * divides float32 field differences by Python floats dx, dy
+ * (src/core/fdtd2d.py:232-235, 250-251), and torch forms that quotient two
+ * ways: on the CPU as IEEE x / fp32(d); on a GPU as x * fp32(1/d), the
+ * reciprocal of the double d rounded once to fp32, then one fp32 product.
+ * Without WF_RUN2D_RECIPROCAL the kernels reproduce the first, with it the
+ * second (taking rdx, rdy from the descriptor). */
+#define WF_RUN2D_RECIPROCAL 1
+
 /* One simulation (one TX run / one batch member). The descriptor is read by
  * every entry point below; `step` is the global index of the next step. */
 typedef struct wf_run2d {
@@ -92,11 +101,13 @@ typedef struct wf_run2d {
     float dh, de, dx, dy;         /* fp32(dt/MU0), fp32(dt/EPS0), fp32(dx), fp32(dy) */
     float cmx, cmy;               /* fp32 Mur coefficients (src/core/boundaries.py:177-178) */
     int32_t step;
-    int32_t flags;                /* reserved, 0 */
+    int32_t flags;                /* WF_RUN2D_* bits, else 0 */
     uint32_t *status;             /* NULL or device word: WF_STATUS_* bits (see above) */
     int32_t step_end;             /* 0, or the step index the run ends at: wf_fdtd2d_tb_advance takes no
                                      step n >= step_end (a launch that reaches it stops the run there, a
                                      launch that starts past it copies the run through) */
+    float rdx, rdy;               /* fp32(1/dx), fp32(1/dy) of the double cell sizes: the multipliers
+                                     of WF_RUN2D_RECIPROCAL */
     int32_t pad;
 } wf_run2d;
 
@@ -170,13 +181,20 @@ int wf_fdtd2d_maxabs(const float *ex, const float *ey, const float *hz,
  * WF_STATUS_TILE_GROUPS / WF_STATUS_TILE_DETS in run->status and are not
  * applied); a step past a waveform table's end sets WF_STATUS_WAVE_END. If maxabs_out is non-NULL, the launch also folds
  * max|E|,|H| of its final level into maxabs_out[3*run + c] (as uint bit
- * patterns; the caller zeroes it first). fast_div selects the fast quotient
- * of wf_div_verify for every |x| < bits(div_span) (IEEE division above it and
- * for NaN); callers enable it only for divisors that wf_div_verify proved
- * exact on [0, div_span), and div_lo must then be 0.
+ * patterns; the caller zeroes it first). fast_div selects how the launch
+ * divides by dx, dy: WF_DIV_IEEE (IEEE x / fp32(d)); WF_DIV_FAST, the fast
+ * quotient of wf_div_verify for every |x| < bits(div_span) (IEEE division
+ * above it and for NaN), equal to IEEE division: callers enable it only for
+ * divisors that wf_div_verify proved exact on [0, div_span), and div_lo must
+ * then be 0; WF_DIV_RECIPROCAL, the product by each run's rdx, rdy (the
+ * quotient of runs flagged WF_RUN2D_RECIPROCAL, so that the blocked path
+ * and wf_fdtd2d_step give the same bits).
  * Replaces FDTD2D.run's Python loop over step() (src/core/fdtd2d.py:328-329)
  * and run_mimo's per-TX loop (examples/brain_mimo_imaging.py:176-201). */
 #define WF_TB_MAX_DEPTH 8
+#define WF_DIV_IEEE 0
+#define WF_DIV_FAST 1
+#define WF_DIV_RECIPROCAL 2
 typedef struct wf_tb_args {
     const wf_run2d *runs;      /* device array of n_runs descriptors */
     const int32_t *tiles;      /* device, 2*n_tiles */
@@ -187,10 +205,10 @@ typedef struct wf_tb_args {
     int32_t step_offset;
     int32_t has_materials;     /* 1 if every run has ca/cb, 0 if none has */
     uint32_t *maxabs_out;      /* NULL or device 3*n_runs words */
-    float rdx, rdy;            /* fp32 reciprocals 1/dx, 1/dy (correctly rounded) */
+    float rdx, rdy;            /* WF_DIV_FAST: fp32 reciprocals 1/fp32(dx), 1/fp32(dy) (correctly rounded) */
     uint32_t div_lo_x, div_span_x;
     uint32_t div_lo_y, div_span_y;
-    int32_t fast_div;          /* 1: corrected reciprocal division (all runs share dx, dy) */
+    int32_t fast_div;          /* WF_DIV_*: WF_DIV_FAST needs all runs to share dx, dy */
     int32_t pad;
     const int32_t *stop;       /* NULL or device n_runs flags: a run whose flag is set is
                                   copied through unchanged (it failed a stability check) */

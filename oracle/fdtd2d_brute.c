@@ -14,6 +14,12 @@
  *   Ampere, free / material        src/core/fdtd2d.py:244-257
  *
  * Layout: row-major (i, j), index i*Ny + j, one grid (no batch).
+ *
+ * Division: the plain entry points divide by dx, dy (IEEE x / fp32(d), what
+ * torch computes for the reference's `x / d` on the CPU). The *_rcp entry
+ * points take the reciprocals fp32(1/dx), fp32(1/dy) in the dx, dy arguments
+ * and multiply (x * fp32(1/d), what torch computes for the same expression
+ * on a GPU).
  */
 /* built with -fopenmp: wf_brute_run_f32 runs each phase's rows in parallel */
 #include <stdlib.h>
@@ -21,7 +27,7 @@
 
 #define IDX(i, j) ((size_t)(i) * (size_t)Ny + (size_t)(j))
 
-#define DEFINE_STEP(NAME, T)                                                        \
+#define DEFINE_STEP(NAME, T, QOP)                                                   \
 void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,           \
           T Dh, T De, T dx, T dy, T cx, T cy,                                       \
           int n_inj, const int *inj_comp, const int *inj_cell, const T *inj_val)   \
@@ -37,8 +43,8 @@ void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,        
     memcpy(Hn, Hz, n * sizeof(T));                                                  \
     for (int i = 0; i < Nx - 1; ++i)              /* 3. Faraday */                  \
         for (int j = 0; j < Ny - 1; ++j) {                                          \
-            T a = (Ex[IDX(i, j + 1)] - Ex[IDX(i, j)]) / dy;                         \
-            T b = (Ey[IDX(i + 1, j)] - Ey[IDX(i, j)]) / dx;                         \
+            T a = (Ex[IDX(i, j + 1)] - Ex[IDX(i, j)]) QOP dy;                       \
+            T b = (Ey[IDX(i + 1, j)] - Ey[IDX(i, j)]) QOP dx;                       \
             T c = a - b;                                                            \
             T d = Dh * c;                                                           \
             Hn[IDX(i, j)] = Hz[IDX(i, j)] + d;                                      \
@@ -62,7 +68,7 @@ void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,        
     /* 5. Ampere from the post-Mur Hz */                                           \
     for (int i = 0; i < Nx; ++i)                                                    \
         for (int j = 1; j < Ny; ++j) {                                              \
-            T g = (Hz[IDX(i, j)] - Hz[IDX(i, j - 1)]) / dy;                         \
+            T g = (Hz[IDX(i, j)] - Hz[IDX(i, j - 1)]) QOP dy;                       \
             if (Ca) {                                                               \
                 T t1 = Ca[IDX(i, j)] * Ex[IDX(i, j)];                               \
                 T t2 = Cb[IDX(i, j)] * g;                                           \
@@ -74,7 +80,7 @@ void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,        
         }                                                                           \
     for (int i = 1; i < Nx; ++i)                                                    \
         for (int j = 0; j < Ny; ++j) {                                              \
-            T g = (Hz[IDX(i, j)] - Hz[IDX(i - 1, j)]) / dx;                         \
+            T g = (Hz[IDX(i, j)] - Hz[IDX(i - 1, j)]) QOP dx;                       \
             if (Ca) {                                                               \
                 T t1 = Ca[IDX(i, j)] * Ey[IDX(i, j)];                               \
                 T t2 = Cb[IDX(i, j)] * g;                                           \
@@ -88,8 +94,9 @@ void NAME(int Nx, int Ny, T *Ex, T *Ey, T *Hz, const T *Ca, const T *Cb,        
     free(Hn);                                                                       \
 }
 
-DEFINE_STEP(wf_brute_step_f32, float)
-DEFINE_STEP(wf_brute_step_f64, double)
+DEFINE_STEP(wf_brute_step_f32, float, /)
+DEFINE_STEP(wf_brute_step_f64, double, /)
+DEFINE_STEP(wf_brute_step_f32_rcp, float, *)
 
 /* n_steps of the same step on one grid, for long runs at full size: the
  * snapshot and Faraday buffers are allocated once and every phase runs its
@@ -100,10 +107,10 @@ DEFINE_STEP(wf_brute_step_f64, double)
  * src_vals[s*n_steps + n] to its cell at step n (the caller forms
  * w[n]*amp as the reference does). Detectors: traces[d*n_steps + n] = Hz at
  * det_cell[d] after step n. */
-void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const float *Ca, const float *Cb,
-                      float Dh, float De, float dx, float dy, float cx, float cy, int n_steps,
-                      int n_src, const int *src_comp, const int *src_cell, const float *src_vals,
-                      int n_det, const int *det_cell, float *traces)
+static void run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const float *Ca, const float *Cb,
+                    float Dh, float De, float dx, float dy, float cx, float cy, int n_steps,
+                    int n_src, const int *src_comp, const int *src_cell, const float *src_vals,
+                    int n_det, const int *det_cell, float *traces, int rcp)
 {
     size_t n = (size_t)Nx * (size_t)Ny;
     float *P = (float *)malloc(n * sizeof(float));
@@ -120,8 +127,10 @@ void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const flo
         #pragma omp parallel for schedule(static)
         for (int i = 0; i < Nx - 1; ++i)                                     /* 3. Faraday */
             for (int j = 0; j < Ny - 1; ++j) {
-                float a = (Ex[IDX(i, j + 1)] - Ex[IDX(i, j)]) / dy;
-                float b = (Ey[IDX(i + 1, j)] - Ey[IDX(i, j)]) / dx;
+                float xa = Ex[IDX(i, j + 1)] - Ex[IDX(i, j)];
+                float xb = Ey[IDX(i + 1, j)] - Ey[IDX(i, j)];
+                float a = rcp ? xa * dy : xa / dy;
+                float b = rcp ? xb * dx : xb / dx;
                 float c = a - b;
                 float d = Dh * c;
                 Hn[IDX(i, j)] = Hz[IDX(i, j)] + d;
@@ -139,7 +148,8 @@ void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const flo
         #pragma omp parallel for schedule(static)
         for (int i = 0; i < Nx; ++i)                                         /* 5. Ampere */
             for (int j = 1; j < Ny; ++j) {
-                float g = (Hz[IDX(i, j)] - Hz[IDX(i, j - 1)]) / dy;
+                float xg = Hz[IDX(i, j)] - Hz[IDX(i, j - 1)];
+                float g = rcp ? xg * dy : xg / dy;
                 if (Ca) {
                     float t1 = Ca[IDX(i, j)] * Ex[IDX(i, j)];
                     float t2 = Cb[IDX(i, j)] * g;
@@ -152,7 +162,8 @@ void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const flo
         #pragma omp parallel for schedule(static)
         for (int i = 1; i < Nx; ++i)
             for (int j = 0; j < Ny; ++j) {
-                float g = (Hz[IDX(i, j)] - Hz[IDX(i - 1, j)]) / dx;
+                float xg = Hz[IDX(i, j)] - Hz[IDX(i - 1, j)];
+                float g = rcp ? xg * dx : xg / dx;
                 if (Ca) {
                     float t1 = Ca[IDX(i, j)] * Ey[IDX(i, j)];
                     float t2 = Cb[IDX(i, j)] * g;
@@ -166,4 +177,23 @@ void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const flo
     }
     free(P);
     free(Hn);
+}
+
+void wf_brute_run_f32(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const float *Ca, const float *Cb,
+                      float Dh, float De, float dx, float dy, float cx, float cy, int n_steps,
+                      int n_src, const int *src_comp, const int *src_cell, const float *src_vals,
+                      int n_det, const int *det_cell, float *traces)
+{
+    run_f32(Nx, Ny, Ex, Ey, Hz, Ca, Cb, Dh, De, dx, dy, cx, cy, n_steps, n_src, src_comp, src_cell, src_vals,
+            n_det, det_cell, traces, 0);
+}
+
+/* The same run with reciprocal quotients: rdx = fp32(1/dx), rdy = fp32(1/dy). */
+void wf_brute_run_f32_rcp(int Nx, int Ny, float *Ex, float *Ey, float *Hz, const float *Ca, const float *Cb,
+                          float Dh, float De, float rdx, float rdy, float cx, float cy, int n_steps,
+                          int n_src, const int *src_comp, const int *src_cell, const float *src_vals,
+                          int n_det, const int *det_cell, float *traces)
+{
+    run_f32(Nx, Ny, Ex, Ey, Hz, Ca, Cb, Dh, De, rdx, rdy, cx, cy, n_steps, n_src, src_comp, src_cell, src_vals,
+            n_det, det_cell, traces, 1);
 }

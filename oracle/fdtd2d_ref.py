@@ -22,6 +22,16 @@ that, on the CPU, fp32 results are bit-identical to the reference itself:
 The class works in fp32 (bit-level oracle) or fp64 (error reference);
 with fp64, pass Ca/Cb already rounded to fp32 and converted, as the
 reference stores them in fp32 (``src/core/materials.py:218-250``).
+
+Division. The reference divides by the Python floats dx, dy
+(``src/core/fdtd2d.py:232-235, 250-251``). ``division="ieee"`` writes that
+expression as the reference does (``x / dy``): on the CPU torch evaluates it
+as IEEE ``x / fp32(dy)``. ``division="reciprocal"`` writes
+``x * fp32(1/dy)`` (the reciprocal of the double, rounded once, as a 0-dim
+tensor of the field dtype): the value torch forms for the reference's
+``x / dy`` on a GPU, restated so the CPU can compute it. ``to(device)`` moves
+the oracle's state, so the ``"ieee"`` form can also be evaluated by torch on
+a GPU, where it must equal the ``"reciprocal"`` form on the CPU.
 """
 
 from __future__ import annotations
@@ -140,6 +150,7 @@ class OracleFDTD2D:
         n_check: int = 100,
         stability_threshold: float = 1e10,
         functional: bool = False,
+        division: str = "ieee",
     ) -> None:
         if Nx < 4 or Ny < 4:
             raise ValueError("grid must be at least 4x4")
@@ -164,6 +175,33 @@ class OracleFDTD2D:
         self.last_field_max = -1.0
         self.n_stability_checks = 0
         self.functional = functional
+        if division not in ("ieee", "reciprocal"):
+            raise ValueError(f"division must be 'ieee' or 'reciprocal', got {division!r}")
+        self.division = division
+        self._rdx = torch.tensor(1.0 / dx, dtype=dtype)   # fp32(1/dx) for fp32 fields
+        self._rdy = torch.tensor(1.0 / dy, dtype=dtype)
+
+    def _over_dx(self, x: torch.Tensor) -> torch.Tensor:
+        return x * self._rdx if self.division == "reciprocal" else x / self.dx
+
+    def _over_dy(self, x: torch.Tensor) -> torch.Tensor:
+        return x * self._rdy if self.division == "reciprocal" else x / self.dy
+
+    def to(self, device) -> "OracleFDTD2D":
+        """Move fields, snapshot, coefficients and sources to ``device``."""
+        for k in self.fields:
+            self.fields[k] = self.fields[k].to(device)
+        self.Hz_prev = self.Hz_prev.to(device)
+        if self.Ca is not None:
+            self.Ca, self.Cb = self.Ca.to(device), self.Cb.to(device)
+        self._rdx, self._rdy = self._rdx.to(device), self._rdy.to(device)
+        for src in self.sources:
+            src.wave = src.wave.to(device)
+            for name in ("i_idx", "j_idx", "amp", "k_idx"):
+                v = getattr(src, name)
+                if v is not None:
+                    setattr(src, name, v.to(device))
+        return self
 
     # --- convenience accessors -------------------------------------------
     @property
@@ -203,20 +241,20 @@ class OracleFDTD2D:
             return self._step_functional()
         n = self.steps_completed
         Ex, Ey, Hz = self.fields["Ex"], self.fields["Ey"], self.fields["Hz"]
-        Dh, De, dx, dy = self.Dh, self.De, self.dx, self.dy
+        Dh, De = self.Dh, self.De
 
         self.snapshot()
         for src in self.sources:
             src.inject(self.fields, n)
 
         Hz[..., :-1, :-1, :] += Dh * (
-            (Ex[..., :-1, 1:, :] - Ex[..., :-1, :-1, :]) / dy
-            - (Ey[..., 1:, :-1, :] - Ey[..., :-1, :-1, :]) / dx
+            self._over_dy(Ex[..., :-1, 1:, :] - Ex[..., :-1, :-1, :])
+            - self._over_dx(Ey[..., 1:, :-1, :] - Ey[..., :-1, :-1, :])
         )
         self.apply_mur(Hz)
 
-        dHz_dy = (Hz[..., :, 1:, :] - Hz[..., :, :-1, :]) / dy
-        dHz_dx = (Hz[..., 1:, :, :] - Hz[..., :-1, :, :]) / dx
+        dHz_dy = self._over_dy(Hz[..., :, 1:, :] - Hz[..., :, :-1, :])
+        dHz_dx = self._over_dx(Hz[..., 1:, :, :] - Hz[..., :-1, :, :])
         if self.Ca is None:
             Ex[..., :, 1:, :] += De * dHz_dy
             Ey[..., 1:, :, :] -= De * dHz_dx
@@ -239,7 +277,7 @@ class OracleFDTD2D:
         arrays and rebuilding the Mur edges by concatenation."""
         n = self.steps_completed
         Ex, Ey, Hz = (self.fields[k][..., 0] for k in ("Ex", "Ey", "Hz"))
-        Dh, De, dx, dy = self.Dh, self.De, self.dx, self.dy
+        Dh, De = self.Dh, self.De
         P = Hz
         comp = {"Ex": Ex, "Ey": Ey, "Hz": Hz}
         for src in self.sources:
@@ -249,7 +287,7 @@ class OracleFDTD2D:
             vals = src.wave[n:n + 1] * src.amp
             comp[src.component] = t.index_put((src.i_idx, src.j_idx), vals, accumulate=True)
         Ex, Ey, H1 = comp["Ex"], comp["Ey"], comp["Hz"]
-        dF = Dh * ((Ex[:-1, 1:] - Ex[:-1, :-1]) / dy - (Ey[1:, :-1] - Ey[:-1, :-1]) / dx)
+        dF = Dh * (self._over_dy(Ex[:-1, 1:] - Ex[:-1, :-1]) - self._over_dx(Ey[1:, :-1] - Ey[:-1, :-1]))
         H2 = H1 + torch.nn.functional.pad(dF, (0, 1, 0, 1))
         cx, cy = self.cx, self.cy
         L = P[1, :] + cx * (H2[1, :] - P[0, :])
@@ -258,8 +296,8 @@ class OracleFDTD2D:
         B = P[:, 1] + cy * (H3[:, 1] - P[:, 0])
         T = P[:, -2] + cy * (H3[:, -2] - P[:, -1])
         H4 = torch.cat([B[:, None], H3[:, 1:-1], T[:, None]], 1)
-        gy = (H4[:, 1:] - H4[:, :-1]) / dy
-        gx = (H4[1:, :] - H4[:-1, :]) / dx
+        gy = self._over_dy(H4[:, 1:] - H4[:, :-1])
+        gx = self._over_dx(H4[1:, :] - H4[:-1, :])
         if self.Ca is None:
             Ex2 = torch.cat([Ex[:, :1], Ex[:, 1:] + De * gy], 1)
             Ey2 = torch.cat([Ey[:1, :], Ey[1:, :] - De * gx], 0)

@@ -101,7 +101,7 @@ def _source_cells(spec, nx, ny):
     raise ValueError(kind)
 
 
-def build_oracle(c, dtype=torch.float32):
+def build_oracle(c, dtype=torch.float32, division="ieee"):
     gen = torch.Generator().manual_seed(c["seed"])
     ca, cb = _materials(c, gen)
     init = _init_fields(c, gen)
@@ -116,14 +116,16 @@ def build_oracle(c, dtype=torch.float32):
         else:
             srcs.append(OracleSource(comp, w, kind=kind, position=spec[3]))
     o = OracleFDTD2D(c["nx"], c["ny"], DX, DX, dt, Ca=ca, Cb=cb, sources=srcs, detectors=c["detectors"],
-                     batch=c["batch"], dtype=dtype, n_check=c["n_check"], stability_threshold=c["threshold"])
+                     batch=c["batch"], dtype=dtype, n_check=c["n_check"], stability_threshold=c["threshold"],
+                     division=division)
     if init is not None:
         for k, v in init.items():
             o.fields[k].copy_(v.to(dtype))
     return o
 
 
-def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_div=True, dtype=torch.float32):
+def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_div=True, dtype=torch.float32,
+                  division="ieee"):
     from waveforge_amd import core
     from waveforge_amd.core.sources import Waveform
 
@@ -174,11 +176,12 @@ def build_product(c, device, *, depth=None, tile=None, n_det_steps=None, fast_di
     if ca is not None:
         kw = dict(Ca=ca.to(device), Cb=cb.to(device))
     sim = core.FDTD2D(grid, fields, boundary, coll, stability_threshold=c["threshold"], n_check=c["n_check"],
-                      detectors=det, depth=depth, tile=tile, fast_div=fast_div, **kw)
+                      detectors=det, depth=depth, tile=tile, fast_div=fast_div, division=division, **kw)
     return sim, det
 
 
 def assert_bitwise(product: torch.Tensor, oracle: torch.Tensor, what: str) -> None:
+    oracle = oracle.detach().cpu()
     p = product.detach().cpu().reshape(oracle.shape)
     same = torch.equal(p, oracle)
     if same and p.dtype == torch.float32:  # bit patterns too: +0 and -0 differ
@@ -200,7 +203,7 @@ def rel_l2(a, b) -> float:
     return float(torch.linalg.norm(a - b)) / (den if den > 0 else 1.0)
 
 
-def oracle_from_sim(sim, dtype=torch.float32, n_check=10 ** 9):
+def oracle_from_sim(sim, dtype=torch.float32, n_check=10 ** 9, division=None):
     """The oracle of a product ``FDTD2D`` built from the same inputs: Ca/Cb as
     stored, point/line sources with the product's fp32 waveform tables (fp32)
     or the tables rebuilt in fp64 by the same ``Waveform`` (fp64, as the
@@ -218,5 +221,7 @@ def oracle_from_sim(sim, dtype=torch.float32, n_check=10 ** 9):
     cells = [] if det is None else [tuple(c) for c in det.cells]
     ca = None if sim._Ca is None else sim._Ca.detach().cpu().reshape(g.Nx, g.Ny, 1)
     cb = None if sim._Cb is None else sim._Cb.detach().cpu().reshape(g.Nx, g.Ny, 1)
+    if division is None:  # the product's own division (fp64 runs divide as IEEE)
+        division = sim._division if dtype == torch.float32 else "ieee"
     return OracleFDTD2D(g.Nx, g.Ny, g.dx, g.dy, g.dt, Ca=ca, Cb=cb, sources=srcs, detectors=cells, dtype=dtype,
-                        n_check=n_check)
+                        n_check=n_check, division=division)

@@ -79,7 +79,7 @@ int main(void) {
   O(wf_run2d, det_cell) O(wf_run2d, traces) O(wf_run2d, trace_stride) O(wf_run2d, n_grp) O(wf_run2d, n_inj)
   O(wf_run2d, n_det) O(wf_run2d, trace_n0) O(wf_run2d, trace_len) O(wf_run2d, nx) O(wf_run2d, ny)
   O(wf_run2d, dh) O(wf_run2d, de) O(wf_run2d, dx) O(wf_run2d, dy) O(wf_run2d, cmx) O(wf_run2d, cmy)
-  O(wf_run2d, step) O(wf_run2d, flags) O(wf_run2d, status) O(wf_run2d, step_end)
+  O(wf_run2d, step) O(wf_run2d, flags) O(wf_run2d, status) O(wf_run2d, step_end) O(wf_run2d, rdx) O(wf_run2d, rdy) O(wf_run2d, pad)
   O(wf_tb_args, runs) O(wf_tb_args, tiles) O(wf_tb_args, n_runs) O(wf_tb_args, n_tiles) O(wf_tb_args, tile_i)
   O(wf_tb_args, tile_j) O(wf_tb_args, depth) O(wf_tb_args, parity) O(wf_tb_args, step_offset)
   O(wf_tb_args, has_materials) O(wf_tb_args, maxabs_out) O(wf_tb_args, rdx) O(wf_tb_args, rdy)

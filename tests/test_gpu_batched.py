@@ -30,7 +30,8 @@ def _media(nx, ny, seed, dt):
     return ca, cb
 
 
-def _specs(device, steps, dt, sizes=((37, 41), (64, 64), (100, 53), (20, 90), (150, 150)), amps=None):
+def _specs(device, steps, dt, sizes=((37, 41), (64, 64), (100, 53), (20, 90), (150, 150)), amps=None,
+           division="reciprocal"):
     from waveforge_amd.mimo import RunSpec
 
     specs, oracles = [], []
@@ -47,24 +48,26 @@ def _specs(device, steps, dt, sizes=((37, 41), (64, 64), (100, 53), (20, 90), (1
         osrc = [OracleSource(c, waves[w], torch.tensor([i]), torch.tensor([j]), torch.tensor([a]))
                 for c, i, j, w, a in src]
         oracles.append(OracleFDTD2D(nx, ny, DX, DX, dt, Ca=ca.unsqueeze(2), Cb=cb.unsqueeze(2), sources=osrc,
-                                    detectors=rx, n_check=10 ** 9))
+                                    detectors=rx, n_check=10 ** 9, division=division))
     return specs, oracles
 
 
+@pytest.mark.parametrize("division", ["reciprocal", "ieee"])
 @pytest.mark.parametrize("depth,tile", [(6, 32), (4, 16), (8, 48), (3, 32)])
-def test_batched_runs_each_equal_standalone_oracle(device, depth, tile):
+def test_batched_runs_each_equal_standalone_oracle(device, depth, tile, division):
     from waveforge_amd.mimo import BatchedRuns
 
     steps = 70
     dt = stable_dt(DX, DX)
-    specs, oracles = _specs(device, steps, dt)
-    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=depth, tile_rows=tile, n_check=25)
+    specs, oracles = _specs(device, steps, dt, division=division)
+    br = BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, depth=depth, tile_rows=tile, n_check=25,
+                     division=division)
     br.advance(30)
     br.advance(steps - 30)
     st = br.status()
     assert not any(st["diverged"]) and st["checks"] == 2
     for k, o in enumerate(oracles):
-        print(f"[batched] run={k} grid={o.Nx}x{o.Ny} depth={depth} tile={tile}", flush=True)
+        print(f"[batched] run={k} grid={o.Nx}x{o.Ny} depth={depth} tile={tile} division={division}", flush=True)
         o.run(steps)
         ex, ey, hz = br.fields(k)
         assert_bitwise(ex, o.Ex.reshape(o.Nx, o.Ny), f"run {k} Ex")
@@ -194,7 +197,7 @@ def test_run_mimo_matches_oracle_per_tx(device):
     for tx, (i, j) in enumerate(ants):
         o = OracleFDTD2D(n, n, DX, DX, g.dt, Ca=Ca.cpu(), Cb=Cb.cpu(),
                          sources=[OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0]))],
-                         detectors=ants, n_check=500)
+                         detectors=ants, n_check=500, division="reciprocal")
         o.run(steps)
         np.testing.assert_array_equal(S[tx], o.trace_array().numpy())
 

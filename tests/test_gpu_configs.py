@@ -84,7 +84,11 @@ def test_blob_4096_4000_steps(device):
     assert sim.n_stability_checks == 40 and sim.launch_shape == (48, 8, 1)
     lib = ctypes.CDLL(str(Path(__file__).resolve().parent.parent / "oracle" / "build" / "libwf_brute.so"))
     P = ctypes.c_void_p
-    lib.wf_brute_run_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 5 + [ctypes.c_float] * 6 + \
+    # the run divides as the reference does on a GPU (x * fp32(1/dx)): the C
+    # restatement's reciprocal entry, given fp32(1/dx), fp32(1/dy)
+    assert sim._division == "reciprocal"
+    fn = lib.wf_brute_run_f32_rcp
+    fn.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 5 + [ctypes.c_float] * 6 + \
         [ctypes.c_int, ctypes.c_int] + [P] * 3 + [ctypes.c_int] + [P] * 2
     o = oracle_from_sim(sim, torch.float32)   # the scalars and tables the oracle uses
     n = 4096
@@ -98,7 +102,7 @@ def test_blob_4096_4000_steps(device):
     tr = np.zeros((len(dcells), steps), dtype=np.float32)
     ptr = lambda a: a.ctypes.data_as(P)
     _log("blob: oracle (C restatement) for 4000 steps on the host")
-    lib.wf_brute_run_f32(n, n, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb), o.Dh, o.De, o.dx, o.dy, o.cx, o.cy,
+    fn(n, n, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb), o.Dh, o.De, float(o._rdx), float(o._rdy), o.cx, o.cy,
                          steps, 1, ptr(np.array([2], np.int32)), ptr(cells), ptr(vals), len(dcells), ptr(dcells),
                          ptr(tr))
     del o

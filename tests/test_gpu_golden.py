@@ -32,7 +32,7 @@ def _product(F, name, device, depth):
     det = core.DetectorSet(c["detectors"], c["steps"], grid=g) if c["detectors"] else None
     sim = core.FDTD2D(g, f, core.MurABC(g, f.Hz), core.SourceCollection(srcs),
                       Ca=torch.from_numpy(F[f"{name}_Ca"]).to(device), Cb=torch.from_numpy(F[f"{name}_Cb"]).to(device),
-                      n_check=c["n_check"], detectors=det, depth=depth)
+                      n_check=c["n_check"], detectors=det, depth=depth, division="ieee")
     return sim, f, det
 
 

@@ -1,5 +1,11 @@
 """HIP stepping kernels vs the CPU oracle: bit-exact fp32 fields and traces.
 
+Two division forms are checked. ``division="ieee"`` (IEEE ``x / fp32(d)``)
+against the oracle as the reference runs on the CPU; ``division="reciprocal"``
+(``x * fp32(1/d)``, the default) against the oracle's reciprocal form and
+against the reference's own expressions evaluated by torch on this GPU,
+which form the quotient that way.
+
 Both the per-step path (``wf_fdtd2d_step``) and the temporally blocked path
 (``wf_fdtd2d_tb_advance``) must reproduce the oracle bit for bit: the kernels
 use the reference's fp32 operation order and no fused multiply-adds. Cases
@@ -140,3 +146,87 @@ def test_fast_division_is_licensed_on_the_device(device):
                                     ctypes.c_void_p(_native.stream_handle(device))), "wf_div_verify")
     print(f"[fast-div] mismatches over every finite float: {int(bad.item())}", flush=True)
     assert int(bad.item()) > 0
+
+
+RCP_CASES = ["tiny4_free", "tiny5x7_mat", "mat_all_sources", "edge_sources", "face_cells", "odd_tiles_mat",
+             "batch3", "free_point_hz", "extreme_mat", "extreme_free"]
+
+
+def _gpu_reference(c, device):
+    """The oracle's literal form of the reference step (``x / dx``),
+    evaluated by torch on the GPU."""
+    return build_oracle(c).to(device)
+
+
+@pytest.mark.parametrize("name", RCP_CASES)
+def test_per_step_path_reciprocal_bitwise(device, name):
+    c = CASES[name]
+    print(f"[per-step rcp] case={name} grid={c['nx']}x{c['ny']} steps={c['steps']}", flush=True)
+    sim, det = build_product(c, device, division="reciprocal")
+    o = build_oracle(c, division="reciprocal")
+    g = _gpu_reference(c, device)
+    for _ in range(c["steps"]):
+        sim.step()
+        o.step()
+        g.step()
+    torch.cuda.synchronize()
+    _compare(sim, det, o, name + " vs CPU reciprocal form")
+    for k in ("Ex", "Ey", "Hz"):
+        assert_bitwise(g.fields[k], o.fields[k], f"{name} {k}: torch-GPU reference vs CPU reciprocal form")
+    if det is not None:
+        assert_bitwise(g.trace_array().cpu(), o.trace_array(), f"{name} traces: torch-GPU reference")
+
+
+@pytest.mark.parametrize("tile", [32, 16, 48, 64, 8, (16, 2), (32, 2), (48, 2)])
+@pytest.mark.parametrize("depth", [1, 2, 3, 5, 7, 8])
+@pytest.mark.parametrize("name", RCP_CASES)
+def test_tb_path_reciprocal_bitwise(device, name, tile, depth):
+    c = CASES[name]
+    print(f"[tb rcp] case={name} tile={tile} depth={depth}", flush=True)
+    sim, det = build_product(c, device, depth=depth, tile=tile, division="reciprocal")
+    o = build_oracle(c, division="reciprocal")
+    sim.run(c["steps"])
+    o.run(c["steps"])
+    torch.cuda.synchronize()
+    _compare(sim, det, o, name)
+
+
+def test_default_division_is_the_reference_on_the_gpu(device):
+    """An FDTD2D built without a division argument steps as the reference's
+    expressions do under torch on this GPU (blocked and per-step paths)."""
+    from waveforge_amd import core
+
+    c = CASES["mat_all_sources"]
+    sim, det = build_product(c, device, division="reciprocal", depth=8, tile=48)
+    assert sim._division == "reciprocal"
+    grid = core.YeeGrid(8, 8, 1e-3, 1e-3, device=device)
+    f = core.FieldSet(grid)
+    assert core.FDTD2D(grid, f, core.MurABC(grid, f.Hz))._division == "reciprocal"
+    g = _gpu_reference(c, device)
+    sim.run(c["steps"])
+    g.run(c["steps"])
+    torch.cuda.synchronize()
+    _compare(sim, det, g, "blocked path vs torch-GPU reference")
+
+
+@pytest.mark.parametrize("d", [1e-3, 5e-4, 2e-3, 7e-4, 1.1e-3, 2.5e-4, 0.37, 3.7])
+def test_torch_gpu_divides_by_a_python_float_as_a_reciprocal_product(device, d):
+    """The fact the reciprocal division form rests on: torch on this GPU
+    evaluates ``float32 tensor / python_float`` as ``x * fp32(1/d)`` (the
+    reciprocal of the double rounded once), not as IEEE ``x / fp32(d)``,
+    which torch computes on the CPU. Values span subnormals to 2^100, both
+    signs and zeros."""
+    import numpy as np
+
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    x = (rng.choice([-1.0, 1.0], n) * rng.uniform(1, 2, n) * np.exp2(rng.uniform(-140, 100, n))).astype(np.float32)
+    x[:8] = 0.0
+    x[8:16] = -0.0
+    q_gpu = (torch.from_numpy(x).to(device) / d).cpu().numpy()
+    rcp = (x * np.float32(1.0 / d)).astype(np.float32)
+    with np.errstate(all="ignore"):
+        ieee = (x / np.float32(d)).astype(np.float32)
+    assert np.array_equal(q_gpu.view(np.uint32), rcp.view(np.uint32))
+    q_cpu = (torch.from_numpy(x) / d).numpy()
+    assert np.array_equal(q_cpu.view(np.uint32), ieee.view(np.uint32))

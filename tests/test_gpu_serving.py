@@ -49,7 +49,8 @@ def test_mix_equals_alone_and_budget(device):
         assert torch.equal(a.image, mixed[r.rid][1]), f"request {r.rid} image differs alone vs mixed"
 
 
-def test_served_request_matches_oracle(device):
+@pytest.mark.parametrize("division", ["reciprocal", "ieee"])
+def test_served_request_matches_oracle(device, division):
     from waveforge_amd.core.sources import RickerWavelet
     from waveforge_amd.serving import ImagingServer
     from waveforge_amd.workloads import imaging_dt
@@ -57,7 +58,7 @@ def test_served_request_matches_oracle(device):
 
     dt = imaging_dt()
     req = _stream(n=1, seed=11)[0]
-    srv = ImagingServer(dt=dt, device=device, quantum=60, depth=5)
+    srv = ImagingServer(dt=dt, device=device, quantum=60, depth=5, division=division)
     srv.serve([req], realtime=False)
     r = srv.done[0]
     nx, ny = req.eps.shape
@@ -66,7 +67,7 @@ def test_served_request_matches_oracle(device):
     S = np.zeros((len(req.antennas), len(req.antennas), req.n_steps), dtype=np.float32)
     for t, (i, j) in enumerate(req.antennas):
         o = OracleFDTD2D(nx, ny, 1e-3, 1e-3, dt, Ca=ca, Cb=cb, detectors=list(req.antennas), n_check=10 ** 9,
-                         sources=[OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0]))])
+                         division=division, sources=[OracleSource("Hz", wave, torch.tensor([i]), torch.tensor([j]), torch.tensor([1.0]))])
         o.run(req.n_steps)
         S[t] = o.trace_array().numpy()
     np.testing.assert_array_equal(r.traces.cpu().numpy(), S)
@@ -74,9 +75,10 @@ def test_served_request_matches_oracle(device):
     np.testing.assert_array_equal(r.image.cpu().numpy(), img)
 
 
-def _brute_traces(req, dt):
+def _brute_traces(req, dt, division="reciprocal"):
     """The request's E x E x T traces from the oracle's C restatement (every TX
-    a Hz point source with the 1 GHz Ricker table, all elements recording Hz)."""
+    a Hz point source with the 1 GHz Ricker table, all elements recording Hz),
+    dividing as ``division`` does."""
     import ctypes
     from pathlib import Path
 
@@ -86,7 +88,8 @@ def _brute_traces(req, dt):
 
     lib = ctypes.CDLL(str(Path(__file__).resolve().parent.parent / "oracle" / "build" / "libwf_brute.so"))
     P = ctypes.c_void_p
-    lib.wf_brute_run_f32.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 5 + [ctypes.c_float] * 6 + \
+    fn = lib.wf_brute_run_f32_rcp if division == "reciprocal" else lib.wf_brute_run_f32
+    fn.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 5 + [ctypes.c_float] * 6 + \
         [ctypes.c_int, ctypes.c_int] + [P] * 3 + [ctypes.c_int] + [P] * 2
     nx, ny = req.eps.shape
     ca, cb = coefficients_from_maps(req.eps, req.sigma, dt)
@@ -101,9 +104,10 @@ def _brute_traces(req, dt):
     for t, (i, j) in enumerate(req.antennas):
         ex, ey, hz = (np.zeros((nx, ny), dtype=np.float32) for _ in range(3))
         tr = np.zeros((len(dcells), req.n_steps), dtype=np.float32)
-        lib.wf_brute_run_f32(nx, ny, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb), c.dh, c.de, c.dx, c.dy, c.cmx,
-                             c.cmy, req.n_steps, 1, ptr(np.array([2], np.int32)),
-                             ptr(np.array([i * ny + j], np.int32)), ptr(vals), len(dcells), ptr(dcells), ptr(tr))
+        qx, qy = (c.rdx, c.rdy) if division == "reciprocal" else (c.dx, c.dy)
+        fn(nx, ny, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb), c.dh, c.de, qx, qy, c.cmx,
+           c.cmy, req.n_steps, 1, ptr(np.array([2], np.int32)),
+           ptr(np.array([i * ny + j], np.int32)), ptr(vals), len(dcells), ptr(dcells), ptr(tr))
         S[t] = tr
     return S
 

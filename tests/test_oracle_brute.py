@@ -5,7 +5,10 @@
 loops. They share no code, so an off-by-one slice, a swapped sign or operand,
 or a Mur edge in the wrong order makes them disagree. In fp32 they must agree
 bit for bit (both round every operation once, in the same order); the fp64
-variants must agree too.
+variants must agree too. Both division forms are pinned: IEEE ``x / fp32(d)``
+(the reference's ``x / d`` on the CPU) and ``x * fp32(1/d)`` (the same
+expression on a GPU; ``division="reciprocal"`` in the oracle, the ``*_rcp``
+entry points in C).
 """
 
 import ctypes
@@ -26,7 +29,8 @@ LIB = ORACLE / "build" / "libwf_brute.so"
 def brute():
     subprocess.run(["make", "-s", "-C", str(ORACLE)], check=True)
     lib = ctypes.CDLL(str(LIB))
-    for name, ct in (("wf_brute_step_f32", ctypes.c_float), ("wf_brute_step_f64", ctypes.c_double)):
+    for name, ct in (("wf_brute_step_f32", ctypes.c_float), ("wf_brute_step_f64", ctypes.c_double),
+                     ("wf_brute_step_f32_rcp", ctypes.c_float)):
         fn = getattr(lib, name)
         P = ctypes.c_void_p
         fn.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P, P, P, ct, ct, ct, ct, ct, ct, ctypes.c_int, P, P, P]
@@ -44,7 +48,12 @@ def brute_run(lib, o, steps, dtype):
     if o.Ca is not None:
         ca = o.Ca.reshape(nx, ny).numpy().astype(npdt).copy()
         cb = o.Cb.reshape(nx, ny).numpy().astype(npdt).copy()
+    rcp = o.division == "reciprocal"
     fn = lib.wf_brute_step_f32 if dtype == torch.float32 else lib.wf_brute_step_f64
+    qx, qy = o.dx, o.dy
+    if rcp:  # the C entry takes the reciprocals in place of the cell sizes
+        assert dtype == torch.float32
+        fn, qx, qy = lib.wf_brute_step_f32_rcp, float(o._rdx), float(o._rdy)
     comp_code = {"Ex": 0, "Ey": 1, "Hz": 2}
     ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
     traces = []
@@ -67,7 +76,7 @@ def brute_run(lib, o, steps, dtype):
         cells_a = np.array(cells, dtype=np.int32)
         vals_a = np.array(vals, dtype=npdt)
         fn(nx, ny, ptr(ex), ptr(ey), ptr(hz), ptr(ca), ptr(cb),
-           o.Dh, o.De, o.dx, o.dy, o.cx, o.cy, len(comps), ptr(comps_a), ptr(cells_a), ptr(vals_a))
+           o.Dh, o.De, qx, qy, o.cx, o.cy, len(comps), ptr(comps_a), ptr(cells_a), ptr(vals_a))
         if o.detectors:
             traces.append([hz[i, j] for i, j in o.detectors])
     return ex, ey, hz, np.array(traces, dtype=npdt).T if traces else None
@@ -99,6 +108,50 @@ def test_oracle_matches_brute(brute, k, dtype):
         np.testing.assert_array_equal(o.trace_array().numpy(), tr)
     # a non-trivial trajectory: fields moved away from the initial state
     assert float(o.Hz.abs().max()) > 0.0
+
+
+@pytest.mark.parametrize("k", range(len(BRUTE_CASES)))
+def test_reciprocal_oracle_matches_brute(brute, k):
+    """The oracle's reciprocal form (the reference's quotients as torch forms
+    them on a GPU) equals the C loops with x * fp32(1/d), bit for bit."""
+    c = BRUTE_CASES[k]
+    o = build_oracle(c, division="reciprocal")
+    ex, ey, hz, tr = brute_run(brute, o, c["steps"], torch.float32)
+    o.run(c["steps"])
+    for got, want in ((o.Ex, ex), (o.Ey, ey), (o.Hz, hz)):
+        assert np.array_equal(got.reshape(c["nx"], c["ny"]).numpy().view(np.int32), want.view(np.int32))
+    if tr is not None:
+        assert np.array_equal(o.trace_array().numpy().view(np.int32), tr.view(np.int32))
+
+
+def test_reciprocal_and_ieee_forms_differ_but_agree_closely():
+    """The two division forms are different arithmetic (the reciprocal of
+    dx = 1 mm rounds to 1000 exactly, not to 1/fp32(0.001)), so their fields
+    part after a few steps, but only by fp32 rounding: well inside the 1e-5
+    relative L2 parity bar after 300 steps of a lossy medium."""
+    c = case(40, 44, steps=300, material=True, init=True, seed=4,
+             sources=[("point", "Hz", 20, 22, "dgauss"), ("point", "Ey", 7, 30, "sin")])
+    a = build_oracle(c)
+    b = build_oracle(c, division="reciprocal")
+    assert float(b._rdx) == 1000.0 and np.float32(1.0) / np.float32(1e-3) != np.float32(1000.0)
+    a.run(c["steps"])
+    b.run(c["steps"])
+    for name in ("Ex", "Ey", "Hz"):
+        x, y = a.fields[name].double(), b.fields[name].double()
+        assert not torch.equal(x, y)
+        assert float(torch.linalg.norm(x - y) / torch.linalg.norm(x)) < 1e-5
+
+
+def test_brute_detects_planted_reciprocal_error(brute):
+    """A reciprocal taken of the fp32 cell size (1/fp32(dx), 999.99994 at
+    1 mm) instead of the double's must be caught by the comparison."""
+    c = BRUTE_CASES[2]
+    o = build_oracle(c, division="reciprocal")
+    ex, ey, hz, _ = brute_run(brute, o, 12, torch.float32)
+    o._rdx = torch.tensor(np.float32(1.0) / np.float32(o.dx))  # plant the mistake in the oracle (CPU only)
+    o._rdy = torch.tensor(np.float32(1.0) / np.float32(o.dy))
+    o.run(12)
+    assert not np.array_equal(o.Hz.reshape(c["nx"], c["ny"]).numpy(), hz)
 
 
 def test_brute_detects_planted_sign_error(brute):

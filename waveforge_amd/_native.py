@@ -12,7 +12,7 @@ import threading
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libwaveforge_hip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 WF_COMP = {"Ex": 0, "Ey": 1, "Hz": 2}
 WF_COMP3 = {"Ex": 0, "Ey": 1, "Hz": 2, "Ez": 3, "Hx": 4, "Hy": 5}
@@ -21,6 +21,8 @@ WF_ADJ_TWO_PASS = 1  # wf_adj_args.flags: two launches per adjoint step
 WF_ADJ_ONE_STEP = 2  # wf_adj_args.flags: one launch per step (no two-step launches)
 WF_ADJ_FAST_DIV = 4  # wf_adj_args.flags: verified fast division in two-step launches
 TB_MAX_DEPTH = 8
+WF_RUN2D_RECIPROCAL = 1  # wf_run2d.flags: quotients as products by rdx, rdy (torch's GPU form)
+WF_DIV_IEEE, WF_DIV_FAST, WF_DIV_RECIPROCAL = 0, 1, 2  # wf_tb_args.fast_div
 
 # Every symbol the header declares; tests check the built library exports them.
 EXPORTED_SYMBOLS = (
@@ -84,7 +86,9 @@ class WfRun2d(ctypes.Structure):
         ("cmx", _f32), ("cmy", _f32),
         ("step", _i32), ("flags", _i32),
         ("status", _vp),
-        ("step_end", _i32), ("pad", _i32),
+        ("step_end", _i32),
+        ("rdx", _f32), ("rdy", _f32),
+        ("pad", _i32),
     ]
 
 

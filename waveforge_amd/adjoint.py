@@ -111,8 +111,9 @@ class TraceProblem:
         if self.full_history():
             return self._forward_full(ca, cb)
         specs = self._specs(ca, cb)
+        # IEEE division: the adjoint kernels transpose the step that divides by dx, dy
         br = BatchedRuns(specs, dx=self.dx, dy=self.dx, dt=self.dt, device=self.device, depth=self.depth,
-                         tile_rows=self.tile_rows, n_check=10 ** 9)
+                         tile_rows=self.tile_rows, n_check=10 ** 9, division="ieee")
         ckpts = []
         done = 0
         K = self.checkpoint
@@ -143,7 +144,7 @@ class TraceProblem:
         specs = self._specs(ca, cb)
         # the runs' source tables, receiver cells and trace buffer (their state is not stepped)
         br = BatchedRuns(specs, dx=self.dx, dy=self.dx, dt=self.dt, device=self.device, depth=1, tile_rows=48,
-                         n_check=10 ** 9)
+                         n_check=10 ** 9, division="ieee")
         self._tables = [r.sources for r in br.runs]
         hist = self._history(ca, cb, self.n_steps, recorders=br.runs)
         self._recompute(hist, None, 0, self.n_steps)

@@ -75,7 +75,8 @@ def _key(runs: Sequence[RunState], device, fast_div: bool, admissible, depths) -
     mats = int(runs[0].ca is not None)
     c = runs[0].coef
     name = torch.cuda.get_device_name(device)
-    return json.dumps([name, grids, mats, bool(fast_div), c.dx, c.dy, sorted(depths), admissible])
+    division = "reciprocal" if runs[0].reciprocal else ("fast" if fast_div else "ieee")
+    return json.dumps([name, grids, mats, division, c.dx, c.dy, sorted(depths), admissible])
 
 
 def _scratch(runs: Sequence[RunState], device) -> list[RunState]:
@@ -83,7 +84,7 @@ def _scratch(runs: Sequence[RunState], device) -> list[RunState]:
     out = []
     for r in runs:
         f = [(torch.rand((r.nx, r.ny), generator=g, device=device) - 0.5) * 1e-2 for _ in range(6)]
-        s = RunState(f[0], f[1], f[2], r.ca, r.cb, r.nx, r.ny, r.coef)
+        s = RunState(f[0], f[1], f[2], r.ca, r.cb, r.nx, r.ny, r.coef, reciprocal=r.reciprocal)
         s.second = (f[3], f[4], f[5])
         out.append(s)
     return out

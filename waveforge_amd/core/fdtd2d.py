@@ -10,6 +10,13 @@ where the work runs:
   temporally blocked LDS kernel, split at the ``n_check`` boundaries so the
   stability check sees the same fields at the same steps as the reference.
 
+Division. The reference divides float32 field differences by the Python
+floats dx, dy (``src/core/fdtd2d.py:232-235, 250-251``); torch forms that
+quotient as ``x * fp32(1/dx)`` on a GPU and as IEEE ``x / fp32(dx)`` on the
+CPU. ``division="reciprocal"`` (the default: this stepper runs on the GPU)
+reproduces the reference as it runs on a GPU, ``division="ieee"`` as it runs
+on the CPU; both bit for bit.
+
 Sources are read from ``self._sources`` at every call (tests reassign it), and
 the field tensors of the ``FieldSet`` are the live state, read and written in
 place, so callers may inspect or modify them between calls. Optional
@@ -30,6 +37,7 @@ from .. import _native
 from .._native import WfF64Args, WfRun2dF64
 from ..engine import (
     RunState,
+    check_division,
     iter_launches,
     judge,
     launch_parity,
@@ -126,6 +134,7 @@ class FDTD2D:
         depth=None,
         tile=None,
         fast_div: bool = True,
+        division: str = "reciprocal",
     ) -> None:
         if grid.Nx < 4:
             raise ValueError(f"grid.Nx must be >= 4 for a valid Yee stencil, got {grid.Nx}")
@@ -164,6 +173,7 @@ class FDTD2D:
         self._tile = tile
         self.launch_shape = None
         self._fast_div = fast_div
+        self._division = check_division(division)
         self.steps_completed: int = 0
         self.elapsed_time: float = 0.0
         self.mcells_per_second: float = 0.0
@@ -295,7 +305,8 @@ class FDTD2D:
             self._status = torch.zeros(1, dtype=torch.int32, device=hzs[0].device)
         runs = []
         for b in range(B):
-            rs = RunState(exs[b], eys[b], hzs[b], cas[b], cbs[b], Nx, Ny, coef, sources=srcs, status=self._status)
+            rs = RunState(exs[b], eys[b], hzs[b], cas[b], cbs[b], Nx, Ny, coef, sources=srcs, status=self._status,
+                          reciprocal=self._division == "reciprocal")
             rs.pstrip = torch.empty(4 * (Nx + Ny), dtype=torch.float32, device=hzs[b].device)
             if det is not None and len(det):
                 rs.det_cell = det.cell_index

@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(kBX * kE2Waves) k_adj_eh2(wf_adj_args a, AdjSt
             px[t] = cb[t] * lex[t];
             py[t] = cb[t] * ley[t];
         }
-        wf::divide_row<FAST, kE2PerWave>(px, divy, py, divx, vx, vy);
+        wf::divide_row<(FAST ? wf::kDivFast : wf::kDivIeee), kE2PerWave>(px, divy, py, divx, vx, vy);
 #pragma unroll
         for (int t = 0; t < kE2PerWave; ++t) s_h[w + kE2Waves * t][q] = vy[t];
         __syncthreads();
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(kBX * kE2Waves) k_adj_eh2(wf_adj_args a, AdjSt
             dgy[t] = h0[t] - hl[t];
             dgx[t] = h0[t] - hu[t];
         }
-        wf::divide_row<FAST, kE2PerWave>(dgy, divy, dgx, divx, gyq, gxq);
+        wf::divide_row<(FAST ? wf::kDivFast : wf::kDivIeee), kE2PerWave>(dgy, divy, dgx, divx, gyq, gxq);
 #pragma unroll
         for (int t = 0; t < kE2PerWave; ++t) {
             const int p = w + kE2Waves * t;

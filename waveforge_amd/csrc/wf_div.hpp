@@ -1,5 +1,11 @@
-// Exact fast fp32 division by a fixed divisor (the stepper's and the
-// adjoint's divisions by the cell sizes dx, dy).
+// fp32 division by a fixed divisor (the stepper's and the adjoint's
+// divisions by the cell sizes dx, dy), in one of three modes:
+//   kDivIeee  IEEE x/d: torch on the CPU divides a float32 tensor by a Python
+//             float this way;
+//   kDivFast  the exact fast quotient below, equal to IEEE x/d;
+//   kDivRcp   x*r with r = fp32(1/d) formed from the double d on the host:
+//             torch on a GPU divides a float32 tensor by a Python float this
+//             way (its reciprocal-multiply path for a CPU-scalar divisor).
 //
 // For d = m*2^e and r = RN(1/d), q0 = x*r is corrected by the residual of the
 // rescaled problem (x*2^k) - q0*(d*2^k), k = 31 - e, formed with one fma and
@@ -12,6 +18,8 @@
 #include <hip/hip_runtime.h>
 
 namespace wf {
+
+constexpr int kDivIeee = 0, kDivFast = 1, kDivRcp = 2;
 
 // Divisors the fast quotient takes: positive normal with exponent in [-95, 31].
 __host__ __device__ inline bool divider_ok(float d) {
@@ -64,11 +72,20 @@ __device__ __forceinline__ float divide(float x, const Divider &v) {
 // The 2*N quotients of one Faraday or Ampere row (N columns per lane, two
 // divisions per cell) with one range ballot and one branch between them: the
 // stepper's instruction stream is as much scalar as vector, so a branch per
-// division would cost as much as its range test.
-template <bool FAST, int N>
+// division would cost as much as its range test. MODE is kDivIeee, kDivFast
+// or kDivRcp (v.r is then the host's fp32(1/d) and each quotient one product).
+template <int MODE, int N>
 __device__ __forceinline__ void divide_row(const float (&x1)[N], const Divider &v1, const float (&x2)[N],
                                            const Divider &v2, float (&q1)[N], float (&q2)[N]) {
-    if (!FAST) {
+    if constexpr (MODE == kDivRcp) {
+#pragma unroll
+        for (int h = 0; h < N; ++h) {
+            q1[h] = x1[h] * v1.r;
+            q2[h] = x2[h] * v2.r;
+        }
+        return;
+    }
+    if constexpr (MODE == kDivIeee) {
 #pragma unroll
         for (int h = 0; h < N; ++h) {
             q1[h] = x1[h] / v1.d;

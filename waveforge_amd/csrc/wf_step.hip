@@ -18,7 +18,9 @@
 // All arithmetic is fp32 with one rounding per operation, in the reference's
 // operation order; the library is compiled with -ffp-contract=off so no
 // multiply-add is fused. Division is IEEE division by fp32(dx), as torch's
-// CPU `tensor / python_float` is.
+// CPU `tensor / python_float` is, or, for a run flagged WF_RUN2D_RECIPROCAL,
+// the product by fp32(1/dx) that torch on a GPU forms for the same
+// expression.
 #include "wf_common.hpp"
 
 #include <mutex>
@@ -78,6 +80,21 @@ __device__ __forceinline__ void pre_body(const R &r, int n) {
     }
 }
 
+// x / dx and x / dy: IEEE division, or the product by the run's reciprocal
+// when it is flagged WF_RUN2D_RECIPROCAL (fp32 runs only).
+template <class R>
+__device__ __forceinline__ real_t<R> over_dx(const R &r, real_t<R> x) {
+    if constexpr (std::is_same<R, wf_run2d>::value)
+        if (r.flags & WF_RUN2D_RECIPROCAL) return x * r.rdx;
+    return x / r.dx;
+}
+template <class R>
+__device__ __forceinline__ real_t<R> over_dy(const R &r, real_t<R> x) {
+    if constexpr (std::is_same<R, wf_run2d>::value)
+        if (r.flags & WF_RUN2D_RECIPROCAL) return x * r.rdy;
+    return x / r.dy;
+}
+
 // Faraday at interior cell (i, j) plus the Mur edge cells it owns.
 template <class R>
 __device__ __forceinline__ void h_body(const R &r, int i, int j) {
@@ -85,8 +102,8 @@ __device__ __forceinline__ void h_body(const R &r, int i, int j) {
     const int nx = r.nx, ny = r.ny;
     const size_t c = (size_t)i * ny + j;
     const T h1 = r.hz[c];
-    const T a = (r.ex[c + 1] - r.ex[c]) / r.dy;
-    const T b = (r.ey[c + ny] - r.ey[c]) / r.dx;
+    const T a = over_dy(r, r.ex[c + 1] - r.ex[c]);
+    const T b = over_dx(r, r.ey[c + ny] - r.ey[c]);
     const T F = h1 + r.dh * (a - b);
     r.hz[c] = F;
     const T cx = r.cmx, cy = r.cmy;
@@ -125,7 +142,7 @@ __device__ __forceinline__ void e_body(const R &r, int i, int j) {
     const size_t c = (size_t)i * ny + j;
     const T h = r.hz[c];
     if (j >= 1) {
-        const T g = (h - r.hz[c - 1]) / r.dy;
+        const T g = over_dy(r, h - r.hz[c - 1]);
         if (kMat) {
             const T t1 = r.ca[c] * r.ex[c];
             const T t2 = r.cb[c] * g;
@@ -136,7 +153,7 @@ __device__ __forceinline__ void e_body(const R &r, int i, int j) {
         }
     }
     if (i >= 1) {
-        const T g = (h - r.hz[c - ny]) / r.dx;
+        const T g = over_dx(r, h - r.hz[c - ny]);
         if (kMat) {
             const T t1 = r.ca[c] * r.ey[c];
             const T t2 = r.cb[c] * g;

@@ -66,7 +66,7 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: unsupported tile_i %d with %d column(s) per lane", a.tile_i, cpl);
         return WF_ERR_UNSUPPORTED;
     }
-    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div || cpl != 1 ||
+    if (a.hist_stride < 0 || (a.hist_stride > 0 && (!a.has_materials || a.fast_div != WF_DIV_IEEE || cpl != 1 ||
                                                       (a.tile_i != 32 && a.tile_i != 48) || a.maxabs_out || a.stop))) {
         wf::set_error("wf_fdtd2d_tb_advance: a state history (hist_stride %lld) needs materials, IEEE division, "
                       "tile_i 32 or 48 with one column per lane and no maxabs_out or stop flags", (long long)a.hist_stride);
@@ -76,21 +76,33 @@ extern "C" int wf_fdtd2d_tb_advance(const wf_tb_args *args, hipStream_t stream) 
         wf::set_error("wf_fdtd2d_tb_advance: tile_lists needs grp_list and det_list");
         return WF_ERR_ARG;
     }
-    if (a.fast_div && (a.div_lo_x != 0u || a.div_lo_y != 0u || a.div_span_x > 0x7f800000u ||
-                       a.div_span_y > 0x7f800000u)) {
-        wf::set_error("wf_fdtd2d_tb_advance: fast_div needs div_lo = 0 and div_span <= bits(inf)");
+    if (a.fast_div < WF_DIV_IEEE || a.fast_div > WF_DIV_RECIPROCAL) {
+        wf::set_error("wf_fdtd2d_tb_advance: fast_div must be WF_DIV_IEEE, WF_DIV_FAST or WF_DIV_RECIPROCAL, got %d",
+                      a.fast_div);
+        return WF_ERR_ARG;
+    }
+    if (a.fast_div == WF_DIV_FAST && (a.div_lo_x != 0u || a.div_lo_y != 0u || a.div_span_x > 0x7f800000u ||
+                                      a.div_span_y > 0x7f800000u)) {
+        wf::set_error("wf_fdtd2d_tb_advance: WF_DIV_FAST needs div_lo = 0 and div_span <= bits(inf)");
         return WF_ERR_ARG;
     }
     hipError_t e;
+    const int dv = a.fast_div;
     if (a.hist_stride > 0)
         e = wf_tb_launch_mat_ieee_hist(a, stream);
     else if (cpl == 2)
-        e = a.has_materials ? (a.fast_div ? wf_tb_launch_mat_fast_w2(a, stream) : wf_tb_launch_mat_ieee_w2(a, stream))
-                            : (a.fast_div ? wf_tb_launch_free_fast_w2(a, stream) : wf_tb_launch_free_ieee_w2(a, stream));
+        e = a.has_materials ? (dv == WF_DIV_FAST ? wf_tb_launch_mat_fast_w2(a, stream)
+                                                 : dv == WF_DIV_RECIPROCAL ? wf_tb_launch_mat_rcp_w2(a, stream)
+                                                                           : wf_tb_launch_mat_ieee_w2(a, stream))
+                            : (dv == WF_DIV_FAST ? wf_tb_launch_free_fast_w2(a, stream)
+                                                 : dv == WF_DIV_RECIPROCAL ? wf_tb_launch_free_rcp_w2(a, stream)
+                                                                           : wf_tb_launch_free_ieee_w2(a, stream));
     else if (a.has_materials)
-        e = a.fast_div ? wf_tb_launch_mat_fast(a, stream) : wf_tb_launch_mat_ieee(a, stream);
+        e = dv == WF_DIV_FAST ? wf_tb_launch_mat_fast(a, stream)
+                              : dv == WF_DIV_RECIPROCAL ? wf_tb_launch_mat_rcp(a, stream) : wf_tb_launch_mat_ieee(a, stream);
     else
-        e = a.fast_div ? wf_tb_launch_free_fast(a, stream) : wf_tb_launch_free_ieee(a, stream);
+        e = dv == WF_DIV_FAST ? wf_tb_launch_free_fast(a, stream)
+                              : dv == WF_DIV_RECIPROCAL ? wf_tb_launch_free_rcp(a, stream) : wf_tb_launch_free_ieee(a, stream);
     return wf::check_hip(e, "wf_fdtd2d_tb_advance");
 }
 

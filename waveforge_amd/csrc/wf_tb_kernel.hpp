@@ -3,21 +3,27 @@
 // Work decomposition. One workgroup of 8 wavefronts owns one core tile of one run: TI rows x
 // TJ = W - 2D columns, plus a halo of D cells per side. Every row of the
 // staged region is W = 64*CPL cells, one wavefront with CPL adjacent columns
-// per lane (lane l holds columns CPL*l .. CPL*l + CPL-1). Of NW wavefronts,
-// wave w holds rows w, w+NW, w+2NW, ... of the region in registers (Ex, Ey, Hz
-// and, on the material path, Ca/Cb). The workgroup advances the region D
-// leapfrog steps without touching HBM, shrinking the valid region by one cell
-// per side per step, then writes the core to another state buffer (no
-// workgroup reads what another writes in the same launch).
+// per lane (lane l holds columns CPL*l .. CPL*l + CPL-1). Wave w holds the
+// NR = ceil((TI + 2D)/8) consecutive rows w*NR .. w*NR + NR-1 of the region
+// in registers (Ex, Ey, Hz and, on the material path, Ca/Cb). The workgroup
+// advances the region D leapfrog steps without touching HBM, shrinking the
+// valid region by one cell per side per step, then writes the core to
+// another state buffer (no workgroup reads what another writes in the same
+// launch).
 // HBM traffic is ~(20*R + 12)/D bytes per cell-update (R = region/core area)
 // instead of the 32 B of one step per pass. Two columns per lane halve the
 // share of halo columns (a 128-wide region keeps 112 columns at D = 8, a
 // 64-wide one 48) at twice the registers per staged row.
 //
 // Neighbours: the j-neighbours of a cell are the lane's other column or the
-// adjacent lane's, fetched with a DPP whole-wave shift; the i-neighbours live
-// in another wavefront and go through LDS, which holds only the Hz and Ey rows
-// (Faraday reads Ey(i+1, j), Ampere reads Hz(i-1, j)).
+// adjacent lane's, fetched with a DPP whole-wave shift; the i-neighbours are
+// the lane's own registers, except across the boundary between two waves'
+// row blocks, where one row per wave goes through LDS each half-step (the Ey
+// of its first row for the wave above, the Hz of its last row for the wave
+// below). Interior tiles step every row of the region without row tests (a
+// row outside the valid trapezoid only feeds rows outside it); edge tiles
+// keep a copy of every Hz row in LDS for the Mur pass and step rows under
+// per-step row masks.
 //
 // Each step reproduces the reference semantics exactly, in the same fp32
 // operation order (the library is built with -ffp-contract=off):
@@ -35,16 +41,22 @@
 //      computed range; the cells the reference never writes (Ex of column
 //      j = 0, Ey of row i = 0) are saved before it and put back after it on
 //      the tiles that hold them, so interior tiles pay no per-cell select;
-//      Hz detectors owned by the core are read from LDS
+//      Hz detectors owned by the core are read from LDS (a tile that holds
+//      detectors keeps every Hz row there)
 //      (examples/brain_mimo_imaging.py:196-198).
 //
-// Division by fp32(dx) / fp32(dy): IEEE division, or (FAST) the reciprocal
-// product corrected by one fused residual step taken at a scale s = 2^k that
-// puts d*s in [2^31, 2^32): q0 = x*r, t = q0*(d*s) - x*s, q = q0 - t*(r/s).
-// The scaling keeps the residual exact down to the smallest subnormal x, and
-// the operand signs make q = x for x = +0 and x = -0. The library proves q
-// equal to IEEE x/d for every |x| below a bound h (wf_div_verify, exhaustive
-// per divisor); NaN and |x| >= h fall back to IEEE division.
+// Division by the cell sizes, DIV (wf_tb_args.fast_div):
+//   kDivIeee: IEEE x/fp32(d), as torch on the CPU divides by a Python float;
+//   kDivFast: the reciprocal product corrected by one fused residual step
+//     taken at a scale s = 2^k that puts d*s in [2^31, 2^32): q0 = x*r,
+//     t = q0*(d*s) - x*s, q = q0 - t*(r/s). The scaling keeps the residual
+//     exact down to the smallest subnormal x, and the operand signs make q = x
+//     for x = +0 and x = -0. The library proves q equal to IEEE x/d for every
+//     |x| below a bound h (wf_div_verify, exhaustive per divisor); NaN and
+//     |x| >= h fall back to IEEE division;
+//   kDivRcp: x*fp32(1/d), the reciprocal of the double cell size rounded once
+//     (each run's rdx, rdy), as torch on a GPU divides by a Python float: one
+//     product per quotient instead of five instructions.
 //
 // Guards. A source step past the end of its waveform table injects nothing
 // and sets WF_STATUS_WAVE_END in the run's status word; a tile region with
@@ -58,6 +70,7 @@
 #pragma once
 #include "wf_common.hpp"
 #include "wf_div.hpp"
+
 
 namespace wf_tb {
 
@@ -75,18 +88,11 @@ __device__ __forceinline__ float lane_prev(float v) {  // value of lane-1 (0 in 
 using wf::Divider;
 using wf::divide;
 using wf::divide_row;
+using wf::kDivFast;
+using wf::kDivIeee;
+using wf::kDivRcp;
 using wf::fast_quotient;
 using wf::make_divider;
-
-// Bits k (k < NR) of the rows w + NW*k of wavefront w that lie in [lo, hi).
-template <int NW, int NR>
-__device__ __forceinline__ unsigned int row_bits(int w, int lo, int hi) {
-    const int a = lo - w, b = hi - w;
-    const unsigned int klo = a <= 0 ? 0u : ((unsigned)a + NW - 1) / NW;
-    const unsigned int khi = b <= 0 ? 0u : min(((unsigned)b + NW - 1) / NW, (unsigned)NR);
-    if (khi <= klo) return 0u;
-    return (unsigned int)(((1ull << khi) - 1ull) & ~((1ull << klo) - 1ull));
-}
 
 // CPL consecutive floats of one LDS row at a lane's first column (8-byte
 // accesses for two columns per lane).
@@ -113,51 +119,54 @@ __device__ __forceinline__ void lds_put(float *p, const float (&v)[CPL]) {
 // budget is sized for. Every shape runs 8 wavefronts: the more waves share a
 // region, the fewer rows each walks per step, which is the latency of a
 // launch on grids too small to fill the GPU (a 16-row tile holds 4 rows per
-// lane). Each lane keeps 5*CPL values per row (Ex, Ey, Hz, Ca, Cb) plus 40-48
-// registers of temporaries; without the cap the scheduler hoists every row's
-// loads and runs at one or two waves per SIMD. An 8-wavefront workgroup puts
-// two waves on each SIMD, so its occupancy is an even number of waves per
-// SIMD: 8 (64 registers), 6 (80) or 4 (128).
+// lane). Each lane keeps 5*CPL values per row (Ex, Ey, Hz, Ca, Cb) plus the
+// temporaries of its division mode (36 registers for reciprocal products, 40
+// for the fast quotient, 48 for IEEE division); without the cap the scheduler
+// hoists every row's loads and runs at one or two waves per SIMD. An
+// 8-wavefront workgroup puts two waves on each SIMD, so its occupancy is an
+// even number of waves per SIMD: 8 (64 registers), 6 (80) or 4 (128).
 template <int TI, int CPL>
 constexpr int kWavesOf = 8;
 template <int TI, int CPL>
 constexpr int kThreadsOf = kWavesOf<TI, CPL> * kW;
 
-template <int TI, int D, int CPL>
+template <int TI, int D, int CPL, int DIV = kDivRcp>
 struct TileGeom {
     static constexpr int NW = kWavesOf<TI, CPL>;
     static constexpr int NT = NW * kW;
     static constexpr int W = kW * CPL;
-    static constexpr int NR = (TI + 2 * D + NW - 1) / NW;
-    static constexpr int kRegs = 5 * NR * CPL + (NW == 8 ? 40 : 48);
-    static constexpr int kWaves = NW == 8 ? (kRegs <= 72 ? 8 : (kRegs <= 80 ? 6 : (kRegs <= 128 ? 4 : 2)))
-                                          : (512 / kRegs > 8 ? 8 : 512 / kRegs);
+    static constexpr int RI = TI + 2 * D;
+    static constexpr int NR = (RI + NW - 1) / NW;  // wave w holds rows w*NR .. w*NR + NR-1
+    static constexpr int kRegs = 5 * NR * CPL + (DIV == kDivRcp ? 36 : (DIV == kDivFast ? 40 : 48));
+    static constexpr int kWaves = kRegs <= 64 ? 8 : (kRegs <= 80 ? 6 : (kRegs <= 128 ? 4 : 2));
     static_assert(NR <= 32, "row flags are one 32-bit word per wavefront");
-    static_assert(NT >= 2 * W + TI + 2 * D, "edge passes need a thread per face cell");
+    static_assert(NT >= 2 * W + RI, "edge passes need a thread per face cell");
 };
 
 // HIST: write the core after every step s to the written state + (s-1)*hist_stride
 // (a state history for the adjoint's recompute) instead of only after the last.
-template <int TI, int D, bool MAT, bool FAST, bool HIST = false, int CPL = 1>
-__global__ void __launch_bounds__((kThreadsOf<TI, CPL>)) __attribute__((amdgpu_waves_per_eu((TileGeom<TI, D, CPL>::kWaves))))
+template <int TI, int D, bool MAT, int DIV, bool HIST = false, int CPL = 1>
+__global__ void __launch_bounds__((kThreadsOf<TI, CPL>)) __attribute__((amdgpu_waves_per_eu((TileGeom<TI, D, CPL, DIV>::kWaves))))
 k_tb(wf_tb_args a) {
-    using G = TileGeom<TI, D, CPL>;
+    using G = TileGeom<TI, D, CPL, DIV>;
     constexpr int W = G::W;
     constexpr int TJ = W - 2 * D;
-    constexpr int RI = TI + 2 * D;
+    constexpr int RI = G::RI;
     constexpr int NW = G::NW;
     constexpr int kNT = G::NT;
     constexpr int NR = G::NR;
-    __shared__ __align__(16) float s_hz[RI * W];  // 8-byte accesses with two columns per lane
-    __shared__ __align__(16) float s_ey[RI * W];
+    __shared__ __align__(16) float s_hz[RI * W];  // edge tiles and tiles with detectors: every row's Hz
+    __shared__ __align__(16) float s_xey[(NW + 1) * W];  // entry w: Ey of wave w's first row (read by wave w-1)
+    __shared__ __align__(16) float s_xhz[(NW + 1) * W];  // entry w+1: Hz of wave w's last row (read by wave w+1)
     __shared__ int s_grp[kMaxGroups];
     __shared__ int s_grp_cell[kMaxGroups];
     __shared__ int s_det[kMaxDets];
     __shared__ int s_det_cell[kMaxDets];
-    __shared__ int s_ngrp, s_ndet, s_ey_src;
-    __shared__ unsigned int s_rowmask[NW];  // per wavefront: bit k set if row w+NW*k holds a source
+    __shared__ int s_ngrp, s_ndet;
+    __shared__ unsigned int s_rowmask[NW];  // per wavefront: bit k set if its row w*NR+k holds a source
     __shared__ unsigned int s_max[3];
     __shared__ float s_prow[2][W];   // edge tiles: snapshot of rows i = 1 and i = nx-2
+    __shared__ float s_frow[2][W];   // edge tiles: snapshot of the face rows i = 0 and i = nx-1
     __shared__ float s_pcol[2][RI];  // edge tiles: snapshot of columns j = 1 and j = ny-2
     __shared__ float s_fcol[2][RI];  // edge tiles: snapshot of the face columns j = 0 and j = ny-1
     __shared__ float s_keep_ex[RI];  // left tiles: Ex of column j = 0 before Ampere
@@ -178,6 +187,7 @@ k_tb(wf_tb_args a) {
     const int J0 = (tile % ntj) * TJ;
     const int gi0 = I0 - D, gj0 = J0 - D;  // global coordinates of region (0, 0)
     const int c0 = lane * CPL;             // region column of the lane's first cell
+    const int r0 = w * NR;                 // region row of the wave's first row
 
     // parity 0: A -> B, 1: B -> A, 2: A -> C, 3: C -> B (A = ex/ey/hz, B = ex2.., C = ex3..)
     const int par = a.parity;
@@ -232,7 +242,6 @@ k_tb(wf_tb_args a) {
     if (tid == 0) {
         s_ngrp = 0;
         s_ndet = 0;
-        s_ey_src = 0;
         s_max[0] = s_max[1] = s_max[2] = 0u;
     }
     if (tid < NW) s_rowmask[tid] = 0u;
@@ -243,10 +252,11 @@ k_tb(wf_tb_args a) {
         const int gj = gj0 + c0 + h;
         col_in[h] = gj >= 0 && gj < ny;
     }
+    // the wave's NR consecutive rows (rows past the region and outside the grid hold 0)
     float ex[NR][CPL], ey[NR][CPL], hz[NR][CPL], ca[NR][CPL], cb[NR][CPL];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const int li = w + NW * k;
+        const int li = r0 + k;
         const int gi = gi0 + li;
 #pragma unroll
         for (int h = 0; h < CPL; ++h) {
@@ -262,10 +272,6 @@ k_tb(wf_tb_args a) {
                     cb[k][h] = R.cb[g];
                 }
             }
-        }
-        if (li < RI) {
-            lds_put<CPL>(&s_hz[li * W + c0], hz[k]);
-            lds_put<CPL>(&s_ey[li * W + c0], ey[k]);
         }
     }
     __syncthreads();
@@ -285,8 +291,7 @@ k_tb(wf_tb_args a) {
             if (k < kMaxGroups) {
                 s_grp[k] = g;
                 s_grp_cell[k] = li * W + lj;
-                atomicOr(&s_rowmask[li % NW], 1u << (li / NW));
-                if (e0.comp == WF_COMP_EY) s_ey_src = 1;
+                atomicOr(&s_rowmask[li / NR], 1u << (li % NR));
             } else if (R.status) {
                 atomicOr(R.status, (unsigned int)WF_STATUS_TILE_GROUPS);
             }
@@ -309,47 +314,108 @@ k_tb(wf_tb_args a) {
     __syncthreads();
     const int ngrp = min(s_ngrp, kMaxGroups);
     const int ndet = min(s_ndet, kMaxDets);
-    const bool ey_src = s_ey_src != 0;
     const unsigned int rowmask = s_rowmask[w];
+    // Edge tiles (touching a grid face) step every row under the row masks and
+    // keep every Hz row in LDS, which the Mur pass reads and writes; interior
+    // tiles step every row of the region unmasked (a row outside the valid
+    // trapezoid only feeds rows outside it) and keep Hz rows in LDS only when
+    // they record detectors.
     const bool edge = (gi0 <= 0) || (gi0 + RI >= nx) || (gj0 <= 0) || (gj0 + W >= ny);
+    const bool mirror = edge || ndet > 0;
     const float dh = R.dh, de = R.de, cx = R.cmx, cy = R.cmy;
-    const Divider divx = make_divider(R.dx, a.rdx, a.div_span_x);
-    const Divider divy = make_divider(R.dy, a.rdy, a.div_span_y);
+    Divider divx{}, divy{};
+    if constexpr (DIV == kDivRcp) {  // quotients are products by the run's fp32(1/d)
+        divx.r = R.rdx;
+        divy.r = R.rdy;
+    } else {
+        divx = make_divider(R.dx, a.rdx, a.div_span_x);
+        divy = make_divider(R.dy, a.rdy, a.div_span_y);
+    }
     (void)de;
     bool lane_core[CPL];
 #pragma unroll
     for (int h = 0; h < CPL; ++h) lane_core[h] = c0 + h >= D && c0 + h < D + TJ && col_in[h];
 
-    if (edge) {  // the first step's Mur snapshot (LDS holds the state before any source)
-            {
-                // the edge passes' values are formed inside their branches (the
-                // empty asm keeps the compiler from hoisting them out of the step
-                // loop, where they would hold scalar registers for every tile)
-                int e_i0 = gi0, e_j0 = gj0;
-                asm volatile("" : "+s"(e_i0), "+s"(e_j0));
-                const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
-                const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;  // region columns of j = 1, ny-2
-                if (tid < W) {
-                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * W + tid];
-                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * W + tid];
-                } else if (tid >= 2 * W && tid < 2 * W + RI) {
-                    const int li = tid - 2 * W;
-                    if (q1 >= 0 && q1 < W) s_pcol[0][li] = s_hz[li * W + q1];
-                    if (q2 >= 0 && q2 < W) s_pcol[1][li] = s_hz[li * W + q2];
-                    if (q1 >= 1 && q1 <= W) s_fcol[0][li] = s_hz[li * W + q1 - 1];
-                    if (q2 >= -1 && q2 + 1 < W) s_fcol[1][li] = s_hz[li * W + q2 + 1];
-                }
+    // Faraday at row k from its registers and the Ey row below (eyd):
+    // Hz += Dh*((Ex(j+1) - Ex(j))/dy - (Ey(i+1) - Ey(i))/dx), every lane (on
+    // the face columns and outside the grid the value is discarded or feeds
+    // only discarded values)
+    auto faraday = [&](const int k, const float (&eyd)[CPL]) __attribute__((always_inline)) {
+        float x1[CPL], x2[CPL], av[CPL], bv[CPL];
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            const float exn = h + 1 < CPL ? ex[k][h + 1] : lane_next(ex[k][0]);
+            x1[h] = exn - ex[k][h];
+            x2[h] = eyd[h] - ey[k][h];
+        }
+        divide_row<DIV, CPL>(x1, divy, x2, divx, av, bv);
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) hz[k][h] = hz[k][h] + dh * (av[h] - bv[h]);
+    };
+    // Ampere at row k from its registers and the (new) Hz row above (hzu)
+    auto ampere = [&](const int k, const float (&hzu)[CPL]) __attribute__((always_inline)) {
+        float x3[CPL], x4[CPL], gyv[CPL], gxv[CPL];
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            const float hzl = h > 0 ? hz[k][h - 1] : lane_prev(hz[k][CPL - 1]);
+            x3[h] = hz[k][h] - hzl;
+            x4[h] = hz[k][h] - hzu[h];
+        }
+        divide_row<DIV, CPL>(x3, divy, x4, divx, gyv, gxv);
+#pragma unroll
+        for (int h = 0; h < CPL; ++h) {
+            if (MAT) {
+                const float t1 = ca[k][h] * ex[k][h];
+                const float t2 = cb[k][h] * gyv[h];
+                ex[k][h] = t1 + t2;
+                const float u1 = ca[k][h] * ey[k][h];
+                const float u2 = cb[k][h] * gxv[h];
+                ey[k][h] = u1 - u2;
+            } else {
+                const float t2 = de * gyv[h];
+                ex[k][h] = ex[k][h] + t2;
+                const float u2 = de * gxv[h];
+                ey[k][h] = ey[k][h] - u2;
             }
+        }
+    };
+
+    // Edge snapshot copies (before the first step, and after each step's Mur
+    // pass for the next): rows i = 1, nx-2 and the face rows i = 0, nx-1;
+    // columns j = 1, ny-2 and the face columns j = 0, ny-1. LDS holds the
+    // state every wave's registers hold, before any source of the step.
+    auto edge_snapshot = [&]() __attribute__((always_inline)) {
+        // the edge passes' values are formed inside their branches (the
+        // empty asm keeps the compiler from hoisting them out of the step
+        // loop, where they would hold registers for every tile)
+        int e_i0 = gi0, e_j0 = gj0, e_t = tid;
+        asm volatile("" : "+s"(e_i0), "+s"(e_j0), "+v"(e_t));
+        const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
+        const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;  // region columns of j = 1, ny-2
+        if (e_t < W) {
+            if (r1 >= 0 && r1 < RI) s_prow[0][e_t] = s_hz[r1 * W + e_t];
+            if (r2 >= 0 && r2 < RI) s_prow[1][e_t] = s_hz[r2 * W + e_t];
+            if (r1 >= 1 && r1 <= RI) s_frow[0][e_t] = s_hz[(r1 - 1) * W + e_t];
+            if (r2 >= -1 && r2 + 1 < RI) s_frow[1][e_t] = s_hz[(r2 + 1) * W + e_t];
+        } else if (e_t >= 2 * W && e_t < 2 * W + RI) {
+            const int li = e_t - 2 * W;
+            if (q1 >= 0 && q1 < W) s_pcol[0][li] = s_hz[li * W + q1];
+            if (q2 >= 0 && q2 < W) s_pcol[1][li] = s_hz[li * W + q2];
+            if (q1 >= 1 && q1 <= W) s_fcol[0][li] = s_hz[li * W + q1 - 1];
+            if (q2 >= -1 && q2 + 1 < W) s_fcol[1][li] = s_hz[li * W + q2 + 1];
+        }
+    };
+    if (edge) {  // the first step's Mur snapshot
+#pragma unroll
+        for (int k = 0; k < NR; ++k)
+            if (r0 + k < RI) lds_put<CPL>(&s_hz[(r0 + k) * W + c0], hz[k]);
         __syncthreads();
+        edge_snapshot();
     }
 #pragma unroll 1
     for (int s = 1; s <= D; ++s) {
         const int n = n0 + s - 1;
         if (n >= n_end) break;  // the run's last step was taken: its state stays as it is
-        // row conditions below are formed from this per-step copy rather than
-        // hoisted out of the step loop as one held mask per row
-        int gib = gi0;
-        asm volatile("" : "+s"(gib));
         // 1. Soft sources into the owning lane's registers (rows that hold a
         // source are flagged per wavefront, so the scan is skipped elsewhere).
         if (ngrp > 0) {
@@ -359,7 +425,7 @@ k_tb(wf_tb_args a) {
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 if (rm & (1u << k)) {
-                    const int li = w + NW * k;
+                    const int li = r0 + k;
                     for (int q = 0; q < ngrp; ++q) {
                         const int cl = s_grp_cell[q];
 #pragma unroll
@@ -378,79 +444,71 @@ k_tb(wf_tb_args a) {
                                         atomicOr(R.status, (unsigned int)WF_STATUS_WAVE_END);
                                     }
                                 }
-                                if (comp == WF_COMP_EX) ex[k][h] = v;
-                                else if (comp == WF_COMP_EY) { ey[k][h] = v; s_ey[cl] = v; }
-                                else hz[k][h] = v;
+                                // value selects, not a store through a selected array
+                                // (which would move the rows to scratch memory)
+                                ex[k][h] = comp == WF_COMP_EX ? v : ex[k][h];
+                                ey[k][h] = comp == WF_COMP_EY ? v : ey[k][h];
+                                hz[k][h] = comp == WF_COMP_HZ ? v : hz[k][h];
                             }
                         }
                     }
                 }
             }
-            if (ey_src) __syncthreads();
         }
-        // 2. Faraday. Row conditions are wavefront-uniform (one bit per row of a
-        // per-step mask). On edge tiles the snapshot of the rows and columns
-        // next to the boundary, which the Mur pass below reads, was kept in
-        // LDS before Faraday overwrites them (before the loop for the first
-        // step, after the previous step's Mur pass for the others).
-        // rows this step's Faraday updates: s-1 <= li < RI-s and 1 <= gi <= nx-2,
-        // one bit per row, so each row's test is one scalar bit test
-        unsigned int fmask = row_bits<NW, NR>(w, max(s - 1, 1 - gib), min(RI - s, nx - 1 - gib));
-        asm volatile("" : "+s"(fmask));
+        // 2. The Ey of each wave's first row, which the wave above reads for
+        // its last row's Faraday (after the sources: Faraday reads injected Ey).
+        lds_put<CPL>(&s_xey[w * W + c0], ey[0]);
+        __syncthreads();
+        // 3. Faraday on every row; its values on the face rows and columns
+        // are replaced by the Mur pass below, and rows outside the valid
+        // trapezoid feed only rows outside it.
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const int li = w + NW * k;
-            if (fmask & (1u << k)) {
-                const int c = li * W + c0;
-                float eyd[CPL], x1[CPL], x2[CPL], av[CPL], bv[CPL];
-                lds_get<CPL>(&s_ey[c + W], eyd);
+            float eyd[CPL];
+            if (k + 1 < NR) {
 #pragma unroll
-                for (int h = 0; h < CPL; ++h) {
-                    const float exn = h + 1 < CPL ? ex[k][h + 1] : lane_next(ex[k][0]);
-                    x1[h] = exn - ex[k][h];
-                    x2[h] = eyd[h] - ey[k][h];
-                }
-                divide_row<FAST, CPL>(x1, divy, x2, divx, av, bv);
-#pragma unroll
-                for (int h = 0; h < CPL; ++h) {
-                    // every lane takes F: on the face columns the Mur pass below
-                    // overwrites it (from the snapshot kept above), and lanes
-                    // outside the grid feed only each other
-                    hz[k][h] = hz[k][h] + dh * (av[h] - bv[h]);
-                }
-                lds_put<CPL>(&s_hz[c], hz[k]);
+                for (int h = 0; h < CPL; ++h) eyd[h] = ey[k + 1][h];
+            } else {
+                lds_get<CPL>(&s_xey[(w + 1) * W + c0], eyd);
             }
+            faraday(k, eyd);
         }
+        if (mirror) {  // edge tiles (the Mur pass) and detectors read Hz rows from LDS
+#pragma unroll
+            for (int k = 0; k < NR; ++k)
+                if (r0 + k < RI) lds_put<CPL>(&s_hz[(r0 + k) * W + c0], hz[k]);
+        }
+        lds_put<CPL>(&s_xhz[(w + 1) * W + c0], hz[NR - 1]);
         __syncthreads();
         if (edge) {
             // Mur (src/core/boundaries.py:259-274) on the faces next to this
-            // step's Faraday rows [s-1, RI-s): f[face] = P[in] + C (f[in] - P[face]),
+            // step's valid Faraday rows [s-1, RI-s): f[face] = P[in] + C (f[in] - P[face]),
             // x faces first; a corner takes the y-face formula over the x-face
             // value beside it. One thread per face cell; every value it reads
             // is Faraday's or the snapshot's, so the pass needs no barrier.
-            int e_i0 = gi0, e_j0 = gj0;
-            asm volatile("" : "+s"(e_i0), "+s"(e_j0));
+            int e_i0 = gi0, e_j0 = gj0, e_t = tid;
+            asm volatile("" : "+s"(e_i0), "+s"(e_j0), "+v"(e_t));
             const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;
             const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;
-            if (tid < 2 * W) {  // x faces: rows i = 0 (from i = 1) and i = nx-1 (from i = nx-2)
-                const int side = tid >= W, l = tid - side * W;
+            if (e_t < 2 * W) {  // x faces: rows i = 0 (from i = 1) and i = nx-1 (from i = nx-2)
+                const int side = e_t >= W, l = e_t - side * W;
                 const int rin = side ? r2 : r1, rf = side ? rin + 1 : rin - 1;
                 const int gjl = e_j0 + l;
                 if (rin >= s - 1 && rin < RI - s && rf >= 0 && rf < RI && gjl >= 1 && gjl <= ny - 2) {
-                    const float p_in = s_prow[side][l], p_f = s_hz[rf * W + l];
+                    const float p_in = s_prow[side][l], p_f = s_frow[side][l];
                     const float L = p_in + cx * (s_hz[rin * W + l] - p_f);
                     if (gjl == 1 && l >= 1) {
-                        const float pc = s_hz[rf * W + l - 1];
+                        const float pc = s_frow[side][l - 1];
                         s_hz[rf * W + l - 1] = p_f + cy * (L - pc);
                     }
                     if (gjl == ny - 2 && l + 1 < W) {
-                        const float pc = s_hz[rf * W + l + 1];
+                        const float pc = s_frow[side][l + 1];
                         s_hz[rf * W + l + 1] = p_f + cy * (L - pc);
                     }
                     s_hz[rf * W + l] = L;
                 }
-            } else if (tid - 2 * W < RI) {  // y faces: j = 0 (from j = 1) and j = ny-1 (from j = ny-2)
-                const int li = tid - 2 * W, gil = e_i0 + li;
+            } else if (e_t - 2 * W < RI) {  // y faces: j = 0 (from j = 1) and j = ny-1 (from j = ny-2)
+                const int li = e_t - 2 * W, gil = e_i0 + li;
                 if (li >= s - 1 && li < RI - s && gil >= 1 && gil <= nx - 2) {
                     if (q1 >= 1 && q1 < W) {
                         const float pf = s_fcol[0][li];
@@ -463,50 +521,26 @@ k_tb(wf_tb_args a) {
                 }
             }
             __syncthreads();
-            // face cells take their Mur values. LDS equals the registers on
-            // every other cell of the rows that stay valid (Faraday wrote its
-            // values to both; a source added outside them lands only on faces,
-            // which Mur overwrites), so whole rows are reloaded.
+            // face cells take their Mur values: LDS equals the registers on
+            // every other cell, so whole rows are reloaded
 #pragma unroll
-            for (int k = 0; k < NR; ++k) {
-                const int li = w + NW * k;
-                if (li < RI) lds_get<CPL>(&s_hz[li * W + c0], hz[k]);
-            }
+            for (int k = 0; k < NR; ++k)
+                if (r0 + k < RI) lds_get<CPL>(&s_hz[(r0 + k) * W + c0], hz[k]);
             // the next step's snapshot: LDS holds this step's final Hz, which
             // nothing rewrites before the next Faraday (Ampere and the
-            // detectors only read it; its barrier orders these copies first)
-            if (s < D)
-            {
-                // the edge passes' values are formed inside their branches (the
-                // empty asm keeps the compiler from hoisting them out of the step
-                // loop, where they would hold scalar registers for every tile)
-                int e_i0 = gi0, e_j0 = gj0;
-                asm volatile("" : "+s"(e_i0), "+s"(e_j0));
-                const int r1 = 1 - e_i0, r2 = nx - 2 - e_i0;  // region rows of i = 1, nx-2
-                const int q1 = 1 - e_j0, q2 = ny - 2 - e_j0;  // region columns of j = 1, ny-2
-                if (tid < W) {
-                    if (r1 >= 0 && r1 < RI) s_prow[0][tid] = s_hz[r1 * W + tid];
-                    if (r2 >= 0 && r2 < RI) s_prow[1][tid] = s_hz[r2 * W + tid];
-                } else if (tid >= 2 * W && tid < 2 * W + RI) {
-                    const int li = tid - 2 * W;
-                    if (q1 >= 0 && q1 < W) s_pcol[0][li] = s_hz[li * W + q1];
-                    if (q2 >= 0 && q2 < W) s_pcol[1][li] = s_hz[li * W + q2];
-                    if (q1 >= 1 && q1 <= W) s_fcol[0][li] = s_hz[li * W + q1 - 1];
-                    if (q2 >= -1 && q2 + 1 < W) s_fcol[1][li] = s_hz[li * W + q2 + 1];
-                }
-            }
+            // detectors only read it; the next step's first barrier orders
+            // these copies before it)
+            if (s < D) edge_snapshot();
             // The cells Ampere must leave as they are: Ex of column j = 0 and
             // Ey of row i = 0 (src/core/fdtd2d.py:244-257 writes Ex[:, 1:] and
             // Ey[1:, :]); kept per lane (LDS written and read by the same
             // lane) and put back after Ampere.
-            int e_j0b = gj0;
-            asm volatile("" : "+s"(e_j0b));
-            if (e_j0b <= 0) {
-                const int cz = -e_j0b;  // region column of j = 0
+            if (e_j0 <= 0) {
+                const int cz = -e_j0;  // region column of j = 0
                 if (lane == cz / CPL) {
 #pragma unroll
                     for (int k = 0; k < NR; ++k) {
-                        const int li = w + NW * k;
+                        const int li = r0 + k;
                         float v = ex[k][0];
 #pragma unroll
                         for (int h = 1; h < CPL; ++h) v = cz % CPL == h ? ex[k][h] : v;
@@ -514,65 +548,46 @@ k_tb(wf_tb_args a) {
                     }
                 }
             }
-            if (gib <= 0 && w == (-gib) % NW) {
-                const int kz = (-gib) / NW;  // wave w holds region row -gib (i = 0) as its row kz
+            if (e_i0 <= 0 && w == (-e_i0) / NR) {
+                const int kz = (-e_i0) % NR;  // wave w holds region row -gi0 (i = 0) as its row kz
+                // a select chain, not ey[kz] (a dynamic index would move the rows to scratch)
+                float v[CPL];
 #pragma unroll
-                for (int k = 0; k < NR; ++k)
-                    if (k == kz) lds_put<CPL>(&s_keep_ey[c0], ey[k]);
+                for (int h = 0; h < CPL; ++h) v[h] = ey[0][h];
+#pragma unroll
+                for (int k = 1; k < NR; ++k)
+#pragma unroll
+                    for (int h = 0; h < CPL; ++h) v[h] = k == kz ? ey[k][h] : v[h];
+                lds_put<CPL>(&s_keep_ey[c0], v);
             }
         }
-        // 3. Ampere + detectors.
+        // 4. Detectors (Hz after the step), then Ampere on every row.
         if (tid < ndet) {
             const int tt = n - R.trace_n0;
             if (tt >= 0 && tt < R.trace_len) R.traces[s_det[tid] * R.trace_stride + tt] = s_hz[s_det_cell[tid]];
         }
-        // rows this step's Ampere updates: s <= li < RI-s and 0 <= gi < nx
-        unsigned int amask = row_bits<NW, NR>(w, max(s, -gib), min(RI - s, nx - gib));
-        asm volatile("" : "+s"(amask));
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            const int li = w + NW * k;
-            if (amask & (1u << k)) {
-                const int c = li * W + c0;
-                float hzu[CPL], x3[CPL], x4[CPL], gyv[CPL], gxv[CPL];
-                lds_get<CPL>(&s_hz[c - W], hzu);
+            float hzu[CPL];
+            if (k > 0) {
 #pragma unroll
-                for (int h = 0; h < CPL; ++h) {
-                    const float hzl = h > 0 ? hz[k][h - 1] : lane_prev(hz[k][CPL - 1]);
-                    x3[h] = hz[k][h] - hzl;
-                    x4[h] = hz[k][h] - hzu[h];
-                }
-                divide_row<FAST, CPL>(x3, divy, x4, divx, gyv, gxv);
-#pragma unroll
-                for (int h = 0; h < CPL; ++h) {
-                    // lanes outside the grid update too: they feed only each
-                    // other and the discarded Faraday values of the face columns
-                    if (MAT) {
-                        const float t1 = ca[k][h] * ex[k][h];
-                        const float t2 = cb[k][h] * gyv[h];
-                        ex[k][h] = t1 + t2;
-                        const float u1 = ca[k][h] * ey[k][h];
-                        const float u2 = cb[k][h] * gxv[h];
-                        ey[k][h] = u1 - u2;
-                    } else {
-                        const float t2 = de * gyv[h];
-                        ex[k][h] = ex[k][h] + t2;
-                        const float u2 = de * gxv[h];
-                        ey[k][h] = ey[k][h] - u2;
-                    }
-                }
-                lds_put<CPL>(&s_ey[c], ey[k]);
+                for (int h = 0; h < CPL; ++h) hzu[h] = hz[k - 1][h];
+            } else if (edge) {  // the row above, after the Mur pass
+                lds_get<CPL>(&s_hz[max(r0 - 1, 0) * W + c0], hzu);
+            } else {
+                lds_get<CPL>(&s_xhz[w * W + c0], hzu);
             }
+            ampere(k, hzu);
         }
         if (edge) {
-            int e_j0b = gj0;
-            asm volatile("" : "+s"(e_j0b));
-            if (e_j0b <= 0) {
-                const int cz = -e_j0b;
+            int e_i0 = gi0, e_j0 = gj0;
+            asm volatile("" : "+s"(e_i0), "+s"(e_j0));
+            if (e_j0 <= 0) {
+                const int cz = -e_j0;
                 if (lane == cz / CPL) {
 #pragma unroll
                     for (int k = 0; k < NR; ++k) {
-                        const int li = w + NW * k;
+                        const int li = r0 + k;
                         if (li < RI) {
                             const float v = s_keep_ex[li];
 #pragma unroll
@@ -582,22 +597,21 @@ k_tb(wf_tb_args a) {
                     }
                 }
             }
-            if (gib <= 0 && w == (-gib) % NW) {
-                const int kz = (-gib) / NW;
+            if (e_i0 <= 0 && w == (-e_i0) / NR) {
+                const int kz = (-e_i0) % NR;
+                float v[CPL];
+                lds_get<CPL>(&s_keep_ey[c0], v);
 #pragma unroll
                 for (int k = 0; k < NR; ++k)
-                    if (k == kz) {
-                        lds_get<CPL>(&s_keep_ey[c0], ey[k]);
-                        lds_put<CPL>(&s_ey[(w + NW * k) * W + c0], ey[k]);
-                    }
+#pragma unroll
+                    for (int h = 0; h < CPL; ++h) ey[k][h] = k == kz ? v[h] : ey[k][h];
             }
         }
-        __syncthreads();
         if (HIST) {
             const int64_t hoff = (int64_t)(s - 1) * a.hist_stride;
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const int li = w + NW * k;
+                const int li = r0 + k;
                 const int gi = gi0 + li;
 #pragma unroll
                 for (int h = 0; h < CPL; ++h) {
@@ -617,7 +631,7 @@ k_tb(wf_tb_args a) {
     unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const int li = w + NW * k;
+        const int li = r0 + k;
         const int gi = gi0 + li;
 #pragma unroll
         for (int h = 0; h < CPL; ++h) {
@@ -641,13 +655,13 @@ k_tb(wf_tb_args a) {
     }
 }
 
-template <int TI, bool MAT, bool FAST, bool HIST = false, int CPL = 1>
+template <int TI, bool MAT, int DIV, bool HIST = false, int CPL = 1>
 hipError_t launch_depth(const wf_tb_args &a, hipStream_t st) {
     const unsigned blocks = 8u * (unsigned)((a.n_tiles + 7) / 8);
     switch (a.depth) {
 #define WF_TB_CASE(DD)                                                                                        \
     case DD:                                                                                                  \
-        hipLaunchKernelGGL((k_tb<TI, DD, MAT, FAST, HIST, CPL>), dim3(blocks), dim3(TileGeom<TI, DD, CPL>::NT), 0, \
+        hipLaunchKernelGGL((k_tb<TI, DD, MAT, DIV, HIST, CPL>), dim3(blocks), dim3(TileGeom<TI, DD, CPL>::NT), 0, \
                            st, a);                                                                            \
         break;
         WF_TB_CASE(1)
@@ -679,25 +693,29 @@ WF_TB_DECLARE(wf_tb_launch_mat_fast_w2);
 WF_TB_DECLARE(wf_tb_launch_mat_ieee_w2);
 WF_TB_DECLARE(wf_tb_launch_free_fast_w2);
 WF_TB_DECLARE(wf_tb_launch_free_ieee_w2);
+WF_TB_DECLARE(wf_tb_launch_mat_rcp);
+WF_TB_DECLARE(wf_tb_launch_free_rcp);
+WF_TB_DECLARE(wf_tb_launch_mat_rcp_w2);
+WF_TB_DECLARE(wf_tb_launch_free_rcp_w2);
 WF_TB_DECLARE(wf_tb_launch_mat_ieee_hist);  // hist_stride != 0: 32- or 48-row tiles, one column per lane
 
 namespace wf_tb {
-template <bool MAT, bool FAST, int CPL>
+template <bool MAT, int DIV, int CPL>
 hipError_t launch_tile(const wf_tb_args &a, hipStream_t st) {
     switch (a.tile_i) {
         case 8:
-            if constexpr (CPL == 1) return launch_depth<8, MAT, FAST, false, CPL>(a, st);
+            if constexpr (CPL == 1) return launch_depth<8, MAT, DIV, false, CPL>(a, st);
             return hipErrorInvalidValue;
-        case 16: return launch_depth<16, MAT, FAST, false, CPL>(a, st);
-        case 32: return launch_depth<32, MAT, FAST, false, CPL>(a, st);
-        case 48: return launch_depth<48, MAT, FAST, false, CPL>(a, st);
+        case 16: return launch_depth<16, MAT, DIV, false, CPL>(a, st);
+        case 32: return launch_depth<32, MAT, DIV, false, CPL>(a, st);
+        case 48: return launch_depth<48, MAT, DIV, false, CPL>(a, st);
         case 64:
-            if constexpr (CPL == 1) return launch_depth<64, MAT, FAST, false, CPL>(a, st);
+            if constexpr (CPL == 1) return launch_depth<64, MAT, DIV, false, CPL>(a, st);
             return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
 }  // namespace wf_tb
 
-#define WF_TB_DEFINE(NAME, MAT, FAST, CPL) \
-    WF_TB_DECLARE(NAME) { return wf_tb::launch_tile<MAT, FAST, CPL>(a, st); }
+#define WF_TB_DEFINE(NAME, MAT, DIV, CPL) \
+    WF_TB_DECLARE(NAME) { return wf_tb::launch_tile<MAT, DIV, CPL>(a, st); }

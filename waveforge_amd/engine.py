@@ -106,6 +106,21 @@ def require_gpu_tensor(t: torch.Tensor, what: str, dtypes=(torch.float32,)) -> N
         raise RuntimeError(f"waveforge_amd: {what} must be {names}, got {t.dtype}")
 
 
+DIVISIONS = ("reciprocal", "ieee")
+
+
+def check_division(division: str) -> str:
+    """How a run forms the reference's quotients ``x / dx``, ``x / dy`` of a
+    float32 tensor by a Python float (``src/core/fdtd2d.py:232-235, 250-251``):
+    ``"reciprocal"``, ``x * fp32(1/dx)`` (the reciprocal of the double rounded
+    once, then one fp32 product), which is what torch computes for that
+    expression on a GPU; ``"ieee"``, IEEE ``x / fp32(dx)``, which is what torch
+    computes on the CPU."""
+    if division not in DIVISIONS:
+        raise ValueError(f"division must be one of {DIVISIONS}, got {division!r}")
+    return division
+
+
 @dataclass
 class StepCoefficients:
     """fp32 scalars of one grid, rounded exactly as the reference's torch ops round them."""
@@ -116,6 +131,8 @@ class StepCoefficients:
     dy: float
     cmx: float
     cmy: float
+    rdx: float = 0.0   # fp32(1/dx) of the double dx: the multiplier of the "reciprocal" division
+    rdy: float = 0.0
 
     @classmethod
     def from_grid(cls, dt: float, dx: float, dy: float, cmx: float | None = None, cmy: float | None = None):
@@ -123,7 +140,8 @@ class StepCoefficients:
             cmx = (C0 * dt - dx) / (C0 * dt + dx)
         if cmy is None:
             cmy = (C0 * dt - dy) / (C0 * dt + dy)
-        return cls(f32(dt / MU0), f32(dt / EPS0), f32(dx), f32(dy), f32(cmx), f32(cmy))
+        return cls(f32(dt / MU0), f32(dt / EPS0), f32(dx), f32(dy), f32(cmx), f32(cmy),
+                   f32(1.0 / dx), f32(1.0 / dy))
 
 
 class CellSource:
@@ -267,6 +285,7 @@ class RunState:
     third: Optional[tuple] = None    # (ex3, ey3, hz3): lets a blocked run use an odd launch count
     status: Optional[torch.Tensor] = None  # device uint32 word: WF_STATUS_* bits the kernels raise
     step_end: int = 0                      # 0, or the step the blocked kernel stops the run at
+    reciprocal: bool = False               # quotients as x * fp32(1/d) (division "reciprocal"), else IEEE
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self) -> None:
@@ -296,6 +315,11 @@ class RunState:
         r.nx, r.ny = self.nx, self.ny
         c = self.coef
         r.dh, r.de, r.dx, r.dy, r.cmx, r.cmy = c.dh, c.de, c.dx, c.dy, c.cmx, c.cmy
+        if self.reciprocal:
+            if not (c.rdx > 0.0 and c.rdy > 0.0):
+                raise ValueError("a run with reciprocal division needs its coefficients' rdx, rdy")
+            r.rdx, r.rdy = c.rdx, c.rdy
+            r.flags = _native.WF_RUN2D_RECIPROCAL
         r.step = self.step
         if self.status is not None:
             r.status = self.status.data_ptr()
@@ -505,7 +529,12 @@ class TBPlan:
         self._tiles: dict[int, tuple[torch.Tensor, int]] = {}
         self._lists: dict = {}
         self.div = None
-        if fast_div:
+        recips = {bool(r.reciprocal) for r in runs}
+        if len(recips) != 1:
+            raise ValueError("temporally blocked launch needs all runs with the same division mode")
+        self.reciprocal = recips.pop()
+        self.div_mode = _native.WF_DIV_RECIPROCAL if self.reciprocal else _native.WF_DIV_IEEE
+        if fast_div and not self.reciprocal:
             dxs = {r.coef.dx for r in runs}
             dys = {r.coef.dy for r in runs}
             if len(dxs) == 1 and len(dys) == 1:
@@ -513,6 +542,7 @@ class TBPlan:
                 vy = verified_division(dys.pop(), device)
                 if vx is not None and vy is not None:
                     self.div = (vx, vy)
+                    self.div_mode = _native.WF_DIV_FAST
         self.n_cells = sum(r.nx * r.ny for r in runs)
 
     @property
@@ -661,7 +691,7 @@ class TBPlan:
             a.tile_lists, a.grp_list, a.det_list = lists.data_ptr(), grp.data_ptr(), det.data_ptr()
             if self.div is not None:
                 (a.rdx, a.div_lo_x, a.div_span_x), (a.rdy, a.div_lo_y, a.div_span_y) = self.div
-                a.fast_div = 1
+            a.fast_div = self.div_mode
             self._args[key] = a
         return a
 

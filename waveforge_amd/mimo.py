@@ -37,6 +37,7 @@ from .engine import (
     SourceTable,
     StepCoefficients,
     TBPlan,
+    check_division,
     judge,
     runs_add_step,
     schedule_launches,
@@ -61,11 +62,15 @@ class RunSpec:
 
 
 class BatchedRuns:
-    """Device state and launch plan of a set of independent runs."""
+    """Device state and launch plan of a set of independent runs.
+
+    ``division`` is how every run divides by dx, dy (``"reciprocal"``, as the
+    reference computes on a GPU, or ``"ieee"``, as on the CPU; see
+    ``core.fdtd2d``)."""
 
     def __init__(self, specs: Sequence[RunSpec], *, dx: float, dy: float, dt: float, device=None,
                  tile_rows=None, depth=None, n_check: int = 500, stability_threshold: float = 1e10,
-                 fast_div: bool = True) -> None:
+                 fast_div: bool = True, division: str = "reciprocal") -> None:
         if not specs:
             raise ValueError("no runs")
         self.device = torch.device(device) if device is not None else torch.device("cuda")
@@ -76,6 +81,7 @@ class BatchedRuns:
         self.dt, self.dx, self.dy = dt, dx, dy
         self.tile_rows, self.depth = tile_rows, depth
         self.n_check, self.threshold = n_check, stability_threshold
+        self.division = check_division(division)
         coef = StepCoefficients.from_grid(dt, dx, dy)
         self.runs: list[RunState] = []
         dev = self.device
@@ -106,7 +112,8 @@ class BatchedRuns:
             srcs = [CellSource(c, w, i, j, a) for (c, i, j, w, a) in sp.sources]
             table = SourceTable(srcs, sp.nx, sp.ny, dev, upload=False)
             tables.append(table)
-            rs = RunState(f[0], f[1], f[2], sp.ca, sp.cb, sp.nx, sp.ny, coef, sources=table)
+            rs = RunState(f[0], f[1], f[2], sp.ca, sp.cb, sp.nx, sp.ny, coef, sources=table,
+                          reciprocal=self.division == "reciprocal")
             rs.second = (f[3], f[4], f[5])
             if sp.receivers:
                 nd = len(sp.receivers)
@@ -275,7 +282,7 @@ def mimo_specs(ca: torch.Tensor, cb: torch.Tensor, antennas, wave: torch.Tensor,
 
 
 def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, depth=None,
-             tile_rows=None, n_check: int = 500, return_tensor: bool = False):
+             tile_rows=None, n_check: int = 500, return_tensor: bool = False, division: str = "reciprocal"):
     """All TX runs of one array in one batched launch sequence.
 
     Same inputs and output as the reference's ``run_mimo``: returns the
@@ -290,7 +297,7 @@ def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, de
     wave = waveform.build(n_steps, grid.dt, grid.device, torch.float32)
     specs = mimo_specs(Ca, Cb, antenna_positions, wave, n_steps)
     br = BatchedRuns(specs, dx=grid.dx, dy=grid.dy, dt=grid.dt, device=grid.device, tile_rows=tile_rows,
-                     depth=depth, n_check=n_check)
+                     depth=depth, n_check=n_check, division=division)
     br.advance(n_steps)
     st = br.status()
     if any(st["diverged"]):
@@ -304,7 +311,7 @@ def run_mimo(grid, Ca, Cb, antenna_positions, n_steps: int, waveform=None, *, de
 
 def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, device, depth=None,
                    tile_rows=None, n_check: int = 500, v_medium: float | None = None, gate_start: int = 0,
-                   use_graph: bool = False, chunk: int = 240):
+                   use_graph: bool = False, chunk: int = 240, division: str = "reciprocal"):
     """Batched MIMO imaging of many phantoms: all TX runs of all phantoms in
     one batched launch sequence, then the DAS images of all phantoms together
     (``wf_das2d_batch``; raw traces).
@@ -333,7 +340,7 @@ def image_phantoms(phantoms, n_steps: int, *, dx: float = 1e-3, dt: float, devic
         specs += sp
         owners += [p] * len(sp)
     br = BatchedRuns(specs, dx=dx, dy=dx, dt=dt, device=device, tile_rows=tile_rows, depth=depth,
-                     n_check=n_check)
+                     n_check=n_check, division=division)
     n_chunks, rest = divmod(n_steps, chunk) if use_graph else (0, n_steps)
     if n_chunks and rest == 1:
         n_chunks, rest = n_chunks - 1, rest + chunk

@@ -39,6 +39,7 @@ from .engine import (
     SourceTable,
     StepCoefficients,
     TBPlan,
+    check_division,
     judge,
     runs_add_step,
     tb_fits_any_tile,
@@ -93,10 +94,12 @@ class ImagingServer:
 
     def __init__(self, *, dt: float, dx: float = 1e-3, device=None, hbm_budget: float = 64e9, quantum: int = 192,
                  depth: int = 8, tile_rows: int = 48, max_requests: Optional[int] = None,
-                 stability_threshold: float = 1e10, v_medium: Optional[float] = None) -> None:
+                 stability_threshold: float = 1e10, v_medium: Optional[float] = None,
+                 division: str = "reciprocal") -> None:
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         if self.device.type != "cuda":
             raise RuntimeError("waveforge_amd: the imaging server needs the ROCm GPU; there is no CPU path")
+        self.division = check_division(division)
         _native.load()
         self.dt, self.dx = dt, dx
         self.budget = hbm_budget
@@ -152,7 +155,8 @@ class ImagingServer:
             table = SourceTable([CellSource("Hz", wave, i, j, 1.0)], nx, ny, dev, upload=False)
             tables.append(table)
             rs = RunState(f[0], f[1], f[2], ca, cb, nx, ny, self.coef, sources=table, det_cell=cells,
-                          traces=req.traces[t], trace_n0=0, step=0, step_end=req.n_steps)
+                          traces=req.traces[t], trace_n0=0, step=0, step_end=req.n_steps,
+                          reciprocal=self.division == "reciprocal")
             rs.second = (f[3], f[4], f[5])
             rs.extra["req"] = req
             rs.extra["det_cells_np"] = cells_host  # the receivers' cells for host-side tables (no read-back)
@@ -331,7 +335,7 @@ def _stats(done, total: float, rounds: int = 0) -> dict:
 
 def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e-3, device=None,
                 realtime: bool = True, v_medium: Optional[float] = None, n_check: int = 100,
-                stability_threshold: float = 1e10) -> dict:
+                stability_threshold: float = 1e10, division: str = "reciprocal") -> dict:
     """The plain one-run-per-launch GPU path as a server, for comparison.
 
     Requests are taken one at a time in arrival order; every TX of a request
@@ -367,7 +371,7 @@ def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e
             det = DetectorSet(req.antennas, req.n_steps, grid=grid)
             src = PointSource(wave, i, j, "Hz", grid=grid, N_steps=req.n_steps)
             sim = FDTD2D(grid, f, MurABC(grid, f.Hz), SourceCollection([src]), Ca=ca, Cb=cb, detectors=det,
-                         n_check=n_check, stability_threshold=stability_threshold)
+                         n_check=n_check, stability_threshold=stability_threshold, division=division)
             try:
                 sim.run(req.n_steps)
             except SimulationDivergedError:
