@@ -85,6 +85,32 @@ __device__ __forceinline__ float lane_prev(float v) {  // value of lane-1 (0 in 
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
 
+// One grid row of one field as a raw buffer: a wave-uniform descriptor and a
+// 32-bit byte offset per lane. An offset at or past the row's end (kOut, or
+// any offset when the row is outside the grid and has 0 bytes) loads 0 and
+// stores nothing, so the loads and stores need no per-lane address
+// arithmetic or masks.
+constexpr uint32_t kOut = 0xffffffffu;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_of(const float *base, int64_t o, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base + o), 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float ld(__amdgpu_buffer_rsrc_t p, uint32_t off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(p, off, 0, 0));
+}
+__device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t p, uint32_t off, float v) {
+    // default cache policy: the next launch reads these cells (the
+    // non-temporal hint measured 0-9 % slower, most at shallow depths)
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), p, off, 0, 0);
+}
+// A pointer the whole wavefront shares, kept in scalar registers.
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T *>(((uint64_t)hi << 32) | lo);
+}
+
 using wf::Divider;
 using wf::divide;
 using wf::divide_row;
@@ -121,7 +147,7 @@ __device__ __forceinline__ void lds_put(float *p, const float (&v)[CPL]) {
 // launch on grids too small to fill the GPU (a 16-row tile holds 4 rows per
 // lane). Each lane keeps 5*CPL values per row (Ex, Ey, Hz, Ca, Cb) plus the
 // temporaries of its division mode (36 registers for reciprocal products, 40
-// for the fast quotient, 48 for IEEE division); without the cap the scheduler
+// for the fast quotient or IEEE division); without the cap the scheduler
 // hoists every row's loads and runs at one or two waves per SIMD. An
 // 8-wavefront workgroup puts two waves on each SIMD, so its occupancy is an
 // even number of waves per SIMD: 8 (64 registers), 6 (80) or 4 (128).
@@ -137,7 +163,7 @@ struct TileGeom {
     static constexpr int W = kW * CPL;
     static constexpr int RI = TI + 2 * D;
     static constexpr int NR = (RI + NW - 1) / NW;  // wave w holds rows w*NR .. w*NR + NR-1
-    static constexpr int kRegs = 5 * NR * CPL + (DIV == kDivRcp ? 36 : (DIV == kDivFast ? 40 : 48));
+    static constexpr int kRegs = 5 * NR * CPL + (DIV == kDivRcp ? 36 : 40);
     static constexpr int kWaves = kRegs <= 64 ? 8 : (kRegs <= 80 ? 6 : (kRegs <= 128 ? 4 : 2));
     static_assert(NR <= 32, "row flags are one 32-bit word per wavefront");
     static_assert(NT >= 2 * W + RI, "edge passes need a thread per face cell");
@@ -253,24 +279,32 @@ k_tb(wf_tb_args a) {
         col_in[h] = gj >= 0 && gj < ny;
     }
     // the wave's NR consecutive rows (rows past the region and outside the grid hold 0)
+    uint32_t coff[CPL];  // byte offset of the lane's cells in a row, kOut outside the grid's columns
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) coff[h] = col_in[h] ? 4u * (uint32_t)(gj0 + c0 + h) : kOut;
+    const float *p_ex = uniform_ptr(src_ex), *p_ey = uniform_ptr(src_ey), *p_hz = uniform_ptr(src_hz);
+    const float *p_ca = MAT ? uniform_ptr(R.ca) : nullptr, *p_cb = MAT ? uniform_ptr(R.cb) : nullptr;
     float ex[NR][CPL], ey[NR][CPL], hz[NR][CPL], ca[NR][CPL], cb[NR][CPL];
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const int li = r0 + k;
-        const int gi = gi0 + li;
+        const int gi = gi0 + r0 + k;
+        const bool in = r0 + k < RI && gi >= 0 && gi < nx;
+        const int64_t ro = in ? (int64_t)gi * ny : 0;
+        const int nb = in ? 4 * ny : 0;
+        const auto rx = row_of(p_ex, ro, nb), ry = row_of(p_ey, ro, nb), rh = row_of(p_hz, ro, nb);
 #pragma unroll
         for (int h = 0; h < CPL; ++h) {
-            ex[k][h] = ey[k][h] = hz[k][h] = 0.f;
+            ex[k][h] = ld(rx, coff[h]);
+            ey[k][h] = ld(ry, coff[h]);
+            hz[k][h] = ld(rh, coff[h]);
             ca[k][h] = cb[k][h] = 0.f;
-            if (li < RI && gi >= 0 && gi < nx && col_in[h]) {
-                const size_t g = (size_t)gi * ny + (gj0 + c0 + h);
-                ex[k][h] = src_ex[g];
-                ey[k][h] = src_ey[g];
-                hz[k][h] = src_hz[g];
-                if (MAT) {
-                    ca[k][h] = R.ca[g];
-                    cb[k][h] = R.cb[g];
-                }
+        }
+        if (MAT) {
+            const auto ra = row_of(p_ca, ro, nb), rb = row_of(p_cb, ro, nb);
+#pragma unroll
+            for (int h = 0; h < CPL; ++h) {
+                ca[k][h] = ld(ra, coff[h]);
+                cb[k][h] = ld(rb, coff[h]);
             }
         }
     }
@@ -335,6 +369,10 @@ k_tb(wf_tb_args a) {
     bool lane_core[CPL];
 #pragma unroll
     for (int h = 0; h < CPL; ++h) lane_core[h] = c0 + h >= D && c0 + h < D + TJ && col_in[h];
+    uint32_t soff[CPL];  // byte offset of the lane's core cells in a row, kOut elsewhere
+#pragma unroll
+    for (int h = 0; h < CPL; ++h) soff[h] = lane_core[h] ? 4u * (uint32_t)(gj0 + c0 + h) : kOut;
+    float *q_ex = uniform_ptr(dst_ex), *q_ey = uniform_ptr(dst_ey), *q_hz = uniform_ptr(dst_hz);
 
     // Faraday at row k from its registers and the Ey row below (eyd):
     // Hz += Dh*((Ex(j+1) - Ex(j))/dy - (Ey(i+1) - Ey(i))/dx), every lane (on
@@ -608,19 +646,20 @@ k_tb(wf_tb_args a) {
             }
         }
         if (HIST) {
+            // the state after step s into its history slot (the written state + (s-1)*hist_stride)
             const int64_t hoff = (int64_t)(s - 1) * a.hist_stride;
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
-                const int li = r0 + k;
-                const int gi = gi0 + li;
+                const int gi = gi0 + r0 + k;
+                const bool core = r0 + k >= D && r0 + k < D + TI && gi < nx;
+                const int64_t ro = core ? hoff + (int64_t)gi * ny : 0;
+                const int nb = core ? 4 * ny : 0;
+                const auto rx = row_of(q_ex, ro, nb), ry = row_of(q_ey, ro, nb), rh = row_of(q_hz, ro, nb);
 #pragma unroll
                 for (int h = 0; h < CPL; ++h) {
-                    if (li >= D && li < D + TI && gi < nx && lane_core[h]) {
-                        const int64_t g = hoff + (int64_t)gi * ny + (gj0 + c0 + h);
-                        dst_ex[g] = ex[k][h];
-                        dst_ey[g] = ey[k][h];
-                        dst_hz[g] = hz[k][h];
-                    }
+                    st(rx, soff[h], ex[k][h]);
+                    st(ry, soff[h], ey[k][h]);
+                    st(rh, soff[h], hz[k][h]);
                 }
             }
         }
@@ -628,25 +667,36 @@ k_tb(wf_tb_args a) {
     if (HIST) return;  // the last step's state is in its history slot
 
     // Write the core back (and fold the stability maxima if requested).
-    unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         const int li = r0 + k;
         const int gi = gi0 + li;
+        const bool core = li >= D && li < D + TI && gi < nx;
+        const int64_t ro = core ? (int64_t)gi * ny : 0;
+        const int nb = core ? 4 * ny : 0;
+        const auto rx = row_of(q_ex, ro, nb), ry = row_of(q_ey, ro, nb), rh = row_of(q_hz, ro, nb);
 #pragma unroll
         for (int h = 0; h < CPL; ++h) {
-            if (li >= D && li < D + TI && gi < nx && lane_core[h]) {
-                const size_t g = (size_t)gi * ny + (gj0 + c0 + h);
-                dst_ex[g] = ex[k][h];
-                dst_ey[g] = ey[k][h];
-                dst_hz[g] = hz[k][h];
-                m0 = max(m0, wf::abs_bits(ex[k][h]));
-                m1 = max(m1, wf::abs_bits(ey[k][h]));
-                m2 = max(m2, wf::abs_bits(hz[k][h]));
-            }
+            st(rx, soff[h], ex[k][h]);
+            st(ry, soff[h], ey[k][h]);
+            st(rh, soff[h], hz[k][h]);
         }
     }
     if (a.maxabs_out) {
+        unsigned int m0 = 0u, m1 = 0u, m2 = 0u;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const int li = r0 + k;
+            const int gi = gi0 + li;
+#pragma unroll
+            for (int h = 0; h < CPL; ++h) {
+                if (li >= D && li < D + TI && gi < nx && lane_core[h]) {
+                    m0 = max(m0, wf::abs_bits(ex[k][h]));
+                    m1 = max(m1, wf::abs_bits(ey[k][h]));
+                    m2 = max(m2, wf::abs_bits(hz[k][h]));
+                }
+            }
+        }
         atomicMax(&s_max[0], m0);
         atomicMax(&s_max[1], m1);
         atomicMax(&s_max[2], m2);
