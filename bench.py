@@ -7,6 +7,9 @@ detectors on a radius-1800-cell circle — 4000 timed steps after a warm-up, run
 through ``FDTD2D.run`` exactly as a user would (stability check every 100
 steps included). A "step" is one leapfrog step of the whole grid; the value is
 cell-updates per second over the timed steps, with all inputs resident in HBM.
+The run divides by the cell sizes as the reference's expressions do under torch
+on a GPU (``division="reciprocal"``, the default of ``FDTD2D``); ``ieee_division``
+reports the same timing for the form the reference computes on the CPU.
 
 ``roofline``: the dominant kernel is the temporally blocked stepper. Its
 algorithmic traffic is 32 B per cell-update (read Ex, Ey, Hz, Ca, Cb, write
@@ -53,6 +56,10 @@ def parse():
     p.add_argument("--depth", type=int, default=None)
     p.add_argument("--tile", type=int, default=None, help="tile rows (tile columns are 64 - 2*depth)")
     p.add_argument("--autotune", action="store_true", help="let the autotuner pick tile rows and depth")
+    p.add_argument("--division", default="reciprocal", choices=("reciprocal", "ieee"),
+                   help="how the run divides by dx, dy: as torch on a GPU (default) or on the CPU")
+    p.add_argument("--no-ieee-compare", action="store_true",
+                   help="skip timing the IEEE-division form beside the headline")
     p.add_argument("--cpu-steps", type=int, default=8, help="timed steps of the CPU baseline sample")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-imaging", action="store_true", help="skip the batched-imaging section")
@@ -697,6 +704,33 @@ def serving_full() -> int:
     return 0
 
 
+def ieee_compare(dev, args, shape) -> dict:
+    """The headline workload with ``division="ieee"`` (the quotient the
+    reference computes on the CPU, through the verified fast quotient): the
+    same warm-up, then the same number of timed steps (at most 800), same
+    launch shape."""
+    import torch
+
+    from waveforge_amd import workloads
+
+    steps = min(args.steps, 800)
+    wl = workloads.blob_medium(dev, n=args.size, n_steps=args.warmup + steps, division="ieee",
+                               tile=(shape[0], shape[2]), depth=shape[1])
+    wl.sim.run(args.warmup)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    wl.sim.run(steps)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    res = {"division": "ieee", "steps": steps, "ms_per_step": round(ms / steps, 5),
+           "mcell_updates_per_s": round(steps * args.size * args.size / (ms / 1e3) / 1e6, 1)}
+    del wl
+    torch.cuda.empty_cache()
+    return res
+
+
 def main() -> int:
     args = parse()
     if args.copy_probe:
@@ -725,7 +759,7 @@ def main() -> int:
         kw["depth"], kw["tile"] = "auto", "auto"
     total = args.warmup + args.steps
     print(f"[bench] building {args.size}x{args.size} blob medium ({total} steps)", file=sys.stderr, flush=True)
-    wl = workloads.blob_medium(dev, n=args.size, n_steps=total, **kw)
+    wl = workloads.blob_medium(dev, n=args.size, n_steps=total, division=args.division, **kw)
     sim = wl.sim
     # the timing events of the timed region are created and recorded once
     # before it (the first record of a fresh HIP event costs far more than a
@@ -802,7 +836,8 @@ def main() -> int:
     if traffic_file.exists():
         tinfo = json.loads(traffic_file.read_text())
         by_depth = {int(k): v for k, v in tinfo.get("hbm_bytes_by_depth", {}).items()}
-        if (tinfo.get("grid"), tinfo.get("tile_rows"), tinfo.get("cpl", 1)) == shape and dominant in by_depth:
+        if (tinfo.get("grid"), tinfo.get("tile_rows"), tinfo.get("cpl", 1)) == shape and dominant in by_depth \
+                and tinfo.get("division") == args.division:
             roofline["traffic"] = by_depth[dominant]
             roofline["traffic_source"] = (f"profiles/{tinfo['summary']} (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                           f"copy-kernel calibrated, per depth-{dominant} launch)")
@@ -814,10 +849,15 @@ def main() -> int:
     if valu_file.exists():
         vinfo = json.loads(valu_file.read_text())
         by_depth = vinfo.get("by_depth", {})
-        if (vinfo.get("grid"), vinfo.get("tile_rows"), vinfo.get("cpl", 1)) == shape and str(dominant) in by_depth:
+        if (vinfo.get("grid"), vinfo.get("tile_rows"), vinfo.get("cpl", 1)) == shape and str(dominant) in by_depth \
+                and vinfo.get("division") == args.division:
             roofline["valu"] = dict(by_depth[str(dominant)], depth=dominant, source="profiles/" + vinfo["summary"])
     tr = wl.detectors.traces
     finite = bool(torch.isfinite(tr).all()) and bool(torch.isfinite(sim.fields.Hz).all())
+    ieee = None
+    if args.division == "reciprocal" and not args.no_ieee_compare:
+        print("[bench] the same run with IEEE division", file=sys.stderr, flush=True)
+        ieee = ieee_compare(dev, args, sim.launch_shape)
     base = das_res = slab_res = None
     dcase = das_case()
     if not args.no_das:
@@ -861,12 +901,14 @@ def main() -> int:
             "tile_rows": tile_rows,
             "depth": depth,
             "cols_per_lane": cpl,
+            "division": args.division,
         },
         "roofline": roofline,
         "autotune": {"choice": [tuned.tile_rows, tuned.depth, tuned.cpl], "used": [tile_rows, depth, cpl],
                      "rates_mcell_per_s": tuned.rates},
         "cpu_baseline": base,
         "fields_finite": finite,
+        "ieee_division": ieee,
         "das": das_res,
         "slab256": slab_res,
         "imaging": imaging,

@@ -182,8 +182,12 @@ k_tb(wf_tb_args a) {
     constexpr int kNT = G::NT;
     constexpr int NR = G::NR;
     __shared__ __align__(16) float s_hz[RI * W];  // edge tiles and tiles with detectors: every row's Hz
-    __shared__ __align__(16) float s_xey[(NW + 1) * W];  // entry w: Ey of wave w's first row (read by wave w-1)
-    __shared__ __align__(16) float s_xhz[(NW + 1) * W];  // entry w+1: Hz of wave w's last row (read by wave w+1)
+    // row exchange between the waves' row blocks, one array so every access is
+    // one base register plus a constant: [0, XH) holds at entry w the Ey of
+    // wave w's first row (read by wave w-1), [XH, 2 XH) at entry w+1 the Hz of
+    // wave w's last row (read by wave w+1)
+    constexpr int XH = (NW + 1) * W;
+    __shared__ __align__(16) float s_x[2 * XH];
     __shared__ int s_grp[kMaxGroups];
     __shared__ int s_grp_cell[kMaxGroups];
     __shared__ int s_det[kMaxDets];
@@ -443,10 +447,12 @@ k_tb(wf_tb_args a) {
             if (q2 >= -1 && q2 + 1 < W) s_fcol[1][li] = s_hz[li * W + q2 + 1];
         }
     };
+    float *const px = &s_x[w * W + c0];       // the wave's exchange entries
+    float *const hrow = &s_hz[r0 * W + c0];   // the wave's first Hz row in LDS
     if (edge) {  // the first step's Mur snapshot
 #pragma unroll
         for (int k = 0; k < NR; ++k)
-            if (r0 + k < RI) lds_put<CPL>(&s_hz[(r0 + k) * W + c0], hz[k]);
+            if (r0 + k < RI) lds_put<CPL>(&hrow[k * W], hz[k]);
         __syncthreads();
         edge_snapshot();
     }
@@ -495,7 +501,7 @@ k_tb(wf_tb_args a) {
         }
         // 2. The Ey of each wave's first row, which the wave above reads for
         // its last row's Faraday (after the sources: Faraday reads injected Ey).
-        lds_put<CPL>(&s_xey[w * W + c0], ey[0]);
+        lds_put<CPL>(&px[0], ey[0]);
         __syncthreads();
         // 3. Faraday on every row; its values on the face rows and columns
         // are replaced by the Mur pass below, and rows outside the valid
@@ -507,16 +513,16 @@ k_tb(wf_tb_args a) {
 #pragma unroll
                 for (int h = 0; h < CPL; ++h) eyd[h] = ey[k + 1][h];
             } else {
-                lds_get<CPL>(&s_xey[(w + 1) * W + c0], eyd);
+                lds_get<CPL>(&px[W], eyd);
             }
             faraday(k, eyd);
         }
         if (mirror) {  // edge tiles (the Mur pass) and detectors read Hz rows from LDS
 #pragma unroll
             for (int k = 0; k < NR; ++k)
-                if (r0 + k < RI) lds_put<CPL>(&s_hz[(r0 + k) * W + c0], hz[k]);
+                if (r0 + k < RI) lds_put<CPL>(&hrow[k * W], hz[k]);
         }
-        lds_put<CPL>(&s_xhz[(w + 1) * W + c0], hz[NR - 1]);
+        lds_put<CPL>(&px[XH + W], hz[NR - 1]);
         __syncthreads();
         if (edge) {
             // Mur (src/core/boundaries.py:259-274) on the faces next to this
@@ -563,7 +569,7 @@ k_tb(wf_tb_args a) {
             // every other cell, so whole rows are reloaded
 #pragma unroll
             for (int k = 0; k < NR; ++k)
-                if (r0 + k < RI) lds_get<CPL>(&s_hz[(r0 + k) * W + c0], hz[k]);
+                if (r0 + k < RI) lds_get<CPL>(&hrow[k * W], hz[k]);
             // the next step's snapshot: LDS holds this step's final Hz, which
             // nothing rewrites before the next Faraday (Ampere and the
             // detectors only read it; the next step's first barrier orders
@@ -613,7 +619,7 @@ k_tb(wf_tb_args a) {
             } else if (edge) {  // the row above, after the Mur pass
                 lds_get<CPL>(&s_hz[max(r0 - 1, 0) * W + c0], hzu);
             } else {
-                lds_get<CPL>(&s_xhz[w * W + c0], hzu);
+                lds_get<CPL>(&px[XH], hzu);
             }
             ampere(k, hzu);
         }
