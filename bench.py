@@ -404,18 +404,17 @@ def serving_suite(dev, n_requests: int) -> dict:
     with grids log-uniform in 128^2..1024^2, E in {8, 16, 32} elements (every
     element transmits, all receive), steps = 8 x grid diagonal, DAS each.
 
-    * Continuous-batching server: capacity from a 48-request burst (all at
-      t = 0), then a Poisson stream of ``n_requests`` at 80 % of that
-      capacity (p50/p99 latency, arrival to DAS image ready).
-    * The same server holding one request at a time, on the first 24 of the
-      burst (what continuous batching adds on its own).
-    * The plain one-run-per-launch GPU path (each TX its own FDTD2D.run, one
-      request at a time): capacity from the first 12 of the burst; then the
-      first 16 arrivals of the continuous server's own stream (equal arrival
-      rate: the plain path cannot keep up, so its queue and p99 grow), and 16
-      arrivals at 80 % of its own capacity (a rate it sustains).
-    Throughput is compared in cell-updates/s, which weighs the mix's grid
-    sizes the same on both sides."""
+    * Capacity, every path on the same 48-request burst (all at t = 0): the
+      continuous-batching server; the same server holding one request at a
+      time (what continuous batching adds on its own); the plain
+      one-run-per-launch GPU path (each TX its own FDTD2D.run, one request at
+      a time).
+    * Sustained: a Poisson stream of ``n_requests`` at 80 % of the continuous
+      server's capacity (p50/p99 latency, arrival to DAS image ready); the
+      plain path on the first 16 arrivals of the same stream (equal arrival
+      rate: it cannot keep up, so its queue and p99 grow), and on 16 arrivals
+      at 80 % of its own capacity (a rate it sustains).
+    Throughput is compared in cell-updates/s."""
     import torch
 
     from waveforge_amd import serving, workloads
@@ -443,15 +442,14 @@ def serving_suite(dev, n_requests: int) -> dict:
     torch.cuda.synchronize()
     log("capacity: 48-request burst, continuous")
     cap_c = server().serve(reqs(48, 1.0, 11, burst=True), realtime=False)
-    log("capacity: first 24 of the burst, one request at a time")
-    cap_1 = server(1).serve(reqs(48, 1.0, 11, burst=True)[:24], realtime=False)
-    log("capacity: first 12 of the burst, plain path")
-    burst_p = reqs(48, 1.0, 11, burst=True)[:12]
+    log("capacity: the same burst, one request at a time")
+    burst_1 = reqs(48, 1.0, 11, burst=True)
+    cap_1 = server(1).serve(burst_1, realtime=False)
+    log("capacity: the same burst, plain path")
+    burst_p = reqs(48, 1.0, 11, burst=True)
     cap_p = serving.serve_plain(burst_p, dt=dt, device=dev, realtime=False)
     same = True
-    ref = {q.rid: q for q in reqs(48, 1.0, 11, burst=True)[:12]}
-    srv_ref = server()
-    srv_ref.serve(list(ref.values()), realtime=False)
+    ref = {q.rid: q for q in burst_1}
     for q in burst_p:
         same = same and torch.equal(q.image, ref[q.rid].image) and torch.equal(q.traces, ref[q.rid].traces)
     rate_c = 0.8 * cap_c["images_per_s"]
@@ -466,7 +464,7 @@ def serving_suite(dev, n_requests: int) -> dict:
     return {
         "workload": f"head phantoms, grid 128..1024 log-uniform, E in {{8,16,32}}, steps = 8*diag, DAS per request; "
                     f"sustained stream of {n_requests} Poisson requests",
-        "capacity": {"continuous_48": cap_c, "one_request_at_a_time_24": cap_1, "plain_run_per_launch_12": cap_p},
+        "capacity": {"continuous_48": cap_c, "one_request_at_a_time_48": cap_1, "plain_run_per_launch_48": cap_p},
         "capacity_ratio_vs_plain": round(cap_c["mcell_updates_per_s"] / cap_p["mcell_updates_per_s"], 2),
         "capacity_ratio_vs_one_at_a_time": round(cap_c["mcell_updates_per_s"] / cap_1["mcell_updates_per_s"], 2),
         "sustained": {"continuous": dict(sus_c, offered_images_per_s=rate_c),
@@ -652,8 +650,9 @@ def serving_full() -> int:
     phantoms, grid log-uniform in 128..1024, E in {8, 16, 32}, steps = 8 x grid
     diagonal, DAS each), Poisson arrivals at 80 % of the continuous server's
     capacity, served in real time. Capacity comes from a 64-request burst of
-    the same mix (all at t = 0); the plain one-run-per-launch path's capacity
-    from the first 12 of those (its full stream would take hours)."""
+    the same mix (all at t = 0); the capacity ratio against the plain
+    one-run-per-launch path is taken on the first 12 of those, served by both
+    paths (the plain path's full stream would take hours)."""
     import torch
 
     from waveforge_amd import serving, workloads
@@ -684,6 +683,11 @@ def serving_full() -> int:
     log("plain path: 12 requests of the burst")
     cap_p = serving.serve_plain(plain, dt=dt, device=dev, realtime=False)
     log(f"plain {cap_p['mcell_updates_per_s'] / 1e3:.1f} G cell-updates/s")
+    same12 = serving.poisson_requests(64, 1.0, n_min=128, n_max=1024, seed=11)[:12]
+    for q in same12:
+        q.arrival = 0.0
+    cap12 = server().serve(same12, realtime=False)
+    log(f"continuous on the same 12: {cap12['mcell_updates_per_s'] / 1e3:.1f} G cell-updates/s")
     rate = 0.8 * cap["images_per_s"]
     stream = serving.poisson_requests(2000, rate, n_min=128, n_max=1024, seed=7)
     work = sum(q.cell_updates for q in stream)
@@ -697,9 +701,10 @@ def serving_full() -> int:
             "cell_updates": work,
             "capacity_burst_64": cap,
             "plain_run_per_launch_12": cap_p,
+            "continuous_same_12": cap12,
             "offered_images_per_s": rate,
             "sustained": sus,
-            "capacity_ratio_vs_plain": round(cap["mcell_updates_per_s"] / cap_p["mcell_updates_per_s"], 2),
+            "capacity_ratio_vs_plain_same_12": round(cap12["mcell_updates_per_s"] / cap_p["mcell_updates_per_s"], 2),
         }}))
     return 0
 

@@ -203,10 +203,11 @@ def test_run_mimo_matches_oracle_per_tx(device):
 
 
 def test_default_launch_shape_follows_the_batch_size(device):
-    """With no shape given, a batch runs 48-row tiles 8 steps deep, or (when
-    it makes fewer than 64 tiles of 48 x 48 cells) 8-row tiles while there are
-    at most 288 of them, else 16-row tiles; one of the two given takes the
-    other from that default."""
+    """With no shape given, a batch runs 8 steps deep with the tile height of
+    FDTD2D's rule over the whole set (32-row tiles below 1000 tiles of 48 x 48
+    cells, 64-row tiles above; 8-row tiles while there are at most 288 of
+    them, else 16-row tiles below 64 tiles of 48 rows); one of the two given
+    takes the other from that default."""
     from waveforge_amd.mimo import BatchedRuns, RunSpec
 
     dt = stable_dt(DX, DX)
@@ -218,13 +219,14 @@ def test_default_launch_shape_follows_the_batch_size(device):
         return BatchedRuns(specs, dx=DX, dy=DX, dt=dt, device=device, **kw)
 
     big = batch(256, 16)     # 16 x 6 x 6 = 576 tiles of 48 rows, 48 columns
-    assert (big.tile_rows, big.depth) == (48, 8)
+    assert (big.tile_rows, big.depth) == (32, 8)
+    assert batch(256, 32).tile_rows == 64   # 1152 such tiles
     small = batch(128, 2)    # 2 x 3 x 3 = 18 such tiles, 2 x 16 x 3 = 96 of 8 rows
     assert (small.tile_rows, small.depth) == (8, 8)
     mid = batch(300, 1)      # 7 x 7 = 49 such tiles, 38 x 7 = 266 of 8 rows
     assert (mid.tile_rows, mid.depth) == (8, 8)
     assert batch(330, 1).tile_rows == 16  # 7 x 7 = 49 tiles of 48 rows, 42 x 7 = 294 of 8 rows
-    assert (batch(256, 16, depth=6).tile_rows, batch(256, 16, tile_rows=32).depth) == (48, 8)
+    assert (batch(256, 16, depth=6).tile_rows, batch(256, 16, tile_rows=48).depth) == (32, 8)
     big.advance(10)
     small.advance(10)
     assert big.steps_completed == small.steps_completed == 10

@@ -211,9 +211,10 @@ def test_snapshot_free_fused_step_condition():
 
 
 def test_default_launch_shape_by_workload():
-    """48-row tiles 8 steps deep; for sets of fewer than 64 such tiles, 8-row
-    tiles while there are at most 288 of them, else 16-row tiles; smaller
-    tiles where a run's detectors crowd one tile."""
+    """8 steps deep; 8-row tiles while there are at most 288 of them, else
+    16-row tiles below 64 tiles of 48 rows, 32-row tiles below 1000 of them,
+    then 64-row tiles (48-row for a run of 2^23 cells or more); smaller tiles
+    where a run's detectors crowd one tile."""
     from types import SimpleNamespace
 
     import numpy as np
@@ -225,19 +226,22 @@ def test_default_launch_shape_by_workload():
     def runs(*shapes, det=None):
         return [SimpleNamespace(nx=a, ny=b, sources=None, det_cell=det, extra={}) for a, b in shapes]
 
-    assert default_launch_shape(runs((4096, 4096))) == (48, 8, 1)   # 86 x 86 tiles
-    assert default_launch_shape(runs((1024, 1024))) == (48, 8, 1)   # 22 x 22 tiles
-    assert default_launch_shape(runs(*[(256, 256)] * 16)) == (48, 8, 1)   # 36 tiles each
+    assert default_launch_shape(runs((4096, 4096))) == (48, 8, 1)   # 86 x 86 tiles, 2^24 cells
+    assert default_launch_shape(runs((2048, 2048))) == (64, 8, 1)   # 43 x 43 tiles, 2^22 cells
+    assert default_launch_shape(runs((1024, 1024))) == (32, 8, 1)   # 22 x 22 tiles
+    assert default_launch_shape(runs(*[(256, 256)] * 16)) == (32, 8, 1)   # 36 tiles each
+    assert default_launch_shape(runs(*[(256, 256)] * 64)) == (64, 8, 1)   # 2304 tiles
     assert default_launch_shape(runs((256, 256))) == (8, 8, 1)   # 32 x 6 = 192 tiles of 8 rows
     assert default_launch_shape(runs((128, 128), (200, 120))) == (8, 8, 1)   # 48 + 75 such tiles
     assert default_launch_shape(runs((300, 380))) == (16, 8, 1)   # 38 x 8 = 304 > 288; 7 x 8 = 56 < 64
-    assert default_launch_shape(runs((384, 384))) == (48, 8, 1)   # 8 x 8 = 64 tiles of 48 rows
-    # a 9 x 9 block of detectors (81 > 64 per tile) in one 48-row tile, split
-    # 4 + 5 rows by the 32-row tile boundary at row 160
+    assert default_launch_shape(runs((384, 384))) == (32, 8, 1)   # 8 x 8 = 64 tiles of 48 rows
+    # a 9 x 9 block of detectors (81 > 64 per tile) inside one 32-row tile at
+    # every depth (rows 140-148), split 4 + 5 rows by the 16-row tile boundary
+    # at row 144: (32, 8) and (32, 6) are refused, (16, 6) is taken
     side = int(np.sqrt(TB_MAX_DETS_PER_TILE)) + 1
-    cells = [(156 + a) * 1024 + 100 + b for a in range(side) for b in range(side)]
+    cells = [(140 + a) * 1024 + 100 + b for a in range(side) for b in range(side)]
     det = torch.tensor(cells, dtype=torch.int32)
-    assert default_launch_shape(runs((1024, 1024), det=det)) == (32, 6, 1)
+    assert default_launch_shape(runs((1024, 1024), det=det)) == (16, 6, 1)
 
 
 def test_grids_beyond_int32_cells_are_rejected():

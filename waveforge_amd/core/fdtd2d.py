@@ -67,6 +67,10 @@ SMALL_SET_TILE_ROWS = 16
 SMALL_SET_MAX_TILES = 64
 TINY_SET_TILE_ROWS = 8
 TINY_SET_MAX_TILES = 288
+MID_SET_TILE_ROWS = 32
+MID_SET_MAX_TILES = 1000
+LARGE_SET_TILE_ROWS = 64
+HUGE_RUN_CELLS = 1 << 23
 FALLBACK_SHAPES = ((32, 6, 1), (16, 6, 1))
 
 
@@ -81,19 +85,24 @@ def tile_shape(tile) -> tuple[int, int]:
 
 
 def default_launch_shape(runs) -> tuple[int, int, int]:
-    """(tile rows, depth, columns per lane) when none is given.
+    """(tile rows, depth, columns per lane) when none is given: always 8
+    steps deep, the tile height from the number n48 of 48-row tiles (48
+    columns) the run set makes, following the autotuner's tables on the
+    MI355X (single grids of 128^2 to 4096^2, 16 to 64 runs of 150^2 to
+    384^2):
 
-    48-row tiles (8-wavefront workgroups at 6 waves per SIMD, a small halo
-    share) 8 steps deep: on every run set timed on the MI355X (single grids
-    from 128^2 to 4096^2, 16 to 64 runs of 128^2 to 512^2) within 10 % of the
-    autotuner's best shape and 10-36 % ahead of 32-row tiles 6 deep. Sets of
-    fewer than 64 such tiles (one grid up to about 300^2) fill the GPU better
-    with smaller tiles, also 8 deep: 8-row tiles while there are at most 288
-    of them (about one per CU: one grid up to about 300^2; 5-10 % ahead of
-    16-row tiles for 1000 steps of 128^2 to 300^2), else 16-row tiles (ahead
-    from 384^2). If a run has more sources or detectors in one tile than the
-    kernel stages, smaller tiles are tried before the caller falls back to
-    the per-step path."""
+    * at most 288 tiles of 8 rows (one grid up to about 300^2): 8-row tiles,
+      which spread a small grid over the most CUs; else, n48 < 64: 16 rows;
+    * n48 < 1000 (one grid of 512^2 to about 1400^2, 16 runs of 256^2): 32
+      rows, 10-35 % ahead of 48;
+    * larger sets: 64 rows (fewer partial and edge tiles per grid: 7-40 %
+      ahead of 48 on 2048^2, 16 x 384^2, 64 x 256^2), except when a run holds
+      2^23 cells or more (4096^2), where 48 and 64 rows tie at depth 8 and 48
+      rows are ahead at depth 7.
+
+    If a run has more sources or detectors in one tile than the kernel
+    stages, smaller tiles are tried before the caller falls back to the
+    per-step path."""
     tj = 64 * DEFAULT_CPL - 2 * DEFAULT_DEPTH
     n = sum(-(-r.nx // DEFAULT_TILE_ROWS) * -(-r.ny // tj) for r in runs)
     n_tiny = sum(-(-r.nx // TINY_SET_TILE_ROWS) * -(-r.ny // tj) for r in runs)
@@ -101,8 +110,12 @@ def default_launch_shape(runs) -> tuple[int, int, int]:
         first = (TINY_SET_TILE_ROWS, DEFAULT_DEPTH, 1)
     elif n < SMALL_SET_MAX_TILES:
         first = (SMALL_SET_TILE_ROWS, DEFAULT_DEPTH, 1)
-    else:
+    elif n < MID_SET_MAX_TILES:
+        first = (MID_SET_TILE_ROWS, DEFAULT_DEPTH, 1)
+    elif max(r.nx * r.ny for r in runs) >= HUGE_RUN_CELLS:
         first = (DEFAULT_TILE_ROWS, DEFAULT_DEPTH, DEFAULT_CPL)
+    else:
+        first = (LARGE_SET_TILE_ROWS, DEFAULT_DEPTH, 1)
     for tile, depth, cpl in (first,) + FALLBACK_SHAPES:
         if all(tb_limits_ok(r, tile, d, cpl) for r in runs for d in range(1, depth + 1)):
             return tile, depth, cpl

@@ -93,7 +93,7 @@ class ImagingServer:
     """Admit, co-schedule and retire imaging requests on one GPU."""
 
     def __init__(self, *, dt: float, dx: float = 1e-3, device=None, hbm_budget: float = 64e9, quantum: int = 192,
-                 depth: int = 8, tile_rows: int = 48, max_requests: Optional[int] = None,
+                 depth: int = 8, tile_rows: int = 64, max_requests: Optional[int] = None,
                  stability_threshold: float = 1e10, v_medium: Optional[float] = None,
                  division: str = "reciprocal") -> None:
         self.device = torch.device(device) if device is not None else torch.device("cuda")
