@@ -123,7 +123,7 @@ def test_receiver_map_folds_records_bit_for_bit(device):
     assert torch.equal(ge1, ge0) and torch.equal(gs1, gs0)
 
 
-@pytest.mark.parametrize("tile,depth", [(32, 6), (48, 8), (48, 3)])
+@pytest.mark.parametrize("tile,depth", [(32, 8), (32, 6), (48, 8), (48, 3)])
 def test_recompute_history_equals_oracle_states(device, tile, depth):
     """The recompute's state-history launches write the state after every one
     of their steps, bit for bit as the fp32 oracle steps from the same start."""

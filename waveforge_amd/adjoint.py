@@ -207,7 +207,10 @@ class TraceProblem:
         table = torch.from_numpy(np.ascontiguousarray(tbl).reshape(-1)).to(self.device)
         return H, table
 
-    recompute_tile_rows = 48  # the state-history launches exist for 32- and 48-row tiles
+    # the state-history launches exist for 32- and 48-row tiles; 32 rows measured
+    # 49 us a launch against 57 us for 48 on 16 runs of 256^2 (768 workgroups,
+    # three a CU, against 576), the history stores bounding both
+    recompute_tile_rows = 32
     recompute_depth = 8       # steps per recompute launch (each step's state is written)
 
     def _tile_list(self, depth: int) -> torch.Tensor:

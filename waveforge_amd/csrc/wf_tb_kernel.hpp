@@ -657,9 +657,10 @@ k_tb(wf_tb_args a) {
 #pragma unroll
             for (int k = 0; k < NR; ++k) {
                 const int gi = gi0 + r0 + k;
-                const bool core = r0 + k >= D && r0 + k < D + TI && gi < nx;
-                const int64_t ro = core ? hoff + (int64_t)gi * ny : 0;
-                const int nb = core ? 4 * ny : 0;
+                // wave-uniform: halo rows issue no history stores at all
+                if (!(r0 + k >= D && r0 + k < D + TI && gi < nx)) continue;
+                const int64_t ro = hoff + (int64_t)gi * ny;
+                const int nb = 4 * ny;
                 const auto rx = row_of(q_ex, ro, nb), ry = row_of(q_ey, ro, nb), rh = row_of(q_hz, ro, nb);
 #pragma unroll
                 for (int h = 0; h < CPL; ++h) {
