@@ -20,6 +20,7 @@ is reported failed without stopping the others.
 
 from __future__ import annotations
 
+import collections
 import heapq
 import math
 import time
@@ -95,7 +96,7 @@ class ImagingServer:
     def __init__(self, *, dt: float, dx: float = 1e-3, device=None, hbm_budget: float = 64e9, quantum: int = 192,
                  depth: int = 8, tile_rows: int = 64, max_requests: Optional[int] = None,
                  stability_threshold: float = 1e10, v_medium: Optional[float] = None,
-                 division: str = "reciprocal") -> None:
+                 division: str = "reciprocal", admit_slice: float = 2e-3) -> None:
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         if self.device.type != "cuda":
             raise RuntimeError("waveforge_amd: the imaging server needs the ROCm GPU; there is no CPU path")
@@ -106,6 +107,7 @@ class ImagingServer:
         self.quantum = quantum
         self.depth, self.tile_rows = depth, tile_rows
         self.max_requests = max_requests
+        self.admit_slice = admit_slice  # host seconds of admissions between two rounds (with requests active)
         self.threshold = stability_threshold
         self.v_medium = v_medium if v_medium is not None else C0 / math.sqrt(40.0)
         self.coef = StepCoefficients.from_grid(dt, dx, dx)
@@ -291,6 +293,7 @@ class ImagingServer:
         now = lambda: time.perf_counter() - t0  # noqa: E731
         k = 0
         last = 0.0
+        ahead: collections.deque = collections.deque()  # an event at the end of each round in flight
         while k < len(pending) or self.active or self.retiring:
             if progress is not None and now() - last >= progress_every:
                 last = now()
@@ -299,12 +302,21 @@ class ImagingServer:
             while k < len(pending) and (pending[k].arrival <= t or not realtime) and self.can_admit(pending[k]):
                 self.admit(pending[k], t)
                 k += 1
+                # a burst is admitted a slice at a time between rounds, so the
+                # GPU starts on the first requests while the host builds the rest
+                if self.active and now() - t >= self.admit_slice:
+                    break
             if self.active:
                 self.advance_round()
                 self.retire_finished()
-                # keep the host at most a couple of rounds ahead of the device
-                if len(self.active) and self.rounds % 2 == 0:
-                    torch.cuda.current_stream(self.device).synchronize()
+                # keep the host at most two rounds ahead of the device without
+                # draining it: wait for the end of the round before the last,
+                # so the next round's launches queue while the last one runs
+                ev = torch.cuda.Event()
+                ev.record()
+                ahead.append(ev)
+                if len(ahead) > 2:
+                    ahead.popleft().synchronize()
             elif self.retiring:
                 torch.cuda.current_stream(self.device).synchronize()
             elif k < len(pending):
@@ -335,7 +347,8 @@ def _stats(done, total: float, rounds: int = 0) -> dict:
 
 def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e-3, device=None,
                 realtime: bool = True, v_medium: Optional[float] = None, n_check: int = 100,
-                stability_threshold: float = 1e10, division: str = "reciprocal") -> dict:
+                stability_threshold: float = 1e10, division: str = "reciprocal",
+                progress: Optional[Callable[[int, float], None]] = None, progress_every: float = 20.0) -> dict:
     """The plain one-run-per-launch GPU path as a server, for comparison.
 
     Requests are taken one at a time in arrival order; every TX of a request
@@ -344,7 +357,9 @@ def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e
     ``n_check`` steps), as a user gets by moving the reference's per-TX
     ``run_mimo`` loop (``examples/brain_mimo_imaging.py:149-204``) onto the
     GPU; then DAS. Fills ``image``, ``traces``, ``finished`` of each request
-    and returns the same statistics as ``ImagingServer.serve``."""
+    and returns the same statistics as ``ImagingServer.serve``;
+    ``progress(done, seconds)`` is called about every ``progress_every``
+    seconds."""
     from .core import DetectorSet, FDTD2D, FieldSet, MurABC, PointSource, SourceCollection, YeeGrid
     from .core.fdtd2d import SimulationDivergedError
 
@@ -354,7 +369,11 @@ def serve_plain(requests: Sequence[ImagingRequest], *, dt: float, dx: float = 1e
     pending = sorted(requests, key=lambda r: r.arrival)
     t0 = time.perf_counter()
     now = lambda: time.perf_counter() - t0  # noqa: E731
-    for req in pending:
+    last = 0.0
+    for k, req in enumerate(pending):
+        if progress is not None and now() - last >= progress_every:
+            last = now()
+            progress(k, last)
         if realtime:
             wait = req.arrival - now()
             if wait > 0:
