@@ -409,11 +409,14 @@ def serving_suite(dev, n_requests: int) -> dict:
       time (what continuous batching adds on its own); the plain
       one-run-per-launch GPU path (each TX its own FDTD2D.run, one request at
       a time).
-    * Sustained: a Poisson stream of ``n_requests`` at 80 % of the continuous
-      server's capacity (p50/p99 latency, arrival to DAS image ready); the
-      plain path on the first 16 arrivals of the same stream (equal arrival
-      rate: it cannot keep up, so its queue and p99 grow), and on 16 arrivals
-      at 80 % of its own capacity (a rate it sustains).
+    * Load curves: Poisson streams of ``n_requests`` at 50, 60, 70, 80 and 90 %
+      of the continuous server's capacity (p50/p99 latency, arrival to DAS
+      image ready), and of 32 requests at 50 and 80 % of the plain path's
+      capacity (a rate it sustains); the plain path also on the first 16
+      arrivals of the continuous 80 % stream (equal arrival rate: it cannot
+      keep up, so its queue and p99 grow).
+    * Throughput at equal p99: for each plain load point, the continuous
+      server's best load point whose p99 is no worse, and the ratio.
     Throughput is compared in cell-updates/s."""
     import torch
 
@@ -452,28 +455,52 @@ def serving_suite(dev, n_requests: int) -> dict:
     ref = {q.rid: q for q in burst_1}
     for q in burst_p:
         same = same and torch.equal(q.image, ref[q.rid].image) and torch.equal(q.traces, ref[q.rid].traces)
-    rate_c = 0.8 * cap_c["images_per_s"]
-    log(f"sustained: {n_requests} Poisson requests at {rate_c:.2f} images/s, continuous")
-    sus_c = server().serve(reqs(n_requests, rate_c, 7), realtime=True)
-    log("plain path: the first 16 arrivals of the same stream (equal arrival rate)")
+    # load curves: each path under Poisson streams at fractions of its own capacity
+    curve_c = {}
+    for frac in (0.5, 0.6, 0.7, 0.8, 0.9):
+        rate = frac * cap_c["images_per_s"]
+        log(f"sustained: {n_requests} Poisson requests at {rate:.2f} images/s ({frac:.0%} of capacity), continuous")
+        curve_c[frac] = dict(server().serve(reqs(n_requests, rate, 7), realtime=True), offered_images_per_s=rate)
+    rate_c = curve_c[0.8]["offered_images_per_s"]
+    sus_c = curve_c[0.8]
+    log("plain path: the first 16 arrivals of the 80 % stream (equal arrival rate)")
     eq_p = serving.serve_plain(reqs(n_requests, rate_c, 7)[:16], dt=dt, device=dev, realtime=True)
-    rate_p = 0.8 * cap_p["images_per_s"]
-    log(f"plain path: 16 Poisson requests at {rate_p:.2f} images/s (80 % of its capacity)")
-    sus_p = serving.serve_plain(reqs(16, rate_p, 7), dt=dt, device=dev, realtime=True)
+    curve_p = {}
+    for frac in (0.5, 0.8):
+        rate = frac * cap_p["images_per_s"]
+        log(f"plain path: 32 Poisson requests at {rate:.2f} images/s ({frac:.0%} of its capacity)")
+        curve_p[frac] = dict(serving.serve_plain(reqs(32, rate, 7), dt=dt, device=dev, realtime=True),
+                             offered_images_per_s=rate)
+    sus_p = curve_p[0.8]
     ratio_sus = sus_c["mcell_updates_per_s"] / sus_p["mcell_updates_per_s"]
+    # throughput at equal p99: for each plain load point, the continuous
+    # server's highest-throughput load point whose p99 is no worse
+    at_p99 = []
+    for fp, p in curve_p.items():
+        ok = [(fc, c) for fc, c in curve_c.items() if c["p99_ms"] <= p["p99_ms"]]
+        row = {"plain_load": fp, "plain_p99_ms": round(p["p99_ms"], 1),
+               "plain_mcell_updates_per_s": round(p["mcell_updates_per_s"], 1)}
+        if ok:
+            fc, c = max(ok, key=lambda x: x[1]["mcell_updates_per_s"])
+            row.update(continuous_load=fc, continuous_p99_ms=round(c["p99_ms"], 1),
+                       continuous_mcell_updates_per_s=round(c["mcell_updates_per_s"], 1),
+                       throughput_ratio=round(c["mcell_updates_per_s"] / p["mcell_updates_per_s"], 2))
+        at_p99.append(row)
     return {
         "workload": f"head phantoms, grid 128..1024 log-uniform, E in {{8,16,32}}, steps = 8*diag, DAS per request; "
                     f"sustained stream of {n_requests} Poisson requests",
         "capacity": {"continuous_48": cap_c, "one_request_at_a_time_48": cap_1, "plain_run_per_launch_48": cap_p},
         "capacity_ratio_vs_plain": round(cap_c["mcell_updates_per_s"] / cap_p["mcell_updates_per_s"], 2),
         "capacity_ratio_vs_one_at_a_time": round(cap_c["mcell_updates_per_s"] / cap_1["mcell_updates_per_s"], 2),
-        "sustained": {"continuous": dict(sus_c, offered_images_per_s=rate_c),
+        "sustained": {"continuous": sus_c,
                       "plain_at_equal_arrival_rate_16": dict(eq_p, offered_images_per_s=rate_c),
-                      "plain_at_80pct_of_own_capacity_16": dict(sus_p, offered_images_per_s=rate_p)},
+                      "plain_at_80pct_of_own_capacity_32": sus_p},
+        "load_curve_continuous": {f"{k:.0%}": v for k, v in curve_c.items()},
+        "load_curve_plain": {f"{k:.0%}": v for k, v in curve_p.items()},
         "sustained_throughput_ratio": round(ratio_sus, 2),
         "p99_continuous_over_plain_at_equal_rate": round(sus_c["p99_ms"] / eq_p["p99_ms"], 3),
         "p99_continuous_over_plain_at_own_rate": round(sus_c["p99_ms"] / sus_p["p99_ms"], 3),
-        "throughput_ratio_at_no_worse_p99": round(ratio_sus, 2) if sus_c["p99_ms"] <= sus_p["p99_ms"] else None,
+        "throughput_at_equal_p99": at_p99,
         "images_equal_across_paths": bool(same),
     }
 
