@@ -49,6 +49,31 @@ def test_mix_equals_alone_and_budget(device):
         assert torch.equal(a.image, mixed[r.rid][1]), f"request {r.rid} image differs alone vs mixed"
 
 
+@pytest.mark.parametrize("fcfs_tiles", [1, 40, None])
+def test_tile_budget_rounds_equal_alone(device, fcfs_tiles):
+    """Rounds that advance only the oldest requests up to a tile budget (1:
+    one request at a time, 40: a few, None: every admitted request) give
+    every request the traces and image it gets alone; with a budget of one
+    tile the requests finish in arrival order."""
+    from waveforge_amd.serving import ImagingServer
+    from waveforge_amd.workloads import imaging_dt
+
+    dt = imaging_dt()
+    srv = ImagingServer(dt=dt, device=device, quantum=48, depth=6, fcfs_tiles=fcfs_tiles)
+    stats = srv.serve(_stream(), realtime=False)
+    assert stats["requests"] == 6 and stats["failed"] == 0
+    if fcfs_tiles == 1:
+        assert [r.rid for r in srv.done] == sorted(r.rid for r in srv.done)
+    mixed = {r.rid: r for r in srv.done}
+    for r in _stream():
+        alone = ImagingServer(dt=dt, device=device, quantum=48, depth=6, max_requests=1)
+        alone.serve([r], realtime=False)
+        a = alone.done[0]
+        print(f"[serving] fcfs_tiles={fcfs_tiles} request {r.rid} grid={r.eps.shape} E={len(r.antennas)}", flush=True)
+        assert torch.equal(a.traces, mixed[r.rid].traces), f"request {r.rid} traces differ"
+        assert torch.equal(a.image, mixed[r.rid].image), f"request {r.rid} image differs"
+
+
 @pytest.mark.parametrize("division", ["reciprocal", "ieee"])
 def test_served_request_matches_oracle(device, division):
     from waveforge_amd.core.sources import RickerWavelet

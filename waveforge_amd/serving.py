@@ -7,11 +7,12 @@ nothing: it images one phantom at a time with a sequential per-TX Python
 loop (``examples/brain_mimo_imaging.py:149-204, 211-266``).
 
 The server keeps every admitted request's field state on the device and
-advances all active runs together, one quantum of steps per scheduling round,
+advances the active runs together, one quantum of steps per scheduling round,
 through the temporally blocked multi-run kernel: requests that arrive while
-others are on the GPU join at the next round once their HBM footprint fits
-the budget (and leave as soon as their runs reach their step count), so work
-of different requests overlaps inside the same launches. Each run reads only
+others are on the GPU are admitted once their HBM footprint fits the budget
+and join the rounds in arrival order, as many as fill a tile budget (and
+leave as soon as their runs reach their step count), so work of different
+requests overlaps inside the same launches. Each run reads only
 its own descriptor and tiles, so a request's result does not depend on what
 else is in flight (the tests check alone == in the mix, bit for bit).
 Stability is judged on the device per run at every round; a diverged request
@@ -21,7 +22,6 @@ is reported failed without stopping the others.
 from __future__ import annotations
 
 import collections
-import heapq
 import math
 import time
 from dataclasses import dataclass, field
@@ -45,6 +45,7 @@ from .engine import (
     runs_add_step,
     tb_fits_any_tile,
     tb_limits_ok,
+    tile_count,
     to_device_async,
 )
 
@@ -96,7 +97,8 @@ class ImagingServer:
     def __init__(self, *, dt: float, dx: float = 1e-3, device=None, hbm_budget: float = 64e9, quantum: int = 192,
                  depth: int = 8, tile_rows: int = 64, max_requests: Optional[int] = None,
                  stability_threshold: float = 1e10, v_medium: Optional[float] = None,
-                 division: str = "reciprocal", admit_slice: float = 2e-3) -> None:
+                 division: str = "reciprocal", admit_slice: float = 2e-3,
+                 fcfs_tiles: Optional[int] = 16384) -> None:
         self.device = torch.device(device) if device is not None else torch.device("cuda")
         if self.device.type != "cuda":
             raise RuntimeError("waveforge_amd: the imaging server needs the ROCm GPU; there is no CPU path")
@@ -108,6 +110,12 @@ class ImagingServer:
         self.depth, self.tile_rows = depth, tile_rows
         self.max_requests = max_requests
         self.admit_slice = admit_slice  # host seconds of admissions between two rounds (with requests active)
+        # each round advances the oldest admitted requests whose runs make at
+        # least this many tiles (enough to fill the GPU many times over), the
+        # rest waiting in HBM; None advances every admitted request (processor
+        # sharing), under which the largest requests share every round with
+        # each newcomer: at 80-90 % load their p99 is about 1.4x as long
+        self.fcfs_tiles = fcfs_tiles
         self.threshold = stability_threshold
         self.v_medium = v_medium if v_medium is not None else C0 / math.sqrt(40.0)
         self.coef = StepCoefficients.from_grid(dt, dx, dx)
@@ -181,7 +189,15 @@ class ImagingServer:
         """Plan over the active runs; their stop flags move to the new plan on
         the device (a gather from the old flags), so no admission or
         retirement waits for the GPU."""
-        runs = [r for req in self.active for r in req._runs]
+        runs = []
+        tiles = 0
+        for req in self.active:  # admission order
+            runs += req._runs
+            if self.fcfs_tiles is not None:
+                nx, ny = req.eps.shape
+                tiles += len(req._runs) * tile_count(nx, ny, self.tile_rows, self.depth)
+                if tiles >= self.fcfs_tiles:
+                    break
         if not runs:
             self._plan = self._stop = None
             self._plan_runs = []
@@ -288,7 +304,9 @@ class ImagingServer:
         the arrivals follow the wall clock from the call; returns latency and
         throughput statistics. ``progress(done, admitted, seconds)`` is
         called about every ``progress_every`` seconds of a long stream."""
-        pending = sorted(requests, key=lambda r: r.arrival)
+        # arrival order; requests that arrive together, the largest first (the
+        # longest jobs of a burst start first and the short ones fill the end)
+        pending = sorted(requests, key=lambda r: (r.arrival, -r.cell_updates))
         t0 = time.perf_counter()
         now = lambda: time.perf_counter() - t0  # noqa: E731
         k = 0
