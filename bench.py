@@ -321,8 +321,9 @@ def das_batch_timing(phantoms, traces, images, dt, dev, reps: int = 10) -> dict:
 def imaging_suite(dev) -> dict:
     """Batched MIMO imaging: 32 seeded head phantoms (8 each of 150^2, 200^2,
     256^2, 384^2), 16 TX x 16 RX, 2000 steps, all 512 runs in one batched
-    launch sequence (HIP-graph replay of 240-step chunks, launch shape from
-    the autotuner), then one DAS image per phantom. Compared with the
+    launch sequence (launch shape from the autotuner), then one DAS image per
+    phantom; timed with HIP-graph replay of 240-step chunks and stepping
+    eagerly, the faster of the two reported as ``batched_s``. Compared with the
     one-run-per-launch GPU paths on the same runs: FDTD2D.run per TX (temporally blocked, one run per launch) and
     FDTD2D.step per TX (three launches per step), each timed on a sample of
     2 runs per grid size and scaled to all 512 runs."""
@@ -341,7 +342,19 @@ def imaging_suite(dev) -> dict:
     images, traces, br = mimo.image_phantoms(phantoms, steps, dt=dt, device=dev, use_graph=True, chunk=240,
                                              tile_rows="auto", depth="auto")
     torch.cuda.synchronize()
-    batched_s = time.perf_counter() - t0
+    graph_s = time.perf_counter() - t0
+    # the same call stepping eagerly (each launch enqueued from the host): the
+    # launches are long enough here that the host keeps ahead, and the graph's
+    # capture costs more than it saves
+    mimo.image_phantoms(phantoms, 480, dt=dt, device=dev, tile_rows="auto", depth="auto")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    images_e, _, _ = mimo.image_phantoms(phantoms, steps, dt=dt, device=dev, tile_rows="auto", depth="auto")
+    torch.cuda.synchronize()
+    eager_s = time.perf_counter() - t0
+    graph_equals_eager = all(torch.equal(a, b) for a, b in zip(images, images_e))
+    del images_e
+    batched_s = min(graph_s, eager_s)
     cu = br.n_cells * steps
     st = br.status()
     # one-run-per-launch GPU paths on a sample, and alone == in-batch on the sampled runs
@@ -383,6 +396,9 @@ def imaging_suite(dev) -> dict:
         "workload": "32 head phantoms (8 x {150,200,256,384}^2), 16 TX x 16 RX, 2000 steps, 512 runs + 32 DAS images",
         "cell_updates": cu,
         "batched_s": round(batched_s, 4),
+        "batched_graph_s": round(graph_s, 4),
+        "batched_eager_s": round(eager_s, 4),
+        "graph_equals_eager": bool(graph_equals_eager),
         "batched_mcell_updates_per_s": round(cu / batched_s / 1e6, 1),
         "images_per_s": round(32 / batched_s, 2),
         "per_image_latency_ms": round(batched_s * 1e3, 1),

@@ -230,3 +230,21 @@ def test_default_launch_shape_follows_the_batch_size(device):
     big.advance(10)
     small.advance(10)
     assert big.steps_completed == small.steps_completed == 10
+
+
+@pytest.mark.parametrize("comp,i,j,amp", [("Hz", 3, 5, 1.0), ("Ex", 0, 0, -0.37), ("Ey", 19, 11, 2.5e-3),
+                                          ("Ez", 4, 4, 1.0)])
+def test_single_cell_source_table_equals_general(device, comp, i, j, amp):
+    """``SourceTable.cell`` (the direct build batched drivers use) holds the
+    same records as the general constructor over one ``CellSource``; a
+    component the 2D kernels do not update goes to the general path (inert)."""
+    from waveforge_amd.engine import CellSource, SourceTable
+
+    wave = torch.linspace(-1.0, 1.0, 37, device=device)
+    a = SourceTable.cell(comp, wave, i, j, amp, 20, 12)
+    b = SourceTable([CellSource(comp, wave, i, j, amp)], 20, 12, device, upload=False)
+    assert (a.n_inj, a.n_grp, a.min_wave_len, len(a.inert)) == (b.n_inj, b.n_grp, b.min_wave_len, len(b.inert))
+    if b.n_grp:
+        assert a.host_inj.tobytes() == b.host_inj.tobytes()
+        assert np.array_equal(a.host_grp, b.host_grp) and np.array_equal(a.group_cells, b.group_cells)
+

@@ -229,6 +229,31 @@ class SourceTable:
             self.inj = None
             self.grp_start = None
 
+    @classmethod
+    def cell(cls, component: str, wave: torch.Tensor, i: int, j: int, amp: float, Nx: int, Ny: int) -> "SourceTable":
+        """The table of one soft source at one cell of a 2D TM grid, built
+        directly (no source object, no sort): the same records as
+        ``SourceTable([CellSource(component, wave, i, j, amp)], Nx, Ny, device,
+        upload=False)``, for drivers that build hundreds of runs. Upload with
+        ``upload_many``."""
+        if component not in _native.WF_COMP:
+            return cls([CellSource(component, wave, i, j, amp)], Nx, Ny, wave.device, upload=False)
+        require_gpu_tensor(wave, "source waveform")
+        wave = wave.contiguous()
+        t = cls.__new__(cls)
+        rec = np.zeros(1, dtype=INJ_DTYPE)
+        rec["wave"] = wave.data_ptr()
+        rec["wave_len"] = wave.numel()
+        rec["comp"] = _native.WF_COMP[component]
+        rec["cell"] = int(i) * int(Ny) + int(j)
+        rec["amp"] = np.float32(amp)
+        t.inert, t.keepalive, t.min_wave_len = [], [wave], wave.numel()
+        t.n_inj = t.n_grp = 1
+        t.group_cells = rec["cell"].copy()
+        t.host_inj, t.host_grp = rec, np.array([0, 1], dtype=np.int32)
+        t.inj = t.grp_start = None
+        return t
+
     @staticmethod
     def upload_many(tables: Sequence["SourceTable"], device: torch.device) -> None:
         """Upload the host records of tables built with ``upload=False`` in one

@@ -109,8 +109,12 @@ class BatchedRuns:
             n = cells[k]
             f = self._state[o_s:o_s + 6 * n].view(6, sp.nx, sp.ny)
             o_s += 6 * n
-            srcs = [CellSource(c, w, i, j, a) for (c, i, j, w, a) in sp.sources]
-            table = SourceTable(srcs, sp.nx, sp.ny, dev, upload=False)
+            if len(sp.sources) == 1:  # the common case, built directly
+                c, i, j, w, a = sp.sources[0]
+                table = SourceTable.cell(c, w, i, j, a, sp.nx, sp.ny)
+            else:
+                srcs = [CellSource(c, w, i, j, a) for (c, i, j, w, a) in sp.sources]
+                table = SourceTable(srcs, sp.nx, sp.ny, dev, upload=False)
             tables.append(table)
             rs = RunState(f[0], f[1], f[2], sp.ca, sp.cb, sp.nx, sp.ny, coef, sources=table,
                           reciprocal=self.division == "reciprocal")

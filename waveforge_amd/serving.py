@@ -35,7 +35,6 @@ from .core.imaging import DasBatch, run_das
 from .core.materials import coefficients_from_maps
 from .core.sources import RickerWavelet
 from .engine import (
-    CellSource,
     RunState,
     SourceTable,
     StepCoefficients,
@@ -162,7 +161,7 @@ class ImagingServer:
         state = torch.zeros((n_ant, 6, nx, ny), dtype=torch.float32, device=dev)  # both states of every TX run
         for t, (i, j) in enumerate(req.antennas):
             f = state[t]
-            table = SourceTable([CellSource("Hz", wave, i, j, 1.0)], nx, ny, dev, upload=False)
+            table = SourceTable.cell("Hz", wave, i, j, 1.0, nx, ny)
             tables.append(table)
             rs = RunState(f[0], f[1], f[2], ca, cb, nx, ny, self.coef, sources=table, det_cell=cells,
                           traces=req.traces[t], trace_n0=0, step=0, step_end=req.n_steps,
