@@ -20,10 +20,10 @@
 // the lane's own registers, except across the boundary between two waves'
 // row blocks, where one row per wave goes through LDS each half-step (the Ey
 // of its first row for the wave above, the Hz of its last row for the wave
-// below). Interior tiles step every row of the region without row tests (a
-// row outside the valid trapezoid only feeds rows outside it); edge tiles
-// keep a copy of every Hz row in LDS for the Mur pass and step rows under
-// per-step row masks.
+// below). Every tile steps every row of the region without row tests (a
+// row outside the valid trapezoid or the grid only feeds rows outside it, or
+// the face cells the Mur pass and the kept Ex/Ey columns overwrite); edge
+// tiles also keep a copy of every Hz row in LDS for the Mur pass.
 //
 // Each step reproduces the reference semantics exactly, in the same fp32
 // operation order (the library is built with -ffp-contract=off):
@@ -353,11 +353,9 @@ k_tb(wf_tb_args a) {
     const int ngrp = min(s_ngrp, kMaxGroups);
     const int ndet = min(s_ndet, kMaxDets);
     const unsigned int rowmask = s_rowmask[w];
-    // Edge tiles (touching a grid face) step every row under the row masks and
-    // keep every Hz row in LDS, which the Mur pass reads and writes; interior
-    // tiles step every row of the region unmasked (a row outside the valid
-    // trapezoid only feeds rows outside it) and keep Hz rows in LDS only when
-    // they record detectors.
+    // Edge tiles (touching a grid face) keep every Hz row in LDS, which the
+    // Mur pass reads and writes; interior tiles keep Hz rows in LDS only when
+    // they record detectors. Both step every row of the region.
     const bool edge = (gi0 <= 0) || (gi0 + RI >= nx) || (gj0 <= 0) || (gj0 + W >= ny);
     const bool mirror = edge || ndet > 0;
     const float dh = R.dh, de = R.de, cx = R.cmx, cy = R.cmy;
