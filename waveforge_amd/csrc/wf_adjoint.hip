@@ -589,7 +589,12 @@ extern "C" int wf_adjoint_step(const wf_adj_args *args, hipStream_t stream) {
         // steps share a launch when the receivers' records of the steps after
         // the first can be folded in (no receivers, or the receiver map).
         const dim3 ge((a.ny + kEhCols - 1) / kEhCols, (a.nx + kEhRows - 1) / kEhRows, a.n_runs);
-        constexpr int kRows2 = 16;
+        // 8-row tiles (a 16-row region, two rows per wavefront, 68 VGPRs, 7
+        // waves per SIMD): the step is latency-bound, and the extra ring work
+        // of the smaller tile costs less than the residency it buys (73 ms
+        // against 79 ms for the 256^2 x 16-run, 3000-step gradient on 16-row
+        // tiles at 5 waves per SIMD)
+        constexpr int kRows2 = 8;
         const dim3 ge2((a.ny + kE2Cols - 1) / kE2Cols, (a.nx + kRows2 - 1) / kRows2, a.n_runs);
         const bool fast = (a.flags & WF_ADJ_FAST_DIV) != 0;
         const AdjState pri{a.lex, a.ley, a.lhz, a.lex2, a.ley2, a.lhz2};
