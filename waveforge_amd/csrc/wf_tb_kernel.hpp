@@ -454,10 +454,6 @@ k_tb(wf_tb_args a) {
         __syncthreads();
         edge_snapshot();
     }
-    // the steps issue ahead of other workgroups' load and store phases on the
-    // same SIMD (those need few issue slots, and their memory latency hides
-    // behind the steps): +2 % on sets of small grids, neutral at 4096^2
-    __builtin_amdgcn_s_setprio(2);
 #pragma unroll 1
     for (int s = 1; s <= D; ++s) {
         const int n = n0 + s - 1;
@@ -673,7 +669,6 @@ k_tb(wf_tb_args a) {
             }
         }
     }
-    __builtin_amdgcn_s_setprio(0);
     if (HIST) return;  // the last step's state is in its history slot
 
     // Write the core back (and fold the stability maxima if requested).
