@@ -286,6 +286,11 @@ k_tb(wf_tb_args a) {
     uint32_t coff[CPL];  // byte offset of the lane's cells in a row, kOut outside the grid's columns
 #pragma unroll
     for (int h = 0; h < CPL; ++h) coff[h] = col_in[h] ? 4u * (uint32_t)(gj0 + c0 + h) : kOut;
+    // the tile's row of the per-tile lists, in flight with the field loads
+    // (read after the barrier below, it would add a load's latency to every
+    // workgroup before its first step)
+    int4 tl = make_int4(0, R.n_grp, 0, R.n_det);
+    if (a.tile_lists) tl = reinterpret_cast<const int4 *>(a.tile_lists)[t];
     const float *p_ex = uniform_ptr(src_ex), *p_ey = uniform_ptr(src_ey), *p_hz = uniform_ptr(src_hz);
     const float *p_ca = MAT ? uniform_ptr(R.ca) : nullptr, *p_cb = MAT ? uniform_ptr(R.cb) : nullptr;
     float ex[NR][CPL], ey[NR][CPL], hz[NR][CPL], ca[NR][CPL], cb[NR][CPL];
@@ -313,13 +318,10 @@ k_tb(wf_tb_args a) {
         }
     }
     __syncthreads();
-    // this tile's source groups and detectors: from the host's per-tile lists,
-    // or by testing every group and detector of the run
-    int g_lo = 0, g_hi = R.n_grp, d_lo = 0, d_hi = R.n_det;
-    if (a.tile_lists) {
-        const int4 tl = reinterpret_cast<const int4 *>(a.tile_lists)[t];
-        g_lo = tl.x, g_hi = tl.y, d_lo = tl.z, d_hi = tl.w;
-    }
+    // this tile's source groups and detectors: from the host's per-tile lists
+    // (their row read with the fields, above), or by testing every group and
+    // detector of the run
+    const int g_lo = tl.x, g_hi = tl.y, d_lo = tl.z, d_hi = tl.w;
     for (int q = g_lo + tid; q < g_hi; q += kNT) {
         const int g = a.tile_lists ? a.grp_list[q] : q;
         const wf_inj2d e0 = R.inj[R.grp_start[g]];
